@@ -1,0 +1,175 @@
+/*
+ * mxr.h -- C ABI of libmxr.so, the MI355X (gfx950) radar signal-processing
+ * library behind the MatX-compatible operator front end (include/matx_amd.h).
+ *
+ * Conventions
+ *   - All tensor memory is caller-owned device memory (hipMalloc / managed).
+ *     The library never frees a caller pointer.  It owns only opaque plans,
+ *     twiddle tables and waveform spectra, released by mxr_clear_caches().
+ *   - Every call is asynchronous on the caller's hipStream_t; nothing
+ *     synchronises the device.  Plans are created (and their device buffers
+ *     allocated) on first use, so warm a shape once before hipGraph capture.
+ *   - Every entry returns an mxr_status (0 = success).  On failure the
+ *     thread-local text from mxr_last_error() says what was wrong.  Shape and
+ *     size errors are detected on the host before any launch, as MatX throws
+ *     matxException before launching (include/matx/core/error.h:55-74).
+ *   - Re-entrant from several host threads; plan lookup is mutex-guarded
+ *     (cf. include/matx/core/cache.h:221-257).
+ *
+ * Citations are to the MatX sources (paths relative to the MatX repository
+ * root) whose behaviour each entry point replaces.
+ */
+#ifndef MXR_H_
+#define MXR_H_
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MXR_MAX_RANK 8
+
+/* Status codes mirror matxError_t (include/matx/core/error.h:55-74). */
+typedef enum mxr_status {
+  MXR_SUCCESS = 0,
+  MXR_NOT_SUPPORTED = 1,
+  MXR_INVALID_PARAMETER = 2,
+  MXR_INVALID_DIM = 3,
+  MXR_INVALID_SIZE = 4,
+  MXR_OUT_OF_MEMORY = 5,
+  MXR_INVALID_TYPE = 6,
+  MXR_HIP_ERROR = 7,
+  MXR_ALIAS_ERROR = 8
+} mxr_status;
+
+typedef enum mxr_dtype {
+  MXR_C64 = 1, /* complex<float>, interleaved re,im */
+  MXR_F32 = 2,
+  MXR_I32 = 3
+} mxr_dtype;
+
+/* A strided view: element (i0,...,i{rank-1}) lives at
+ * data + sum_d i_d*stride[d] elements (cf. tensor_impl.h:91-160). */
+typedef struct mxr_tensor {
+  void *data;
+  int32_t dtype;
+  int32_t rank;
+  int64_t shape[MXR_MAX_RANK];
+  int64_t stride[MXR_MAX_RANK];
+} mxr_tensor;
+
+/* Convolution modes (include/matx/core/operator_options.h:52-60). */
+typedef enum mxr_conv_mode { MXR_CONV_FULL = 0, MXR_CONV_SAME = 1, MXR_CONV_VALID = 2 } mxr_conv_mode;
+
+/* FFT normalisation (FFTNorm, include/matx/core/operator_options.h:45-49). */
+typedef enum mxr_fft_norm { MXR_NORM_BACKWARD = 0, MXR_NORM_FORWARD = 1, MXR_NORM_ORTHO = 2 } mxr_fft_norm;
+typedef enum mxr_fft_dir { MXR_FFT_FORWARD = 0, MXR_FFT_INVERSE = 1 } mxr_fft_dir;
+
+/* ------------------------------------------------------------------ misc */
+const char *mxr_last_error(void);
+int mxr_version(void);
+/* Frees every cached plan, twiddle table and waveform spectrum
+ * (cf. ClearCachesAndAllocations, test/main.cu). */
+mxr_status mxr_clear_caches(void);
+
+/* ------------------------------------------------------------------ FFT */
+/* Batched 1-D complex-to-complex FFT along `axis`.
+ * n = transform size; 0 means in->shape[axis].  The input is zero-padded or
+ * truncated to n, and out->shape[axis] must equal n; every other dimension of
+ * in and out must match (batch dims).  Any strides; in == out allowed.
+ * Power-of-two n from 1 to 16384.
+ * Replaces fft_impl / ifft_impl (include/matx/transforms/fft/fft_cuda.h:680-774,
+ * size rules include/matx/operators/fft.h:111-199 and fft_common.h:39-140,
+ * normalisation fft_cuda.h:126-188). */
+mxr_status mxr_fft_c2c(const mxr_tensor *out, const mxr_tensor *in, int64_t n,
+                       int dir, int norm, int axis, hipStream_t stream);
+
+/* fft_conv: out = ifft(fft(x, n) .* conj(fft(w, n)))  (conj_w = 1, circular
+ * cross-correlation, the BASELINE operator) or ifft(fft(x,n) .* fft(w,n))
+ * (conj_w = 0, circular convolution), along the last axis of x.
+ * x: [..., S] c64; w: [L] c64; out: [..., nout] c64 with nout <= n, holding
+ * lags 0..nout-1.  There is no MatX operator of this name; it is the idiom of
+ * examples/fft_conv.cu:90-108 and the conj-multiply of
+ * examples/simple_radar_pipeline.h:295-300. */
+mxr_status mxr_fft_conv(const mxr_tensor *out, const mxr_tensor *x,
+                        const mxr_tensor *w, int64_t n, int conj_w,
+                        hipStream_t stream);
+
+/* Pulse compression stage, examples/simple_radar_pipeline.h:273-301:
+ *   w1 = w .* hamming(L) (if MXR_PC_WINDOW); w2 = w1/||w1|| (if
+ *   MXR_PC_NORMALIZE); y[..., r] = (1/n) ifft(fft(x, n) .* conj(fft(w2, n)))[r]
+ *   for r < y->shape[last].
+ * x: [..., S] c64 (last axis unit stride); y: [..., nout] c64, nout <= n.
+ * y->data == x->data is allowed (in place, as MatX transforms inputView in
+ * place); then each row is finished before any of it is overwritten.
+ * MXR_PC_WRITEBACK stores w2 back into w, reproducing MatX's in-place
+ * re-windowing of waveformView on every call (:288-294). */
+#define MXR_PC_WINDOW 1
+#define MXR_PC_NORMALIZE 2
+#define MXR_PC_WRITEBACK 4
+mxr_status mxr_pulse_compress(const mxr_tensor *y, const mxr_tensor *x,
+                              const mxr_tensor *w, int64_t n, int flags,
+                              hipStream_t stream);
+
+/* ------------------------------------------------------------ pulse axis */
+/* tpc_mode value for mxr_doppler: y already holds the canceller output; all
+ * of its rows feed the FFT (MatX's DopplerProcessing stage on tpcView,
+ * simple_radar_pipeline.h:344-357, window on rows < win_rows). */
+#define MXR_TPC_NONE 3
+
+/* Three-pulse canceller + Doppler window + Doppler FFT (+ |.|^2), fused.
+ * y: [C, P, R] c64 (R axis unit stride).  tpc_mode MXR_CONV_SAME reproduces
+ * MatX's stage dataflow (simple_radar_pipeline.h:321-357: SAME [1,-2,1] over
+ * all P pulses, Hamming(win_rows) on rows < win_rows, the remaining rows
+ * unwindowed, zero rows up to nd, nd-point FFT along pulses);
+ * MXR_CONV_VALID the numpy model (01_radar.py:65-78: rows 1..P-2).
+ * rd_out: [C, nd, R] c64 range-Doppler map (may be NULL); it may be exactly
+ * y (same data, shape and strides: the in-place stage form), never a partial
+ * overlap.  pow_out: [C, nd, R] f32 |rd|^2 (may be NULL).  At least one
+ * output is required. */
+mxr_status mxr_doppler(const mxr_tensor *pow_out, const mxr_tensor *rd_out,
+                       const mxr_tensor *y, int tpc_mode, int64_t win_rows,
+                       int64_t nd, hipStream_t stream);
+
+/* ------------------------------------------------------------------ CFAR */
+/* Cell-averaging CFAR with a rectangular training window of win_y x win_x
+ * cells minus a centred hole of guard_y x guard_x cells (CUT + guards), zero
+ * padded at the edges, Pfa-derived threshold:
+ *   sum = masked box sum; norm = in-bounds masked cell count;
+ *   ba = sum/norm; alpha = norm*(pfa^(-1/norm) - 1); det = pow > alpha*ba.
+ * MatX's mask (simple_radar_pipeline.h:237-241) is win 5x13, guard 3x3.
+ * Replaces conv2d(xPow, mask, FULL) + trim + divide + calcDets
+ * (simple_radar_pipeline.h:394-436, op body :86-98).
+ * in: [C, Ny, Nx] f32 power, or c64 (|.|^2 taken on load, = abs2(tpcView)).
+ * dets: [C, Ny, Nx] i32.  ba_out: [C, Ny, Nx] f32 or NULL (any strides, e.g.
+ * MatX's trimmed slice of the padded ba tensor).
+ * pow_out: [C, Ny, Nx] f32 or NULL (writes |in|^2 when in is c64). */
+typedef struct mxr_cfar_params {
+  int32_t win_y, win_x;     /* odd */
+  int32_t guard_y, guard_x; /* odd, <= win */
+  float pfa;
+} mxr_cfar_params;
+mxr_status mxr_cfar(const mxr_tensor *dets, const mxr_tensor *ba_out,
+                    const mxr_tensor *pow_out, const mxr_tensor *in,
+                    const mxr_cfar_params *params, hipStream_t stream);
+
+/* --------------------------------------------------------- convolution */
+/* Direct 1-D convolution along the last axis, batched over the leading
+ * dims (filter rank 1 is broadcast, as conv1d_impl clones it,
+ * include/matx/transforms/conv.h:467-532).  in/out c64 or f32, filter f32 or
+ * c64; output length by mode (FULL n+L-1, SAME n, VALID n-L+1), SAME start
+ * as include/matx/kernels/conv.cuh:155-168.  Any strides. */
+mxr_status mxr_conv1d(const mxr_tensor *out, const mxr_tensor *in,
+                      const mxr_tensor *filt, int mode, hipStream_t stream);
+
+/* Direct 2-D convolution over the last two axes, batched over leading dims
+ * (include/matx/transforms/conv.h:547-595; index rule :340-380). */
+mxr_status mxr_conv2d(const mxr_tensor *out, const mxr_tensor *in,
+                      const mxr_tensor *filt, int mode, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MXR_H_ */

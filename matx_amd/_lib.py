@@ -1,0 +1,125 @@
+"""ctypes binding of libmxr.so (the C ABI declared in include/mxr.h).
+
+The product path has no fallback: if the HIP library is not built, or a call
+fails, this module raises.  Tensors are torch device tensors; descriptors carry
+their data pointer, shape and element strides (the mxr_tensor struct).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmxr.so")
+
+MXR_MAX_RANK = 8
+C64, F32, I32 = 1, 2, 3
+FULL, SAME, VALID = 0, 1, 2
+TPC_NONE = 3
+NORM_BACKWARD, NORM_FORWARD, NORM_ORTHO = 0, 1, 2
+FORWARD, INVERSE = 0, 1
+PC_WINDOW, PC_NORMALIZE, PC_WRITEBACK = 1, 2, 4
+
+# Every entry point include/mxr.h declares.
+EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_fft_c2c",
+           "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar",
+           "mxr_conv1d", "mxr_conv2d")
+
+STATUS_NAMES = {0: "SUCCESS", 1: "NOT_SUPPORTED", 2: "INVALID_PARAMETER",
+                3: "INVALID_DIM", 4: "INVALID_SIZE", 5: "OUT_OF_MEMORY",
+                6: "INVALID_TYPE", 7: "HIP_ERROR", 8: "ALIAS_ERROR"}
+
+
+class MatxError(RuntimeError):
+    """Raised for a non-zero mxr_status (mirrors matxException)."""
+
+    def __init__(self, status: int, msg: str):
+        self.status = status
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+
+
+class mxr_tensor(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("dtype", ctypes.c_int32),
+                ("rank", ctypes.c_int32),
+                ("shape", ctypes.c_int64 * MXR_MAX_RANK),
+                ("stride", ctypes.c_int64 * MXR_MAX_RANK)]
+
+
+class mxr_cfar_params(ctypes.Structure):
+    _fields_ = [("win_y", ctypes.c_int32), ("win_x", ctypes.c_int32),
+                ("guard_y", ctypes.c_int32), ("guard_x", ctypes.c_int32),
+                ("pfa", ctypes.c_float)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libmxr.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build the HIP library with "
+            "`python -m matx_amd.build` (there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    T = ctypes.POINTER(mxr_tensor)
+    V = ctypes.c_void_p
+    I64 = ctypes.c_int64
+    I = ctypes.c_int
+    L.mxr_last_error.restype = ctypes.c_char_p
+    L.mxr_last_error.argtypes = []
+    L.mxr_version.restype = I
+    L.mxr_clear_caches.restype = I
+    sigs = {
+        "mxr_fft_c2c": [T, T, I64, I, I, I, V],
+        "mxr_fft_conv": [T, T, T, I64, I, V],
+        "mxr_pulse_compress": [T, T, T, I64, I, V],
+        "mxr_doppler": [T, T, T, I, I64, I64, V],
+        "mxr_cfar": [T, T, T, T, ctypes.POINTER(mxr_cfar_params), V],
+        "mxr_conv1d": [T, T, T, I, V],
+        "mxr_conv2d": [T, T, T, I, V],
+    }
+    for name, argt in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = argt
+        fn.restype = I
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != 0:
+        msg = lib().mxr_last_error().decode(errors="replace")
+        raise MatxError(status, msg)
+
+
+def desc(t):
+    """mxr_tensor for a torch tensor (or None -> NULL pointer)."""
+    if t is None:
+        return None
+    import torch
+    dt = {torch.complex64: C64, torch.float32: F32, torch.int32: I32}.get(t.dtype)
+    if dt is None:
+        raise MatxError(6, f"unsupported dtype {t.dtype}")
+    if t.dim() > MXR_MAX_RANK:
+        raise MatxError(3, f"rank {t.dim()} > {MXR_MAX_RANK}")
+    d = mxr_tensor()
+    d.data = t.data_ptr()
+    d.dtype = dt
+    d.rank = t.dim()
+    for i, (s, st) in enumerate(zip(t.shape, t.stride())):
+        d.shape[i] = s
+        d.stride[i] = st
+    return d
+
+
+def ref(d):
+    return ctypes.byref(d) if d is not None else None
+
+
+def stream_handle(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)

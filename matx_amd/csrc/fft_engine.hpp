@@ -1,0 +1,325 @@
+// Workgroup FFT engine for gfx950: radix-2/4/8/16 Stockham autosort passes
+// with butterflies held in VGPRs and the data exchanged through LDS between
+// passes.
+//
+// Stockham DIT pass with radix R over a length-N sequence, with sub-transform
+// length Ls done so far (Ls = product of earlier radices):
+//   butterfly j in [0, N/R), k = j mod Ls
+//   inputs  a_m = X[j + m*N/R]                     m = 0..R-1
+//   a_m *= W_{Ls*R}^{m*k}                           (W_n = e^{-2 pi i/n} forward)
+//   b = DFT_R(a)
+//   Y[(j-k)*R + q*Ls + k] = b_q                     q = 0..R-1
+// After the last pass Y is in natural order.  Twiddles come from tables built
+// on the host in double precision and rounded once to float (one table per
+// (Ls, R) pass, k-major with stride R so a thread reads its R entries as one
+// contiguous run).
+#pragma once
+
+#include "mxr_internal.hpp"
+
+namespace mxr {
+
+// ------------------------------------------------- in-register small DFTs
+// Forward W_n^e = exp(-2 pi i e / n); inverse uses the conjugate.
+template <bool INV>
+__device__ __forceinline__ float2 mul_w4(float2 a) {  // * W_4^1 = -i (fwd) / +i (inv)
+  return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft2(float2 &a0, float2 &a1) {
+  float2 t = a0;
+  a0 = cadd(t, a1);
+  a1 = csub(t, a1);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 &a0, float2 &a1, float2 &a2, float2 &a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  float2 t2 = cadd(a1, a3), t3 = mul_w4<INV>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+// Multiply by W_16^e (compile-time e in [0,16)), trivial factors special-cased.
+template <int E, bool INV>
+__device__ __forceinline__ float2 mul_w16(float2 a) {
+  constexpr int e = E & 15;
+  constexpr float C1 = 0.92387953251128675613f;  // cos(pi/8)
+  constexpr float S1 = 0.38268343236508977173f;  // sin(pi/8)
+  constexpr float H = 0.70710678118654752440f;   // sqrt(1/2)
+  if constexpr (e == 0) return a;
+  if constexpr (e == 4) return mul_w4<INV>(a);
+  if constexpr (e == 8) return make_float2(-a.x, -a.y);
+  if constexpr (e == 12) return mul_w4<!INV>(a);
+  // general: angle = -2 pi e/16 (fwd); cos/sin by symmetry from pi/8 multiples
+  constexpr float cs[16] = {1.f, C1, H, S1, 0.f, -S1, -H, -C1, -1.f, -C1, -H, -S1, 0.f, S1, H, C1};
+  constexpr float sn[16] = {0.f, S1, H, C1, 1.f, C1, H, S1, 0.f, -S1, -H, -C1, -1.f, -C1, -H, -S1};
+  // W = cos(2pi e/16) - i sin(2pi e/16) forward
+  constexpr float wr = cs[e];
+  constexpr float wi_f = -sn[e];
+  const float wi = INV ? -wi_f : wi_f;
+  if constexpr (e == 2 || e == 6 || e == 10 || e == 14) {
+    // (x + iy)(wr + i wi) with |wr| = |wi| = H
+    float sr = a.x * wr - a.y * wi;
+    float si = a.x * wi + a.y * wr;
+    return make_float2(sr, si);
+  }
+  return make_float2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft8(float2 (&a)[8]) {
+  // 8 = 2 x 4: a[m1 + 2 m2]; inner DFT4 over m2, twiddle W8^{m1 c}, DFT2 over m1.
+  dft4<INV>(a[0], a[2], a[4], a[6]);
+  dft4<INV>(a[1], a[3], a[5], a[7]);
+  a[3] = mul_w16<2, INV>(a[3]);
+  a[5] = mul_w16<4, INV>(a[5]);
+  a[7] = mul_w16<6, INV>(a[7]);
+  float2 y[8];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float2 u = a[2 * c], v = a[2 * c + 1];
+    y[c] = cadd(u, v);
+    y[c + 4] = csub(u, v);
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = y[i];
+}
+
+template <bool INV>
+__device__ __forceinline__ void dft16(float2 (&a)[16]) {
+  // 16 = 4 x 4: inner DFT4 over m2 (stride 4), twiddle W16^{m1 c}, DFT4 over m1.
+#pragma unroll
+  for (int m1 = 0; m1 < 4; ++m1) dft4<INV>(a[m1], a[m1 + 4], a[m1 + 8], a[m1 + 12]);
+  // Z[m1][c] sits in a[m1 + 4c]
+  a[5] = mul_w16<1, INV>(a[5]);
+  a[9] = mul_w16<2, INV>(a[9]);
+  a[13] = mul_w16<3, INV>(a[13]);
+  a[6] = mul_w16<2, INV>(a[6]);
+  a[10] = mul_w16<4, INV>(a[10]);
+  a[14] = mul_w16<6, INV>(a[14]);
+  a[7] = mul_w16<3, INV>(a[7]);
+  a[11] = mul_w16<6, INV>(a[11]);
+  a[15] = mul_w16<9, INV>(a[15]);
+  float2 y[16];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float2 z0 = a[4 * c + 0], z1 = a[4 * c + 1], z2 = a[4 * c + 2], z3 = a[4 * c + 3];
+    dft4<INV>(z0, z1, z2, z3);
+    y[c] = z0;
+    y[c + 4] = z1;
+    y[c + 8] = z2;
+    y[c + 12] = z3;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) a[i] = y[i];
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2 (&a)[R]) {
+  if constexpr (R == 2) dft2<INV>(a[0], a[1]);
+  else if constexpr (R == 4) dft4<INV>(a[0], a[1], a[2], a[3]);
+  else if constexpr (R == 8) dft8<INV>(a);
+  else if constexpr (R == 16) dft16<INV>(a);
+  else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "radix");
+}
+
+// --------------------------------------------------------------- plans
+// Radix sequence per power-of-two size (forward order).  The inverse of a
+// fused forward->inverse pair runs the reversed sequence, so its first pass
+// has the forward's last radix and can reuse the registers directly.
+struct RadixSeq {
+  int n;
+  int r[6];
+};
+__host__ __device__ constexpr RadixSeq radix_seq(int N) {
+  switch (N) {
+    case 1: return {0, {0}};
+    case 2: return {1, {2}};
+    case 4: return {1, {4}};
+    case 8: return {1, {8}};
+    case 16: return {1, {16}};
+    case 32: return {2, {8, 4}};
+    case 64: return {2, {16, 4}};
+    case 128: return {2, {16, 8}};
+    case 256: return {2, {16, 16}};
+    case 512: return {3, {16, 8, 4}};
+    case 1024: return {3, {16, 16, 4}};
+    case 2048: return {3, {16, 16, 8}};
+    case 4096: return {3, {16, 16, 16}};
+    case 8192: return {4, {16, 16, 8, 4}};
+    case 16384: return {4, {16, 16, 16, 4}};
+    default: return {-1, {0}};
+  }
+}
+__host__ __device__ constexpr RadixSeq reversed(RadixSeq s) {
+  RadixSeq o{s.n, {0}};
+  for (int i = 0; i < s.n; ++i) o.r[i] = s.r[s.n - 1 - i];
+  return o;
+}
+// Offset (in float2) of pass p's twiddle table inside a sequence's table.
+__host__ __device__ constexpr int64_t tw_offset(RadixSeq s, int p) {
+  int64_t off = 0, Ls = 1;
+  for (int i = 0; i < p; ++i) {
+    off += (Ls > 1) ? Ls * s.r[i] : 0;
+    Ls *= s.r[i];
+  }
+  return off;
+}
+__host__ __device__ constexpr int64_t tw_size(RadixSeq s) { return tw_offset(s, s.n); }
+__host__ __device__ constexpr int pass_ls(RadixSeq s, int p) {
+  int Ls = 1;
+  for (int i = 0; i < p; ++i) Ls *= s.r[i];
+  return Ls;
+}
+
+// LDS padding: one spare float2 every 16, which makes the stride-R writes of
+// the early passes bank-conflict free for R in {4, 8, 16}.
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lds_padded_len(int n) { return n + (n >> 4) + 1; }
+
+// --------------------------------------------------------- pass helpers
+template <int R, bool INV>
+__device__ __forceinline__ void apply_twiddles(float2 (&a)[R], const float2 *__restrict__ tw, int k) {
+  // tw points at the pass table; entries k*R + m, m = 1..R-1 (entry 0 holds
+  // 1 and is skipped).  Two twiddles per 16-byte load.
+  const float4 *t4 = reinterpret_cast<const float4 *>(tw + (int64_t)k * R);
+#pragma unroll
+  for (int h = 0; h < R / 2; ++h) {
+    float4 w = t4[h];
+    if (h > 0) a[2 * h] = cmul(a[2 * h], make_float2(w.x, INV ? -w.y : w.y));
+    a[2 * h + 1] = cmul(a[2 * h + 1], make_float2(w.z, INV ? -w.w : w.w));
+  }
+}
+
+// Host: build the twiddle table of a radix sequence for length N in double
+// precision, rounded once to float.  Layout per pass p (Ls > 1):
+// entry k*R + m = exp(-2 pi i m k / (Ls R)).
+void build_twiddles(int N, RadixSeq s, float2 *out);
+
+// Per-device cache of twiddle tables keyed by (N, reversed?).  Returns a
+// device pointer owned by the library.
+const float2 *twiddles_device(int N, bool reversed_seq);
+
+// Workgroup barrier that is also a compiler fence: no load or store is moved
+// across it, so twiddle and spectrum loads stay inside their pass instead of
+// being hoisted (and held in VGPRs) across the whole kernel.
+__device__ __forceinline__ void wg_barrier() {
+  asm volatile("" ::: "memory");
+  __syncthreads();
+  asm volatile("" ::: "memory");
+}
+
+// ------------------------------------------------------------ layouts
+// ROW: TB transforms stored one after another, each padded (lds_pad);
+//      consecutive lanes take consecutive butterflies of one transform.
+// COL: element i of transform (column) t at i*TB + t; consecutive lanes take
+//      consecutive columns, so every LDS access is unit stride across lanes.
+template <int N, int TB, bool COL>
+struct Layout {
+  static constexpr int TSTRIDE = lds_padded_len(N);
+  static constexpr int LDS_ELEMS = COL ? N * TB : TB * TSTRIDE;
+  __device__ static __forceinline__ int addr(int t, int i) {
+    if constexpr (COL) return i * TB + t;
+    else return t * TSTRIDE + lds_pad(i);
+  }
+  template <int R>
+  __device__ static __forceinline__ void bfly(int g, int &t, int &j) {
+    constexpr int NB = N / R;
+    if constexpr (COL) {
+      t = g % TB;
+      j = g / TB;
+    } else {
+      t = g / NB;
+      j = g % NB;
+    }
+  }
+};
+
+template <int N, bool REV>
+struct Seq {
+  static constexpr RadixSeq s = REV ? reversed(radix_seq(N)) : radix_seq(N);
+  static constexpr int n = s.n;
+  template <int P>
+  static constexpr int R = s.r[P];
+  template <int P>
+  static constexpr int LS = pass_ls(s, P);
+  template <int P>
+  static constexpr int64_t TWOFF = tw_offset(s, P);
+};
+
+// One Stockham pass: butterflies of this thread held in v[u][m].
+template <class LY, int R, int N, int NT, int TB>
+struct Pass {
+  static constexpr int NB = N / R;
+  static constexpr int TOT = NB * TB;
+  static constexpr int BPT = (TOT + NT - 1) / NT;
+  float2 v[BPT][R];
+
+  __device__ __forceinline__ static bool active(int u) {
+    return (TOT % NT == 0) || (int)(threadIdx.x + u * NT) < TOT;
+  }
+  // load(t, i) -> float2 : element i of transform t
+  template <class LoadF>
+  __device__ __forceinline__ void load(LoadF &&f) {
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+      if (!active(u)) continue;
+      int t, j;
+      LY::template bfly<R>(threadIdx.x + u * NT, t, j);
+#pragma unroll
+      for (int m = 0; m < R; ++m) v[u][m] = f(t, j + m * NB);
+    }
+  }
+  // twiddle + DFT_R in registers
+  template <int LS, bool INV>
+  __device__ __forceinline__ void compute(const float2 *__restrict__ tw) {
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+      if (!active(u)) continue;
+      if constexpr (LS > 1) {
+        int t, j;
+        LY::template bfly<R>(threadIdx.x + u * NT, t, j);
+        apply_twiddles<R, INV>(v[u], tw, j & (LS - 1));
+      }
+      dft<R, INV>(v[u]);
+    }
+  }
+  // store(t, i, value) for the Stockham output positions
+  template <int LS, class StoreF>
+  __device__ __forceinline__ void store(StoreF &&f) {
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+      if (!active(u)) continue;
+      int t, j;
+      LY::template bfly<R>(threadIdx.x + u * NT, t, j);
+      const int k = j & (LS - 1);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) f(t, base + q * LS, v[u][q]);
+    }
+  }
+};
+
+// Runs passes [P, PE) of sequence SQ with both input and output in LDS.
+// Pre: LDS holds pass P's input, visible to all threads (barrier passed).
+// Post: LDS holds pass PE-1's output, visible to all threads.
+template <class LY, class SQ, int N, int NT, int TB, bool INV, int P, int PE>
+__device__ __forceinline__ void lds_passes(float2 *lds, const float2 *__restrict__ tw) {
+  if constexpr (P < PE) {
+    constexpr int R = SQ::template R<P>;
+    constexpr int LS = SQ::template LS<P>;
+    Pass<LY, R, N, NT, TB> ps;
+    ps.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
+    wg_barrier();
+    ps.template compute<LS, INV>(tw + SQ::template TWOFF<P>);
+    ps.template store<LS>([&](int t, int i, float2 x) { lds[LY::addr(t, i)] = x; });
+    wg_barrier();
+    lds_passes<LY, SQ, N, NT, TB, INV, P + 1, PE>(lds, tw);
+  }
+}
+
+}  // namespace mxr

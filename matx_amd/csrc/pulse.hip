@@ -1,0 +1,413 @@
+// Pulse compression and fft_conv: matched filtering of every pulse (row)
+// against one waveform, i.e. y = (1/n) ifft(fft(x, n) .* conj(fft(w, n))),
+// the PulseCompression stage of examples/simple_radar_pipeline.h:273-301 and
+// the fft_conv idiom of examples/fft_conv.cu:90-108.
+//
+// MI355X design.  The lags a radar keeps (r < R = S-L+1, or all n for a full
+// circular fft_conv) are produced by overlap-save over blocks of M <= n points:
+// block b reads x[(bV + i) mod n], i < M, correlates it circularly with the
+// M-point spectrum conj(fft(w, M)) and keeps its first V = M-L+1 lags, which
+// equal lags bV..bV+V-1 of the n-point circular correlation exactly (no lag
+// inside a block wraps).  A planner picks M to minimise blocks*M*log2(M)
+// (4096 for a 9000-sample pulse and a 1000-sample waveform, 36% fewer flops
+// than one 16384-point transform), and M = n reproduces the plain n-point
+// form.  Each block is one fused kernel pass: load (zero-padding implicit),
+// forward Stockham passes, conj-multiply, inverse passes with the 1/M scale
+// folded in, store of the kept lags.  The last forward pass and the first
+// inverse pass share one radix, so they run back to back in registers.
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "fft_engine.hpp"
+#include "mxr_cache.hpp"
+
+namespace mxr {
+
+struct PcArgs {
+  const float2 *x;
+  float2 *y;
+  int64_t rows;
+  BatchIdx bi;      // row index -> offsets (sa: x, sb: y)
+  int64_t seff;     // valid input samples per row (min(S, n))
+  int64_t n;        // circular period of the correlation
+  int64_t nout;     // lags kept per row
+  int64_t V;        // kept lags per block
+  int64_t nblk;     // blocks per row
+  int64_t bpw;      // blocks per workgroup (nblk when y aliases x)
+  int64_t groups;   // total workgroups
+  const float2 *wc; // M-point waveform spectrum
+  const float2 *twf, *twi;
+  float scale;
+};
+
+template <int M>
+struct PcCfg {
+  static constexpr int NT = M / 16;
+};
+
+// ROWLOOP: one workgroup walks blocks [b0, b1) of its row in order (needed
+// when y aliases x: a block only overwrites lags below the next block's
+// first input).  Otherwise each workgroup does exactly one block.
+template <int M, bool ROWLOOP>
+__global__ __launch_bounds__(PcCfg<M>::NT) __attribute__((amdgpu_waves_per_eu(4)))
+void pc_ols_kernel(PcArgs a) {
+  constexpr int NT = PcCfg<M>::NT;
+  using LY = Layout<M, 1, false>;
+  using SF = Seq<M, false>;
+  using SI = Seq<M, true>;
+  constexpr int K = SF::n;
+  constexpr int R0 = SF::template R<0>;
+  constexpr int RL = SF::template R<K - 1>;
+  constexpr int LSL = SF::template LS<K - 1>;
+  static_assert(K >= 2, "pc kernel needs at least two passes");
+  __shared__ float2 lds[LY::LDS_ELEMS];
+
+  const int64_t g = xcd_remap(blockIdx.x, a.groups);
+  const int64_t gpr = (a.nblk + a.bpw - 1) / a.bpw;
+  const int64_t row = g / gpr;
+  const int64_t b0 = (g % gpr) * a.bpw;
+  const int64_t b1 = ROWLOOP ? (b0 + a.bpw < a.nblk ? b0 + a.bpw : a.nblk) : b0 + 1;
+  int64_t ox, oy;
+  batch_offsets(a.bi, row, &ox, &oy);
+  const float2 *__restrict__ xr = a.x + ox;
+  float2 *__restrict__ yr = a.y + oy;
+
+  const int n32 = (int)a.n, seff32 = (int)a.seff;
+  for (int64_t b = b0; b < b1; ++b) {
+    const int lo = (int)(b * a.V);
+    if (b != b0) wg_barrier();  // previous block's LDS reads are done
+
+    // forward pass 0 straight from HBM (zero padding and wrap implicit)
+    {
+      Pass<LY, R0, M, NT, 1> ps;
+      ps.load([&](int, int i) {
+        int gi = lo + i;
+        if (gi >= n32) gi -= n32;
+        return gi < seff32 ? xr[gi] : make_float2(0.f, 0.f);
+      });
+      ps.template compute<1, false>(nullptr);
+      ps.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
+    }
+    wg_barrier();
+    lds_passes<LY, SF, M, NT, 1, false, 1, K - 1>(lds, a.twf);
+
+    // last forward pass, conj-multiply, first inverse pass: one register pass
+    {
+      Pass<LY, RL, M, NT, 1> ps;
+      ps.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
+      wg_barrier();
+      ps.template compute<LSL, false>(a.twf + SF::template TWOFF<K - 1>);
+#pragma unroll
+      for (int u = 0; u < ps.BPT; ++u) {
+        if (!ps.active(u)) continue;
+        const int j = threadIdx.x + u * NT;  // k == j in the last pass
+#pragma unroll
+        for (int q = 0; q < RL; ++q) ps.v[u][q] = cmul(ps.v[u][q], a.wc[j + q * LSL]);
+      }
+      ps.template compute<1, true>(nullptr);
+      ps.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
+    }
+    wg_barrier();
+    lds_passes<LY, SI, M, NT, 1, true, 1, K - 1>(lds, a.twi);
+
+    // last inverse pass straight to HBM with the 1/M scale
+    {
+      constexpr int RI = SI::template R<K - 1>;
+      constexpr int LSI = SI::template LS<K - 1>;
+      Pass<LY, RI, M, NT, 1> ps;
+      ps.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
+      ps.template compute<LSI, true>(a.twi + SI::template TWOFF<K - 1>);
+      const float s = a.scale;
+      const int lim = (int)((a.nout - lo) < a.V ? (a.nout - lo) : a.V);
+      ps.template store<LSI>([&](int, int i, float2 v) {
+        if (i < lim) yr[lo + i] = cscale(v, s);
+      });
+    }
+  }
+}
+
+// Waveform spectrum: Wc = conj?(fft_M(pad_M(w2))), w2 = w1/||w1|| (optional),
+// w1 = w .* hamming(L) (optional); optionally writes w2 back to w.
+struct WfArgs {
+  float2 *w;
+  int64_t wstride;
+  int64_t L;
+  int window, normalize, writeback, conj;
+  float2 *wc;
+  const float2 *tw;
+};
+
+template <int M>
+__global__ __launch_bounds__(PcCfg<M>::NT) void wfspec_kernel(WfArgs a) {
+  constexpr int NT = PcCfg<M>::NT;
+  using LY = Layout<M, 1, false>;
+  using SF = Seq<M, false>;
+  __shared__ float2 lds[LY::LDS_ELEMS];
+  __shared__ double red[NT];
+
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < M; i += NT) {
+    float2 v = make_float2(0.f, 0.f);
+    if (i < a.L) {
+      v = a.w[(int64_t)i * a.wstride];
+      if (a.window) v = cscale(v, hamming_f32(i, (int)a.L));
+      acc += (double)v.x * v.x + (double)v.y * v.y;
+    }
+    lds[LY::addr(0, i)] = v;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (a.normalize) {
+    const float inv = (float)(1.0 / sqrt(red[0]));
+    for (int i = threadIdx.x; i < M; i += NT) {
+      float2 v = cscale(lds[LY::addr(0, i)], inv);
+      lds[LY::addr(0, i)] = v;
+      if (a.writeback && i < a.L) a.w[(int64_t)i * a.wstride] = v;
+    }
+  } else if (a.writeback) {
+    for (int i = threadIdx.x; i < a.L; i += NT) a.w[(int64_t)i * a.wstride] = lds[LY::addr(0, i)];
+  }
+  __syncthreads();
+  lds_passes<LY, SF, M, NT, 1, false, 0, SF::n>(lds, a.tw);
+  for (int i = threadIdx.x; i < M; i += NT) {
+    float2 v = lds[LY::addr(0, i)];
+    if (a.conj) v.y = -v.y;
+    a.wc[i] = v;
+  }
+}
+
+// ------------------------------------------------------------------ host
+namespace {
+
+struct PcPlan {
+  int M;
+  int64_t V, nblk;
+};
+
+int ilog2(int64_t v) {
+  int l = 0;
+  while ((int64_t(1) << l) < v) ++l;
+  return l;
+}
+
+// Forced block size for experiments/tuning: MXR_PC_BLOCK=<M> (0 = auto).
+int forced_block() {
+  static int v = [] {
+    const char *e = getenv("MXR_PC_BLOCK");
+    return e ? atoi(e) : 0;
+  }();
+  return v;
+}
+
+// Chooses the overlap-save block size.  allow_ols is false for convolution
+// (conj_w = 0), where only the single n-point block is exact.
+mxr_status plan_pc(int64_t seff, int64_t L, int64_t n, int64_t nout, bool allow_ols, bool in_place,
+                   PcPlan *out) {
+  PcPlan best{0, 0, 0};
+  double best_cost = 1e300;
+  auto consider = [&](int M) {
+    PcPlan p;
+    p.M = M;
+    if (M == n) {
+      p.V = nout;
+      p.nblk = 1;
+    } else {
+      if (M < L || M > n) return;
+      p.V = M - L + 1;
+      p.nblk = (nout + p.V - 1) / p.V;
+      if (in_place) {
+        // a block that wraps past n would read lags already overwritten
+        const int64_t last_lo = (p.nblk - 1) * p.V;
+        if (last_lo + M > n && (last_lo + M - n) > 0 && seff > 0) return;
+      }
+    }
+    const double cost = (double)p.nblk * M * ilog2(M) * (M <= 16384 ? 1.0 : 1e9);
+    if (cost < best_cost || (cost == best_cost && M > best.M)) {
+      best_cost = cost;
+      best = p;
+    }
+  };
+  const int fb = forced_block();
+  if (fb > 0) {
+    consider(fb);
+  } else {
+    if (n <= 16384) consider((int)n);
+    if (allow_ols)
+      for (int M = 1024; M <= 16384 && M < n; M *= 2) consider(M);
+  }
+  MXR_REQUIRE(best.M > 0 && best.M <= 16384, MXR_NOT_SUPPORTED,
+              "pulse compression: no block size fits (S=%lld, L=%lld, n=%lld, nout=%lld%s)",
+              (long long)seff, (long long)L, (long long)n, (long long)nout,
+              in_place ? ", in place" : "");
+  *out = best;
+  return MXR_SUCCESS;
+}
+
+// Per (device, stream, M) waveform-spectrum buffer: calls on different
+// streams never share one (cf. per-stream workspaces, cache.h:261-291).
+std::mutex g_wc_mu;
+std::map<std::tuple<int, uintptr_t, int>, float2 *> g_wc;
+
+float2 *wc_buffer(int M, hipStream_t s) {
+  const auto key = std::make_tuple(device_id(), (uintptr_t)s, M);
+  std::lock_guard<std::mutex> g(g_wc_mu);
+  auto it = g_wc.find(key);
+  if (it != g_wc.end()) return it->second;
+  float2 *p = nullptr;
+  if (hipMalloc(&p, sizeof(float2) * M) != hipSuccess) return nullptr;
+  g_wc[key] = p;
+  return p;
+}
+
+void clear_wc() {
+  std::lock_guard<std::mutex> g(g_wc_mu);
+  for (auto &kv : g_wc) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(std::get<0>(kv.first));
+    (void)hipFree(kv.second);
+    (void)hipSetDevice(cur);
+  }
+  g_wc.clear();
+}
+CacheClearRegistrar g_reg_wc(clear_wc);
+
+template <int M>
+mxr_status launch_pc(const WfArgs &wa, const PcArgs &pa, hipStream_t s) {
+  wfspec_kernel<M><<<dim3(1), dim3(PcCfg<M>::NT), 0, s>>>(wa);
+  MXR_HIP_TRY(hipGetLastError());
+  if (pa.groups > 0) {
+    if (pa.bpw > 1)
+      pc_ols_kernel<M, true><<<dim3((unsigned)pa.groups), dim3(PcCfg<M>::NT), 0, s>>>(pa);
+    else
+      pc_ols_kernel<M, false><<<dim3((unsigned)pa.groups), dim3(PcCfg<M>::NT), 0, s>>>(pa);
+    MXR_HIP_TRY(hipGetLastError());
+  }
+  return MXR_SUCCESS;
+}
+
+mxr_status dispatch_pc(int M, const WfArgs &wa, const PcArgs &pa, hipStream_t s) {
+  switch (M) {
+    case 1024: return launch_pc<1024>(wa, pa, s);
+    case 2048: return launch_pc<2048>(wa, pa, s);
+    case 4096: return launch_pc<4096>(wa, pa, s);
+    case 8192: return launch_pc<8192>(wa, pa, s);
+    case 16384: return launch_pc<16384>(wa, pa, s);
+    default: break;
+  }
+  set_error(MXR_NOT_SUPPORTED, "pulse compression: block size %d unsupported (1024..16384)", M);
+  return MXR_NOT_SUPPORTED;
+}
+
+}  // namespace
+
+// Shared body of mxr_pulse_compress and mxr_fft_conv.
+mxr_status pc_common(const char *fn, const mxr_tensor *y, const mxr_tensor *x, const mxr_tensor *w,
+                     int64_t n, int window, int normalize, int writeback, int conj_w,
+                     hipStream_t stream) {
+  mxr_status st;
+  if ((st = check_tensor(y, "output", MXR_C64, 1, MXR_MAX_RANK))) return st;
+  if ((st = check_tensor(x, "input", MXR_C64, 1, MXR_MAX_RANK))) return st;
+  if ((st = check_tensor(w, "waveform", MXR_C64, 1, 1))) return st;
+  MXR_REQUIRE(x->rank == y->rank, MXR_INVALID_DIM, "%s: input rank %d != output rank %d", fn,
+              x->rank, y->rank);
+  const int rank = x->rank;
+  for (int d = 0; d < rank - 1; ++d)
+    MXR_REQUIRE(x->shape[d] == y->shape[d], MXR_INVALID_SIZE, "%s: batch dim %d differs", fn, d);
+  MXR_REQUIRE(x->stride[rank - 1] == 1 && y->stride[rank - 1] == 1, MXR_INVALID_PARAMETER,
+              "%s: the sample axis must have unit stride", fn);
+  const int64_t S = x->shape[rank - 1];
+  const int64_t L = w->shape[0];
+  const int64_t nout = y->shape[rank - 1];
+  MXR_REQUIRE(n > 0 && (n & (n - 1)) == 0, MXR_NOT_SUPPORTED,
+              "%s: transform size %lld must be a power of two", fn, (long long)n);
+  MXR_REQUIRE(L >= 1 && L <= n, MXR_INVALID_SIZE, "%s: waveform length %lld not in [1, n=%lld]", fn,
+              (long long)L, (long long)n);
+  MXR_REQUIRE(nout <= n, MXR_INVALID_SIZE, "%s: %lld output lags exceed the transform size %lld", fn,
+              (long long)nout, (long long)n);
+  MXR_REQUIRE(!views_overlap(*w, *x) && !views_overlap(*w, *y), MXR_ALIAS_ERROR,
+              "%s: waveform aliases the signal", fn);
+  bool in_place = false;
+  if (views_overlap(*x, *y)) {
+    bool same_rows = x->data == y->data;
+    for (int d = 0; d < rank - 1; ++d) same_rows = same_rows && x->stride[d] == y->stride[d];
+    MXR_REQUIRE(same_rows, MXR_ALIAS_ERROR,
+                "%s: output overlaps input without being the same rows (in place)", fn);
+    in_place = true;
+  }
+  const int64_t seff = S < n ? S : n;
+  PcPlan plan;
+  if ((st = plan_pc(seff, L, n, nout, conj_w != 0, in_place, &plan))) return st;
+  MXR_REQUIRE(L <= plan.M, MXR_NOT_SUPPORTED, "%s: waveform longer than the block", fn);
+
+  float2 *wc = wc_buffer(plan.M, stream);
+  MXR_REQUIRE(wc != nullptr, MXR_OUT_OF_MEMORY, "%s: spectrum buffer allocation failed", fn);
+  const float2 *twf = twiddles_device(plan.M, false);
+  const float2 *twi = twiddles_device(plan.M, true);
+  MXR_REQUIRE(twf && twi, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
+
+  WfArgs wa{};
+  wa.w = (float2 *)w->data;
+  wa.wstride = w->stride[0];
+  wa.L = L;
+  wa.window = window;
+  wa.normalize = normalize;
+  wa.writeback = writeback;
+  wa.conj = conj_w;
+  wa.wc = wc;
+  wa.tw = twf;
+
+  PcArgs pa{};
+  pa.x = (const float2 *)x->data;
+  pa.y = (float2 *)y->data;
+  pa.rows = 1;
+  pa.bi.nd = rank - 1;
+  for (int d = 0; d < rank - 1; ++d) {
+    pa.bi.shape[d] = x->shape[d];
+    pa.bi.sa[d] = x->stride[d];
+    pa.bi.sb[d] = y->stride[d];
+    pa.rows *= x->shape[d];
+  }
+  pa.seff = seff;
+  pa.n = n;
+  pa.nout = nout;
+  pa.V = plan.V;
+  pa.nblk = plan.nblk;
+  pa.bpw = in_place ? plan.nblk : 1;
+  const int64_t gpr = (pa.nblk + pa.bpw - 1) / pa.bpw;
+  pa.groups = nout > 0 ? pa.rows * gpr : 0;
+  MXR_REQUIRE(pa.groups <= 0x7fffffff, MXR_INVALID_SIZE, "%s: too many rows", fn);
+  pa.wc = wc;
+  pa.twf = twf;
+  pa.twi = twi;
+  pa.scale = (float)(1.0 / (double)plan.M);
+  return dispatch_pc(plan.M, wa, pa, stream);
+}
+
+}  // namespace mxr
+
+extern "C" {
+
+mxr_status mxr_pulse_compress(const mxr_tensor *y, const mxr_tensor *x, const mxr_tensor *w,
+                              int64_t n, int flags, hipStream_t stream) {
+  return mxr::guarded("mxr_pulse_compress", [&] {
+    return mxr::pc_common("pulse_compress", y, x, w, n, (flags & MXR_PC_WINDOW) != 0,
+                          (flags & MXR_PC_NORMALIZE) != 0, (flags & MXR_PC_WRITEBACK) != 0, 1,
+                          stream);
+  });
+}
+
+mxr_status mxr_fft_conv(const mxr_tensor *out, const mxr_tensor *x, const mxr_tensor *w, int64_t n,
+                        int conj_w, hipStream_t stream) {
+  return mxr::guarded("mxr_fft_conv", [&] {
+    return mxr::pc_common("fft_conv", out, x, w, n, 0, 0, 0, conj_w ? 1 : 0, stream);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+// libmxr runtime: thread-local errors, descriptor checks, per-device caches
+// of twiddle tables (the analogue of MatX's plan cache,
+// include/matx/core/cache.h:167-291), and mxr_clear_caches.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "fft_engine.hpp"
+#include "mxr_cache.hpp"
+
+namespace mxr {
+
+static thread_local std::string g_last_error;
+
+void set_error(mxr_status st, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = std::string("[mxr status ") + std::to_string((int)st) + "] " + buf;
+}
+
+mxr_status check_tensor(const mxr_tensor *t, const char *name, int want_dtype, int min_rank,
+                        int max_rank) {
+  MXR_REQUIRE(t != nullptr, MXR_INVALID_PARAMETER, "%s: null descriptor", name);
+  MXR_REQUIRE(t->rank >= min_rank && t->rank <= max_rank, MXR_INVALID_DIM,
+              "%s: rank %d outside [%d, %d]", name, t->rank, min_rank, max_rank);
+  if (want_dtype) {
+    MXR_REQUIRE(t->dtype == want_dtype, MXR_INVALID_TYPE, "%s: dtype %d, expected %d", name,
+                t->dtype, want_dtype);
+  }
+  MXR_REQUIRE(dtype_size(t->dtype) != 0, MXR_INVALID_TYPE, "%s: unknown dtype %d", name, t->dtype);
+  for (int d = 0; d < t->rank; ++d) {
+    MXR_REQUIRE(t->shape[d] >= 0, MXR_INVALID_SIZE, "%s: negative extent in dim %d", name, d);
+  }
+  MXR_REQUIRE(t->data != nullptr || numel(*t) == 0, MXR_INVALID_PARAMETER, "%s: null data", name);
+  return MXR_SUCCESS;
+}
+
+void byte_span(const mxr_tensor &t, uintptr_t *lo, uintptr_t *hi) {
+  int64_t mn = 0, mx = 0;
+  for (int d = 0; d < t.rank; ++d) {
+    if (t.shape[d] == 0) {
+      *lo = *hi = (uintptr_t)t.data;
+      return;
+    }
+    int64_t e = (t.shape[d] - 1) * t.stride[d];
+    if (e < 0) mn += e;
+    else mx += e;
+  }
+  const size_t es = dtype_size(t.dtype);
+  *lo = (uintptr_t)t.data + (intptr_t)(mn * (int64_t)es);
+  *hi = (uintptr_t)t.data + (uintptr_t)((mx + 1) * (int64_t)es);
+}
+
+bool views_overlap(const mxr_tensor &a, const mxr_tensor &b) {
+  uintptr_t alo, ahi, blo, bhi;
+  byte_span(a, &alo, &ahi);
+  byte_span(b, &blo, &bhi);
+  return alo < bhi && blo < ahi;
+}
+
+int device_id() {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  return d;
+}
+
+int max_lds_per_block() {
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  std::lock_guard<std::mutex> g(mu);
+  int d = device_id();
+  auto it = cache.find(d);
+  if (it != cache.end()) return it->second;
+  int v = 65536;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, d) != hipSuccess) v = 65536;
+  cache[d] = v;
+  return v;
+}
+
+void build_twiddles(int N, RadixSeq s, float2 *out) {
+  const double two_pi = 6.283185307179586476925286766559;
+  int64_t Ls = 1;
+  for (int p = 0; p < s.n; ++p) {
+    const int R = s.r[p];
+    if (Ls > 1) {
+      float2 *t = out + tw_offset(s, p);
+      const int64_t len = Ls * R;
+      for (int64_t k = 0; k < Ls; ++k) {
+        for (int m = 0; m < R; ++m) {
+          // reduce m*k modulo len before scaling: the angle index stays exact
+          const int64_t e = (m * k) % len;
+          const double a = -two_pi * (double)e / (double)len;
+          t[k * R + m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        }
+      }
+    }
+    Ls *= R;
+  }
+  (void)N;
+}
+
+// --------------------------------------------------------- device caches
+namespace {
+struct TwKey {
+  int dev, n, rev;
+  bool operator<(const TwKey &o) const {
+    return dev != o.dev ? dev < o.dev : (n != o.n ? n < o.n : rev < o.rev);
+  }
+};
+std::mutex g_tw_mu;
+std::map<TwKey, float2 *> g_tw;
+
+std::mutex g_hooks_mu;
+std::vector<void (*)()> &clear_hooks() {
+  static std::vector<void (*)()> h;
+  return h;
+}
+}  // namespace
+
+void register_cache_clear(void (*fn)()) {
+  std::lock_guard<std::mutex> g(g_hooks_mu);
+  clear_hooks().push_back(fn);
+}
+
+const float2 *twiddles_device(int N, bool rev) {
+  const int dev = device_id();
+  std::lock_guard<std::mutex> g(g_tw_mu);
+  TwKey key{dev, N, rev ? 1 : 0};
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) return it->second;
+  RadixSeq s = radix_seq(N);
+  if (s.n < 0) return nullptr;
+  if (rev) s = reversed(s);
+  const int64_t sz = tw_size(s) < 2 ? 2 : tw_size(s);
+  std::vector<float2> host((size_t)sz, make_float2(1.f, 0.f));
+  build_twiddles(N, s, host.data());
+  float2 *d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * sz) != hipSuccess) return nullptr;
+  // a synchronous copy: tables are built once per (device, size), before use
+  if (hipMemcpy(d, host.data(), sizeof(float2) * sz, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  g_tw[key] = d;
+  return d;
+}
+
+static void clear_twiddles() {
+  std::lock_guard<std::mutex> g(g_tw_mu);
+  for (auto &kv : g_tw) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(kv.first.dev);
+    (void)hipFree(kv.second);
+    (void)hipSetDevice(cur);
+  }
+  g_tw.clear();
+}
+
+}  // namespace mxr
+
+extern "C" {
+
+const char *mxr_last_error(void) { return mxr::g_last_error.c_str(); }
+
+int mxr_version(void) { return 100; }
+
+mxr_status mxr_clear_caches(void) {
+  return mxr::guarded("mxr_clear_caches", [&]() -> mxr_status {
+    MXR_HIP_TRY(hipDeviceSynchronize());
+    std::vector<void (*)()> hooks;
+    {
+      std::lock_guard<std::mutex> g(mxr::g_hooks_mu);
+      hooks = mxr::clear_hooks();
+    }
+    for (auto fn : hooks) fn();
+    mxr::clear_twiddles();
+    return MXR_SUCCESS;
+  });
+}
+
+}  // extern "C"

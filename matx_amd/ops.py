@@ -1,0 +1,135 @@
+"""MatX operator semantics over torch device tensors, executed by libmxr.so.
+
+Each function mirrors the MatX operator of the same name (paths relative to
+the MatX repository root) and calls the HIP library through its C ABI.  Output
+tensors may be passed in (`out=`) to write into existing views -- MatX's
+``(lhs = op).run(exec)`` -- otherwise they are allocated on the input's device.
+Everything is enqueued on the current torch stream (or ``stream=``).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import _lib as L
+
+_MODES = {"full": L.FULL, "same": L.SAME, "valid": L.VALID}
+_NORMS = {"backward": L.NORM_BACKWARD, "forward": L.NORM_FORWARD, "ortho": L.NORM_ORTHO}
+
+
+def _mode(m):
+    return _MODES[m] if isinstance(m, str) else int(m)
+
+
+def _fft(x, n, axis, norm, out, inverse, stream):
+    ax = axis % x.dim()
+    n = int(n) if n else x.shape[ax]
+    if out is None:
+        shape = list(x.shape)
+        shape[ax] = n
+        out = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    L.check(L.lib().mxr_fft_c2c(L.ref(L.desc(out)), L.ref(L.desc(x)), n,
+                                L.INVERSE if inverse else L.FORWARD,
+                                _NORMS[norm], ax, L.stream_handle(stream)))
+    return out
+
+
+def fft(x, n: int = 0, axis: int = -1, norm: str = "backward", out=None, stream=None):
+    """``fft(x, n)`` along ``axis`` (include/matx/operators/fft.h:538-577):
+    zero-pads or truncates to n; forward unscaled under the default norm."""
+    return _fft(x, n, axis, norm, out, False, stream)
+
+
+def ifft(x, n: int = 0, axis: int = -1, norm: str = "backward", out=None, stream=None):
+    """``ifft`` (include/matx/operators/fft.h:649-680), 1/n under "backward"."""
+    return _fft(x, n, axis, norm, out, True, stream)
+
+
+def fft_conv(x, w, n: int, conj: bool = True, nout: int | None = None, out=None, stream=None):
+    """out = ifft(fft(x, n) * conj(fft(w, n))) along the last axis (lags
+    0..nout-1, default all n); conj=False gives circular convolution."""
+    nout = n if nout is None else nout
+    if out is None:
+        out = torch.empty(x.shape[:-1] + (nout,), dtype=torch.complex64, device=x.device)
+    L.check(L.lib().mxr_fft_conv(L.ref(L.desc(out)), L.ref(L.desc(x)), L.ref(L.desc(w)),
+                                 int(n), 1 if conj else 0, L.stream_handle(stream)))
+    return out
+
+
+def pulse_compress(x, w, n: int, nout: int | None = None, window: bool = True,
+                   normalize: bool = True, writeback: bool = False, out=None, stream=None):
+    """PulseCompression stage (examples/simple_radar_pipeline.h:273-301):
+    matched filter of every row of x against hamming-windowed, unit-energy w,
+    keeping lags 0..nout-1 (default S-L+1).  ``out=x`` works in place."""
+    S = x.shape[-1]
+    nout = S - w.shape[0] + 1 if nout is None else nout
+    if out is None:
+        out = torch.empty(x.shape[:-1] + (nout,), dtype=torch.complex64, device=x.device)
+    flags = (L.PC_WINDOW if window else 0) | (L.PC_NORMALIZE if normalize else 0) | \
+            (L.PC_WRITEBACK if writeback else 0)
+    L.check(L.lib().mxr_pulse_compress(L.ref(L.desc(out)), L.ref(L.desc(x)), L.ref(L.desc(w)),
+                                       int(n), flags, L.stream_handle(stream)))
+    return out
+
+
+def doppler(y, nd: int, tpc: str = "same", win_rows: int | None = None, rd=None, pw=None,
+            want_rd: bool = False, want_pow: bool = True, stream=None):
+    """Three-pulse canceller + Hamming window + nd-point Doppler FFT (+ |.|^2).
+    tpc: "same" (MatX stage dataflow), "valid" (numpy model) or "none"
+    (input already cancelled).  Returns (pow, rd)."""
+    C, P, R = y.shape
+    mode = {"same": L.SAME, "valid": L.VALID, "none": L.TPC_NONE}[tpc]
+    rows = P - 2 if mode == L.VALID else P
+    win_rows = (P - 2 if mode != L.TPC_NONE else rows) if win_rows is None else win_rows
+    if pw is None and want_pow:
+        pw = torch.empty((C, nd, R), dtype=torch.float32, device=y.device)
+    if rd is None and want_rd:
+        rd = torch.empty((C, nd, R), dtype=torch.complex64, device=y.device)
+    L.check(L.lib().mxr_doppler(L.ref(L.desc(pw)), L.ref(L.desc(rd)), L.ref(L.desc(y)), mode,
+                                int(win_rows), int(nd), L.stream_handle(stream)))
+    return pw, rd
+
+
+def cfar(inp, win=(5, 13), guard=(3, 3), pfa: float = 1e-5, dets=None, ba=None, pw=None,
+         want_ba: bool = False, stream=None):
+    """CA-CFAR (examples/simple_radar_pipeline.h:394-436): inp is f32 power or
+    c64 (|.|^2 on load).  Returns (dets, ba, pow)."""
+    if inp.dim() == 2:
+        inp = inp.unsqueeze(0)
+    if dets is None:
+        dets = torch.empty(inp.shape, dtype=torch.int32, device=inp.device)
+    if ba is None and want_ba:
+        ba = torch.empty(inp.shape, dtype=torch.float32, device=inp.device)
+    p = L.mxr_cfar_params(win[0], win[1], guard[0], guard[1], float(pfa))
+    L.check(L.lib().mxr_cfar(L.ref(L.desc(dets)), L.ref(L.desc(ba)), L.ref(L.desc(pw)),
+                             L.ref(L.desc(inp)), L.ctypes.byref(p), L.stream_handle(stream)))
+    return dets, ba, pw
+
+
+def conv1d(x, f, mode="full", out=None, stream=None):
+    """Direct conv1d along the last axis (include/matx/operators/conv.h:244-266)."""
+    m = _mode(m=mode)
+    n, k = x.shape[-1], f.shape[-1]
+    olen = {L.FULL: n + k - 1, L.SAME: n, L.VALID: n - k + 1}[m]
+    cplx = x.is_complex() or f.is_complex()
+    if out is None:
+        out = torch.empty(x.shape[:-1] + (olen,), device=x.device,
+                          dtype=torch.complex64 if cplx else torch.float32)
+    L.check(L.lib().mxr_conv1d(L.ref(L.desc(out)), L.ref(L.desc(x)), L.ref(L.desc(f)), m,
+                               L.stream_handle(stream)))
+    return out
+
+
+def conv2d(x, f, mode="full", out=None, stream=None):
+    """Direct conv2d over the last two axes (include/matx/operators/conv.h:463-490)."""
+    m = _mode(m=mode)
+    H, W = x.shape[-2:]
+    fh, fw = f.shape
+    oh, ow = {L.FULL: (H + fh - 1, W + fw - 1), L.SAME: (H, W),
+              L.VALID: (H - fh + 1, W - fw + 1)}[m]
+    cplx = x.is_complex() or f.is_complex()
+    if out is None:
+        out = torch.empty(x.shape[:-2] + (oh, ow), device=x.device,
+                          dtype=torch.complex64 if cplx else torch.float32)
+    L.check(L.lib().mxr_conv2d(L.ref(L.desc(out)), L.ref(L.desc(x)), L.ref(L.desc(f)), m,
+                               L.stream_handle(stream)))
+    return out

@@ -1,0 +1,286 @@
+"""GPU parity of the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances: relative L2 <= 1e-5 for every floating-point output (FP32 arithmetic
+against an FP64 oracle); CFAR detection masks identical except at cells whose
+oracle margin |pow - alpha*ba| / (alpha*ba) is below 1e-5 (float rounding may
+legitimately flip those); integer outputs exact.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+RTOL = 1e-5
+MARGIN = 1e-5
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def dev(a, dtype=torch.complex64):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def crandn(rng, *shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+@pytest.fixture(scope="module")
+def mx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import matx_amd
+    matx_amd.lib()
+    return matx_amd
+
+
+def dets_match(got, ref):
+    """Masks equal except where the oracle is within MARGIN of threshold."""
+    got = np.asarray(got)
+    near = np.abs(ref["margin"]) < MARGIN
+    bad = (got != ref["dets"]) & ~near
+    return int(bad.sum()), int(near.sum())
+
+
+# ----------------------------------------------------------------- FFT
+@pytest.mark.parametrize("n", [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384])
+def test_fft_sizes(mx, orc, n):
+    rng = np.random.default_rng(n)
+    b = max(1, 8192 // n)
+    x = crandn(rng, b, n)
+    X = host(mx.fft(dev(x)))
+    assert rel(X, orc.fft(x)) < RTOL
+    xi = host(mx.ifft(dev(X)))
+    assert rel(xi, x) < RTOL
+
+
+def test_fft_pad_truncate_norms(mx, orc):
+    rng = np.random.default_rng(1)
+    x = crandn(rng, 3, 100)
+    for n in (128, 64):
+        for norm in ("backward", "forward", "ortho"):
+            assert rel(host(mx.fft(dev(x), n, norm=norm)), orc.fft(x, n, norm=norm)) < RTOL
+            assert rel(host(mx.ifft(dev(x), n, norm=norm)), orc.ifft(x, n, norm=norm)) < RTOL
+
+
+def test_fft_strided_axis_and_inplace(mx, orc):
+    rng = np.random.default_rng(2)
+    x = crandn(rng, 4, 256, 300)
+    xd = dev(x)
+    # along a strided axis (the Doppler form, fft(tpcView.Permute({0,2,1})))
+    got = host(mx.fft(xd, axis=1))
+    assert rel(got, orc.fft(x, axis=1)) < RTOL
+    # permuted view in, permuted view out, in place
+    v = xd.permute(0, 2, 1)
+    mx.fft(v, out=v)
+    assert rel(host(xd), orc.fft(x, axis=1)) < RTOL
+
+
+# --------------------------------------------------------- fft_conv / PC
+def lfm(L, bw, rng=None):
+    n = np.arange(L)
+    t = n / 1.0
+    f0, f1 = -bw / 2, bw / 2
+    ph = 2 * np.pi * (f0 * t + 0.5 * (f1 - f0) / L * t * t)
+    return (np.exp(1j * ph) * np.hamming(L)).astype(np.complex64)
+
+
+def test_fft_conv_config1(mx, orc):
+    """4 x 32 x 2048 cube, 256-sample LFM (0.5 fs), N = 4096, all lags."""
+    rng = np.random.default_rng(3)
+    x = crandn(rng, 4, 32, 2048)
+    w = lfm(256, 0.5)
+    got = host(mx.fft_conv(dev(x), dev(w), 4096))
+    assert got.shape == (4, 32, 4096)
+    assert rel(got, orc.fft_conv(x, w, 4096)) < RTOL
+    conv = host(mx.fft_conv(dev(x), dev(w), 4096, conj=False))
+    ref = np.fft.ifft(np.fft.fft(x.astype(complex), 4096) * np.fft.fft(w.astype(complex), 4096))
+    assert rel(conv, ref) < RTOL
+
+
+@pytest.mark.parametrize("S,L,n,nout", [
+    (9000, 1000, 16384, 8001),    # radar config: overlap-save, M = 4096
+    (2048, 256, 4096, 4096),      # full circular, single block
+    (16000, 64, 16384, 16384),    # overlap-save blocks that wrap circularly
+    (3000, 700, 4096, 2301),
+    (1500, 1500, 2048, 1),        # waveform as long as the pulse
+])
+def test_fft_conv_shapes(mx, orc, S, L, n, nout):
+    rng = np.random.default_rng(S + L)
+    x = crandn(rng, 3, S)
+    w = crandn(rng, L)
+    got = host(mx.fft_conv(dev(x), dev(w), n, nout=nout))
+    ref = orc.fft_conv(x, w, n)[:, :nout]
+    assert rel(got, ref) < RTOL
+
+
+def test_pulse_compress_matx_input(mx, orc, matx_inputs, golden):
+    x, w = matx_inputs
+    Wc = orc.waveform_spectrum(w, 16384)
+    ref = orc.pulse_compress(x, Wc, 8001)
+    got = host(mx.pulse_compress(dev(x), dev(w), 16384))
+    assert rel(got, ref) < RTOL
+    yi = golden["model_y_idx"]
+    assert rel(got[yi[:, 0], yi[:, 1]], golden["model_y_val"]) < RTOL
+
+
+def test_pulse_compress_inplace_writeback(mx, orc, matx_inputs):
+    x, w = matx_inputs
+    C, N, S, R = 2, 16384, 9000, 8001
+    buf = torch.zeros((C, 128, N), dtype=torch.complex64, device="cuda")
+    buf[:, :, :S] = dev(x)
+    wv = torch.zeros(N, dtype=torch.complex64, device="cuda")
+    wv[:1000] = dev(w)
+    mx.pulse_compress(buf[:, :, :S], wv[:1000], N, nout=R, writeback=True, out=buf[:, :, :R])
+    Wc, w2 = orc.waveform_spectrum(w, N, return_prepared=True)
+    ref = orc.pulse_compress(x, Wc, R)
+    got = host(buf)
+    for c in range(C):
+        assert rel(got[c, :, :R], ref) < RTOL
+    assert np.array_equal(got[:, :, R:S], np.broadcast_to(x[:, R:S], (C, 128, S - R)))
+    assert rel(host(wv)[:1000], w2) < RTOL
+    assert np.all(host(wv)[1000:] == 0)
+
+
+# ------------------------------------------------------------ pulse axis
+@pytest.mark.parametrize("mode,nd", [("same", 256), ("valid", 128), ("valid", 256), ("same", 512)])
+def test_doppler_modes(mx, orc, mode, nd):
+    rng = np.random.default_rng(nd)
+    C, P, R = 2, 128, 333
+    y = crandn(rng, C, P, R)
+    pw, rd = mx.doppler(dev(y), nd, tpc=mode, want_rd=True)
+    t = orc.tpc(y, mode)
+    D = orc.doppler(t, P - 2, nd)
+    assert rel(host(rd), D) < RTOL
+    assert rel(host(pw), np.abs(D) ** 2) < RTOL
+
+
+def test_doppler_inplace_stage_form(mx, orc):
+    rng = np.random.default_rng(5)
+    C, P, Pr, R = 2, 128, 256, 200
+    t = np.zeros((C, Pr, R), np.complex64)
+    t[:, :P] = crandn(rng, C, P, R)
+    td = dev(t)
+    mx.doppler(td, Pr, tpc="none", win_rows=P - 2, rd=td, want_pow=False)
+    D = orc.doppler(t[:, :P], P - 2, Pr)
+    assert rel(host(td), D) < RTOL
+
+
+# ------------------------------------------------------------------ CFAR
+def test_cfar_random_maps(mx, orc):
+    rng = np.random.default_rng(6)
+    pw = rng.exponential(1.0, (3, 128, 1500)).astype(np.float32)
+    for _ in range(40):
+        pw[rng.integers(3), rng.integers(128), rng.integers(1500)] *= rng.uniform(5, 1e6)
+    ref = orc.cfar(pw)
+    dets, ba, _ = mx.cfar(dev(pw, torch.float32), want_ba=True)
+    assert rel(host(ba), ref["ba"]) < RTOL
+    bad, near = dets_match(host(dets), ref)
+    assert bad == 0, (bad, near)
+    assert host(dets).sum() > 0
+
+
+def test_cfar_complex_input_and_edges(mx, orc):
+    rng = np.random.default_rng(7)
+    D = crandn(rng, 1, 9, 20)     # smaller than the window in rows
+    D[0, 4, 10] *= 30
+    pwr = np.abs(D.astype(complex)) ** 2
+    ref = orc.cfar(pwr)
+    pw_out = torch.empty((1, 9, 20), dtype=torch.float32, device="cuda")
+    dets, ba, _ = mx.cfar(dev(D), pw=pw_out, want_ba=True)
+    assert rel(host(pw_out), pwr) < RTOL
+    assert rel(host(ba), ref["ba"]) < RTOL
+    bad, _ = dets_match(host(dets), ref)
+    assert bad == 0
+
+
+# ------------------------------------------------------------------ conv
+def test_conv_kats_gpu(mx):
+    import json
+    import os
+    k = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kat_convcorr.json")))
+    x = dev(np.array(k["conv1d"]["in"], np.float32), torch.float32)
+    f = dev(np.array(k["conv1d"]["filter"], np.float32), torch.float32)
+    for m in ("full", "same", "valid"):
+        assert np.allclose(host(mx.conv1d(x, f, m)), k["conv1d"][m], atol=1e-5)
+    x2 = dev(np.array(k["conv2d"]["in"], np.float32), torch.float32)
+    f2 = dev(np.array(k["conv2d"]["filter"], np.float32), torch.float32)
+    for m in ("full", "same", "valid"):
+        assert np.allclose(host(mx.conv2d(x2, f2, m)), k["conv2d"][m], atol=1e-4)
+
+
+def test_conv_random_strided(mx, orc):
+    rng = np.random.default_rng(8)
+    x = crandn(rng, 3, 50, 40)
+    f = rng.standard_normal(7).astype(np.float32)
+    xd = dev(x)
+    v = xd.permute(0, 2, 1)        # conv along the middle axis via a permuted view
+    for m in ("full", "same", "valid"):
+        got = host(mx.conv1d(v, dev(f, torch.float32), m))
+        ref = orc.conv1d(np.transpose(x, (0, 2, 1)), f, m)
+        assert rel(got, ref) < RTOL
+    f2 = crandn(rng, 5, 13)
+    for m in ("full", "same", "valid"):
+        assert rel(host(mx.conv2d(xd, dev(f2), m)), orc.conv2d(x, f2, m)) < RTOL
+
+
+# ------------------------------------------------- end to end vs reference
+def test_model_chain_17_detections(mx, orc, matx_inputs, golden):
+    """numpy-model dataflow on MatX's test input: 17 detections per channel
+    (test/01_radar/MultiChannelRadarPipeline.cu:187-197), 16 channels."""
+    x, w = matx_inputs
+    C = 16
+    xd = dev(x).unsqueeze(0).expand(C, -1, -1).contiguous()
+    pipe = mx.FusedRadarPipeline(C, 128, 9000, 1000, nd=256, tpc="valid")
+    dets = host(pipe.step(xd, dev(w)))
+    assert dets.sum() == 17 * C
+    for c in range(C):
+        assert np.array_equal(np.argwhere(dets[c] == 1), golden["model_det_idx"])
+    pw = host(pipe.pw)[0]
+    st = orc.radar_pipeline(x[None], w, Nd=256, mode="model", stages=True)
+    assert rel(pw, st["pow"][0]) < RTOL
+
+
+def test_matx_stage_api(mx, orc, matx_inputs, golden):
+    """The four stage tests of MultiChannelRadarPipeline.cu, run end to end on
+    the MatX-compatible RadarPipeline (SAME canceller, Pr = 256)."""
+    x, w = matx_inputs
+    C, P, S, L = 4, 128, 9000, 1000
+    p = mx.RadarPipeline(P, S, L, C)
+    p.GetInputView()[:, :, :S] = dev(x)
+    p.GetwaveformView()[:L] = dev(w)
+    st = orc.radar_pipeline(x[None], w, mode="matx", stages=True)
+    p.PulseCompression()
+    got = host(p.GetInputView())[:, :, :8001]
+    for c in range(C):
+        assert rel(got[c], st["y"][0]) < RTOL
+    p.ThreePulseCanceller()
+    tp = host(p.GetTPCView())
+    for c in range(C):
+        assert rel(tp[c, :P], st["tpc"][0]) < RTOL
+        assert np.all(tp[c, P:] == 0)
+    p.DopplerProcessing()
+    dop = host(p.GetTPCView())
+    for c in range(C):
+        assert rel(dop[c], st["D"][0]) < RTOL
+    p.CFARDetections()
+    dets = host(p.GetDetections())
+    ba = host(p.GetBackgroundAverages())[:, 2:258, 6:8007]
+    ref = {"dets": st["dets"][0], "margin": st["margin"][0]}
+    for c in range(C):
+        assert rel(ba[c], st["ba"][0]) < RTOL
+        bad, _ = dets_match(dets[c], ref)
+        assert bad == 0
+        assert np.array_equal(np.argwhere(dets[c] == 1), golden["matx_det_idx"])
+    nt = host(p.GetnormT())[0, 2:258, 6:8007]
+    assert np.array_equal(nt, st["norm"][0])

@@ -1,0 +1,141 @@
+"""CPU-side checks of the native library: it loads, exports every entry point
+include/mxr.h declares, and rejects bad calls on the host before any launch
+(the MatX size/shape checks that throw matxException, e.g.
+test/00_transform/FFT.cu:661-719).  No call here reaches a GPU: every case
+fails validation, which runs before any HIP call."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "mxr.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mxr_\w+)\s*\(", text, re.M)))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from matx_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmxr.so not built (run python -m matx_amd.build)")
+    return _lib
+
+
+def fake(L, shape, dtype, addr=0x10000, strides=None):
+    d = L.mxr_tensor()
+    d.data = addr
+    d.dtype = dtype
+    d.rank = len(shape)
+    if strides is None:
+        strides, s = [], 1
+        for n in reversed(shape):
+            strides.insert(0, s)
+            s *= n
+    for i, (n, st) in enumerate(zip(shape, strides)):
+        d.shape[i] = n
+        d.stride[i] = st
+    return d
+
+
+def test_exports_every_declared_symbol(L):
+    syms = declared_symbols()
+    assert len(syms) >= 10
+    assert set(syms) == set(L.EXPORTS)
+    lib = L.lib()
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_version_and_last_error(L):
+    assert L.lib().mxr_version() >= 100
+    assert isinstance(L.lib().mxr_last_error(), bytes)
+
+
+def status(L, fn, *args):
+    r = fn(*args)
+    msg = L.lib().mxr_last_error().decode()
+    return r, msg
+
+
+def test_fft_size_checks(L):
+    lib = L.lib()
+    tc = fake(L, [16], L.C64)
+    # C2C output larger than input without an explicit size: no implicit pad
+    t2 = fake(L, [32], L.C64, addr=0x90000)
+    r, msg = status(L, lib.mxr_fft_c2c, ctypes.byref(t2), ctypes.byref(tc), 0, 0, 0, 0, None)
+    assert r == 4 and "does not match" in msg
+    # output smaller than the transform
+    t3 = fake(L, [9], L.C64, addr=0x90000)
+    r, _ = status(L, lib.mxr_fft_c2c, ctypes.byref(t3), ctypes.byref(tc), 0, 0, 0, 0, None)
+    assert r == 4
+    # non-power-of-two size is reported as unsupported
+    t5 = fake(L, [12], L.C64, addr=0x90000)
+    r, _ = status(L, lib.mxr_fft_c2c, ctypes.byref(t5), ctypes.byref(tc), 12, 0, 0, 0, None)
+    assert r == 1
+    # wrong dtype
+    tf = fake(L, [16], L.F32, addr=0x90000)
+    r, _ = status(L, lib.mxr_fft_c2c, ctypes.byref(tf), ctypes.byref(tc), 0, 0, 0, 0, None)
+    assert r == 6
+    # partial overlap is an alias error
+    ta = fake(L, [16], L.C64, addr=0x10000 + 8)
+    r, _ = status(L, lib.mxr_fft_c2c, ctypes.byref(ta), ctypes.byref(tc), 0, 0, 0, 0, None)
+    assert r == 8
+
+
+def test_conv_size_checks(L):
+    lib = L.lib()
+    x = fake(L, [4], L.F32)
+    f = fake(L, [3], L.F32, addr=0x20000)
+    bad = fake(L, [5], L.F32, addr=0x30000)
+    r, _ = status(L, lib.mxr_conv1d, ctypes.byref(bad), ctypes.byref(x), ctypes.byref(f), 0, None)
+    assert r == 4                                # FULL needs 6
+    cplx_out = fake(L, [6], L.F32, addr=0x30000)
+    fc = fake(L, [3], L.C64, addr=0x20000)
+    r, _ = status(L, lib.mxr_conv1d, ctypes.byref(cplx_out), ctypes.byref(x), ctypes.byref(fc), 0,
+                  None)
+    assert r == 6                                # complex filter needs complex output
+    x2 = fake(L, [3, 3], L.F32)
+    f2 = fake(L, [2, 2], L.F32, addr=0x20000)
+    o2 = fake(L, [3, 3], L.F32, addr=0x30000)
+    r, _ = status(L, lib.mxr_conv2d, ctypes.byref(o2), ctypes.byref(x2), ctypes.byref(f2), 0, None)
+    assert r == 4                                # FULL needs 4x4
+
+
+def test_pipeline_stage_checks(L):
+    lib = L.lib()
+    y = fake(L, [2, 8, 100], L.C64)
+    pw = fake(L, [2, 4, 100], L.F32, addr=0x900000)
+    # a 4-point Doppler FFT cannot hold 8 canceller rows
+    r, msg = status(L, lib.mxr_doppler, ctypes.byref(pw), None, ctypes.byref(y), 1, 6, 4, None)
+    assert r == 4 and "shorter" in msg
+    r, _ = status(L, lib.mxr_doppler, None, None, ctypes.byref(y), 1, 6, 8, None)
+    assert r == 2                                # no output
+    inp = fake(L, [2, 8, 100], L.F32)
+    dets = fake(L, [2, 8, 100], L.I32, addr=0x900000)
+    p = L.mxr_cfar_params(5, 13, 4, 3, 1e-5)     # even guard
+    r, _ = status(L, lib.mxr_cfar, ctypes.byref(dets), None, None, ctypes.byref(inp),
+                  ctypes.byref(p), None)
+    assert r == 2
+    p = L.mxr_cfar_params(5, 13, 3, 3, 1.5)      # pfa out of range
+    r, _ = status(L, lib.mxr_cfar, ctypes.byref(dets), None, None, ctypes.byref(inp),
+                  ctypes.byref(p), None)
+    assert r == 2
+    x = fake(L, [4, 9000], L.C64)
+    w = fake(L, [1000], L.C64, addr=0x9000000)
+    out = fake(L, [4, 20000], L.C64, addr=0xA000000)
+    r, msg = status(L, lib.mxr_pulse_compress, ctypes.byref(out), ctypes.byref(x), ctypes.byref(w),
+                    16384, 3, None)
+    assert r == 4 and "exceed" in msg
+    r, _ = status(L, lib.mxr_pulse_compress, ctypes.byref(out), ctypes.byref(x), ctypes.byref(w),
+                  10000, 3, None)
+    assert r == 1                                # n not a power of two
+
+
+def test_header_matches_python_constants(L):
+    text = open(os.path.join(ROOT, "include", "mxr.h")).read()
+    assert "MXR_TPC_NONE 3" in text and L.TPC_NONE == 3
+    assert "MXR_PC_WRITEBACK 4" in text and L.PC_WRITEBACK == 4
