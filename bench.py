@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Radar-pipeline throughput on one MI355X.
+
+Workload (one step = one cube): 16 channels x 128 pulses x 9000 samples,
+complex64, resident in HBM; 1000-sample LFM (bandwidth 0.4 fs, Hamming applied
+by the matched filter); noise sigma = 1 plus 20 point targets (seeded range,
+Doppler, SNR 0-20 dB).  Pulse compression (N = 16384 circular correlation,
+lags 0..8000) -> three-pulse canceller [1,-2,1] (126 rows) -> Hamming(126) +
+128-point Doppler FFT -> |.|^2 -> CA-CFAR (5x13 window, 3x3 guard hole,
+Pfa 1e-5) -> int32 detections.
+
+The cube sits in HBM in MatX's inputView layout {16, 128, 16384} (pulses
+padded to the FFT size; only the 9000 samples are read); intermediate rows
+are padded to 64 elements.
+
+Prints one JSON line: cubes/s, the pulse-compression kernel's roofline from
+HIP events recorded around every launch of it inside the timed region, and
+the FP64 CPU oracle timed on one channel of the same cube (also used there
+as the parity check for that channel).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="wall budget of the CPU baseline leg (0 disables it)")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="time without per-kernel events (roofline omitted)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 5 plain 512 MiB device copies (scale reference for the "
+                         "FETCH_SIZE / WRITE_SIZE counters) and exit")
+    return ap.parse_args()
+
+
+def synth_cube(C, P, S, L, nd, n_targets, snr_db, seed, device):
+    """Noise (E|n|^2 = 1) plus point-target echoes of an unwindowed LFM."""
+    import torch
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    # MatX's inputView layout {C, P, N}: pulses padded to the FFT size, only
+    # the first S samples hold data (and only those are read)
+    N = 1
+    while N < S:
+        N *= 2
+    xbuf = torch.zeros((C, P, N), dtype=torch.complex64, device=device)
+    x = xbuf[:, :, :S]
+    x.copy_(torch.randn((C, P, S), dtype=torch.complex64, device=device,
+                        generator=torch.Generator(device=device).manual_seed(seed)))
+    n = torch.arange(L, dtype=torch.float64)
+    bw = 0.4
+    ph = 2 * math.pi * (-bw / 2 * n + 0.5 * bw / L * n * n)
+    lfm = torch.polar(torch.ones(L, dtype=torch.float64), ph)
+    targets = []
+    p = torch.arange(P, dtype=torch.float64)
+    for _ in range(n_targets):
+        rng_bin = int(torch.randint(0, S - L + 1, (1,), generator=g))
+        dop = int(torch.randint(0, nd, (1,), generator=g))
+        snr = float(torch.rand(1, generator=g)) * (snr_db[1] - snr_db[0]) + snr_db[0]
+        phase = float(torch.rand(1, generator=g)) * 2 * math.pi
+        amp = 10 ** (snr / 20)
+        ramp = torch.polar(torch.full((P,), amp, dtype=torch.float64),
+                           2 * math.pi * dop * p / nd + phase)
+        echo = (ramp[:, None] * lfm[None, :]).to(torch.complex64).to(device)
+        x[:, :, rng_bin:rng_bin + L] += echo[None]
+        targets.append((rng_bin, dop, snr))
+    return x, lfm.to(torch.complex64).to(device), targets
+
+
+def cpu_baseline_leg(x0, w, pow0, dets0, nd, budget_s, C):
+    """Times the FP64 oracle (C, OpenMP) on channel 0 of the cube, and checks
+    the GPU's channel 0 against it."""
+    import numpy as np
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as orc
+    orc.build()
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    reps, t0 = 0, time.perf_counter()
+    st = None
+    while True:
+        st = orc.radar_pipeline(x0[None], w, Nd=nd, mode="model", stages=True)
+        reps += 1
+        if time.perf_counter() - t0 >= budget_s or reps >= 2000:
+            break
+    el = time.perf_counter() - t0
+    pw_ref = st["pow"][0]
+    rel = float(np.linalg.norm(pow0 - pw_ref) / np.linalg.norm(pw_ref))
+    near = np.abs(st["margin"][0]) < 1e-5
+    mism = int(((dets0 != st["dets"][0]) & ~near).sum())
+    cpu = {"value": reps / C / el, "unit": "cubes/s", "cores": cores, "kind": "port",
+           "sample": f"FP64 oracle (oracle/mxr_oracle.c, OpenMP) on channel 0 of the "
+                     f"same cube (1/{C} cube), {reps} runs in {el:.1f} s"}
+    parity = {"channel": 0, "pow_rel_l2_vs_fp64": rel, "det_mismatch_outside_1e-5": mism,
+              "dets_gpu": int(dets0.sum()), "dets_oracle": int(st["dets"][0].sum())}
+    return cpu, parity
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "pc_ols_kernel_radar16x128x9000_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by channel with no exchange; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if args.calibrate_copy:
+        a = torch.ones(128 << 20, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": a.numel() * 4, "copies": 5}))
+        return
+    import matx_amd
+    from matx_amd import _lib
+
+    dev = "cuda:0"
+    C, P, S, L = 16, 128, 9000, 1000
+    nd = 128
+    x, w, targets = synth_cube(C, P, S, L, nd, 20, (0.0, 20.0), args.seed, dev)
+    pipe = matx_amd.FusedRadarPipeline(C, P, S, L, nd=nd, tpc="valid")
+    for _ in range(args.warmup):
+        pipe.step(x, w)
+    torch.cuda.synchronize()
+
+    lib = _lib.lib()
+    prof = not args.no_profile
+    if prof:
+        _lib.check(lib.mxr_profile_reset())
+        _lib.check(lib.mxr_profile_enable(1))
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        pipe.step(x, w)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    ms = e0.elapsed_time(e1) / args.steps
+    if prof:
+        _lib.check(lib.mxr_profile_enable(0))
+
+    B = pipe.algorithmic_bytes()
+    value = 1000.0 / ms
+    kern = {}
+    if prof:
+        for k in ("wfspec_kernel", "pc_ols_kernel", "doppler_kernel", "cfar_kernel"):
+            n, tot = _lib.profile_query(k)
+            kern[k] = {"launches": n, "avg_us": 1000.0 * tot / max(n, 1)}
+    R = S - L + 1
+    pc_bytes = 8 * C * P * S + 8 * C * P * R          # read x, write compressed cube
+    roofline = None
+    if prof and kern["pc_ols_kernel"]["launches"]:
+        avg_s = kern["pc_ols_kernel"]["avg_us"] * 1e-6
+        ach = pc_bytes / avg_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": load_traffic(), "kernel": "pc_ols_kernel<4096>",
+                    "algorithmic_bytes_per_launch": pc_bytes}
+
+    dets = pipe.dets
+    det_cells = int(dets.sum().item())
+    # every injected target with a Doppler bin away from zero survives the
+    # canceller; count how many land on a detection at (Doppler, range)
+    d_host = dets.cpu()
+    hit = sum(int(d_host[:, dop, rb].any()) for rb, dop, _ in targets)
+
+    cpu, parity = None, None
+    if args.cpu_seconds > 0:
+        cpu, parity = cpu_baseline_leg(x[0].cpu().numpy(), w.cpu().numpy(),
+                                       pipe.pw[0].cpu().numpy(), d_host[0].numpy(), nd,
+                                       args.cpu_seconds, C)
+
+    out = {
+        "metric": "cubes/s (16x128x9000 c64 radar pipeline)",
+        "value": round(value, 2),
+        "unit": "cubes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic (noise + 20 LFM point targets, seeded)",
+        "config": {"workload": "radar pipeline 16ch x 128 pulses x 9000 samples c64, "
+                               "L=1000 LFM, N=16384 pulse compression, [1,-2,1] canceller "
+                               "(126 rows), Hamming + 128-pt Doppler FFT, CA-CFAR 5x13 "
+                               "window / 3x3 guard hole, Pfa 1e-5",
+                   "channels": C, "pulses": P, "samples": S, "waveform": L, "doppler": nd,
+                   "algorithmic_bytes_per_cube": B,
+                   "pipeline_hbm_frac": round(B * value / (HBM_PEAK_GBS * 1e9), 4),
+                   "per_kernel_us": {k: round(v["avg_us"], 2) for k, v in kern.items()},
+                   "detections": det_cells, "targets_detected": f"{hit}/{len(targets)}",
+                   "parity": parity},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

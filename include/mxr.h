@@ -74,6 +74,16 @@ int mxr_version(void);
  * (cf. ClearCachesAndAllocations, test/main.cu). */
 mxr_status mxr_clear_caches(void);
 
+/* Kernel timing, the analogue of the executor timers start_timer /
+ * stop_timer / get_time_ms (include/matx/executors/cuda_executor_common.h:
+ * 142-169).  While enabled, every kernel the library launches is bracketed by
+ * a pair of hipEvents recorded on the stream it is launched on.
+ * mxr_profile_query sums the recorded launches of kernels whose name contains
+ * `name` (waiting for their events); mxr_profile_reset drops the records. */
+mxr_status mxr_profile_enable(int on);
+mxr_status mxr_profile_reset(void);
+mxr_status mxr_profile_query(const char *name, int64_t *count, double *total_ms);
+
 /* ------------------------------------------------------------------ FFT */
 /* Batched 1-D complex-to-complex FFT along `axis`.
  * n = transform size; 0 means in->shape[axis].  The input is zero-padded or

@@ -21,7 +21,8 @@ FORWARD, INVERSE = 0, 1
 PC_WINDOW, PC_NORMALIZE, PC_WRITEBACK = 1, 2, 4
 
 # Every entry point include/mxr.h declares.
-EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_fft_c2c",
+EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_enable",
+           "mxr_profile_reset", "mxr_profile_query", "mxr_fft_c2c",
            "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar",
            "mxr_conv1d", "mxr_conv2d")
 
@@ -72,6 +73,13 @@ def lib():
     L.mxr_last_error.argtypes = []
     L.mxr_version.restype = I
     L.mxr_clear_caches.restype = I
+    L.mxr_profile_enable.argtypes = [I]
+    L.mxr_profile_enable.restype = I
+    L.mxr_profile_reset.argtypes = []
+    L.mxr_profile_reset.restype = I
+    L.mxr_profile_query.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
+                                    ctypes.POINTER(ctypes.c_double)]
+    L.mxr_profile_query.restype = I
     sigs = {
         "mxr_fft_c2c": [T, T, I64, I, I, I, V],
         "mxr_fft_conv": [T, T, T, I64, I, V],
@@ -117,6 +125,15 @@ def desc(t):
 
 def ref(d):
     return ctypes.byref(d) if d is not None else None
+
+
+def profile_query(name: str):
+    """(launch count, total ms) of recorded launches whose kernel name
+    contains `name`."""
+    n = ctypes.c_int64()
+    t = ctypes.c_double()
+    check(lib().mxr_profile_query(name.encode(), ctypes.byref(n), ctypes.byref(t)))
+    return n.value, t.value
 
 
 def stream_handle(stream=None):

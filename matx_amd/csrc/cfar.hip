@@ -14,12 +14,17 @@
 // ba = sum/norm (IEEE division, as MatX divides), alpha = norm*(pfa^(-1/norm)-1)
 // in float as calcDets evaluates it, det = pow > alpha*ba.
 //
-// Tiling: a 256-thread workgroup owns TILE_Y x TILE_X output cells; it stages
-// (TILE_Y + 2HY) x (TILE_X + 2*HALO) input cells in LDS (zero outside the
-// map), each thread then produces RPT rows x CPT columns.
+// Tiling: a one-wave workgroup owns 256 consecutive columns x 16 rows of one
+// channel.  It stages the (16 + 2HY) x (256 + 2*HALO) input cells in LDS with
+// 16-byte loads (zero outside the map); each lane then walks down its four
+// adjacent columns, forming the horizontal window sums of the four columns
+// with shared partial sums, keeping the last 2HY+1 rows in registers, and
+// writes each detection row as one 16-byte store per lane.
 #include <cmath>
+#include <type_traits>
 
 #include "mxr_internal.hpp"
+#include "mxr_cache.hpp"
 
 namespace mxr {
 
@@ -34,113 +39,237 @@ struct CfarArgs {
   float *pwo;
   int64_t os_c, os_y, os_x;
   int C, Ny, Nx;
-  int ey, ex;
   float pfa;
+  int vec_in;    // f32 input, unit column stride, 16-byte aligned rows
+  int vec_dets;  // detections: unit column stride, 16-byte aligned rows
 };
 
-constexpr int CF_NT = 256;
-constexpr int CF_CPT = 4;                   // columns per thread
-constexpr int CF_TX = 64 * CF_CPT;          // tile width (one wave across)
-constexpr int CF_RPT = 8;                   // rows per thread
-constexpr int CF_TY = (CF_NT / 64) * CF_RPT;  // tile height
+constexpr int CF_NT = 64;             // one wave per workgroup
+constexpr int CF_CPL = 4;             // adjacent columns per lane
+constexpr int CF_TX = CF_NT * CF_CPL;  // 256 columns per tile
+constexpr int CF_TY = 16;             // output rows per tile
 
-template <int HY, int HX>
+// Sums of the four windows u[j .. j+W-1], j = 0..3, sharing partial sums
+// (positive terms only, any association is exact up to rounding).
+template <int W>
+__device__ __forceinline__ void win4(const float *u, float (&o)[4]) {
+  if constexpr (W <= 0) {
+    o[0] = o[1] = o[2] = o[3] = 0.f;
+  } else if constexpr (W == 1) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = u[j];
+  } else if constexpr (W == 2) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) o[j] = u[j] + u[j + 1];
+  } else if constexpr (W == 3) {
+    const float p1 = u[1] + u[2], p3 = u[3] + u[4];
+    o[0] = u[0] + p1;
+    o[1] = p1 + u[3];
+    o[2] = u[2] + p3;
+    o[3] = p3 + u[5];
+  } else {
+    float core = u[3];  // u[3 .. W-1] is inside all four windows
+#pragma unroll
+    for (int i = 4; i < W; ++i) core += u[i];
+    const float d = u[1] + u[2], e = u[W] + u[W + 1];
+    o[0] = (u[0] + d) + core;
+    o[1] = (d + core) + u[W];
+    o[2] = (u[2] + core) + e;
+    o[3] = (core + e) + u[W + 2];
+  }
+}
+
+// HY/HX: half window (rows/cols); EY/EX: half guard hole.  CIN: input is
+// complex (|.|^2 on load).  EXACT: ba = s/norm (IEEE division, written) and
+// det = pow > alpha*ba, MatX's operation order; otherwise det = pow >
+// (alpha/norm)*s with alpha/norm tabulated per norm count (the same
+// threshold up to a few ulp of float rounding).  WPOW: also write pow.
+template <int HY, int HX, int EY, int EX, bool CIN, bool EXACT, bool WPOW>
 __global__ __launch_bounds__(CF_NT) void cfar_kernel(CfarArgs a) {
-  constexpr int HALO = (HX + 3) & ~3;  // keep the LDS row 16-byte aligned
+  constexpr int HALO = (HX + 3) & ~3;  // keeps LDS rows and reads 16-byte aligned
   constexpr int LW = CF_TX + 2 * HALO;
   constexpr int LH = CF_TY + 2 * HY;
   constexpr int NCELL = (2 * HY + 1) * (2 * HX + 1);
+  constexpr int RING = 2 * HY + 1;
+  constexpr int NV = CF_CPL + 2 * HALO;  // values a lane reads per row
   __shared__ __attribute__((aligned(16))) float tile[LH * LW];
-  __shared__ float alpha_tab[NCELL + 1];
+  __shared__ float thr_tab[NCELL + 1];
 
   const int c = blockIdx.z;
   const int y0 = blockIdx.y * CF_TY;
   const int x0 = blockIdx.x * CF_TX;
+  const int lane = threadIdx.x;
+  const int Nx = a.Nx, Ny = a.Ny;
 
-  for (int n = threadIdx.x; n <= NCELL; n += CF_NT) {
+  for (int n = lane; n <= NCELL; n += CF_NT) {
     const float nf = (float)n;
-    alpha_tab[n] = nf * (powf(a.pfa, -1.0f / nf) - 1.0f);
+    const float alpha = nf * (powf(a.pfa, -1.0f / nf) - 1.0f);
+    thr_tab[n] = EXACT ? alpha : alpha / nf;
   }
-  // stage the tile (zero padding outside the map)
-  for (int e = threadIdx.x; e < LH * LW; e += CF_NT) {
-    const int ly = e / LW, lx = e % LW;
-    const int gy = y0 - HY + ly, gx = x0 - HALO + lx;
-    float v = 0.f;
-    if (gy >= 0 && gy < a.Ny && gx >= 0 && gx < a.Nx) {
-      const int64_t o = (int64_t)c * a.is_c + (int64_t)gy * a.is_y + (int64_t)gx * a.is_x;
-      if (a.in_c64) {
-        const float2 z = reinterpret_cast<const float2 *>(a.in)[o];
-        v = z.x * z.x + z.y * z.y;
-      } else {
-        v = reinterpret_cast<const float *>(a.in)[o];
+
+  using T = typename std::conditional<CIN, float2, float>::type;
+  const T *const in_c = reinterpret_cast<const T *>(a.in) + (int64_t)c * a.is_c;
+  auto power = [](T z) -> float {
+    if constexpr (CIN) return z.x * z.x + z.y * z.y;
+    else return z;
+  };
+  const int isy = (int)a.is_y, isx = (int)a.is_x;
+  // Stage rows y0-HY .. y0+TY+HY-1, columns x0-HALO .. x0+TX+HALO-1 (zero
+  // outside the map).  Lane l owns tile columns 4l..4l+3; lanes < HALO/2
+  // also fetch the halo quads.  Every load is issued before the LDS writes.
+  {
+    constexpr int NQ = CF_NT + HALO / 2;  // quads per row (body + halo)
+    for (int q = lane; q < NQ; q += CF_NT) {
+      // quad q < 64: body columns 4q..; else halo quads left then right
+      int tcol;
+      if (q < CF_NT) tcol = HALO + 4 * q;
+      else {
+        const int h = q - CF_NT;
+        tcol = h < HALO / 4 ? 4 * h : HALO + CF_TX + 4 * (h - HALO / 4);
       }
+      const int gx = x0 - HALO + tcol;
+      const bool vec = a.vec_in && gx >= 0 && gx + 3 < Nx;
+      float4 v[LH];
+#pragma unroll
+      for (int ly = 0; ly < LH; ++ly) {
+        const int gy = y0 - HY + ly;
+        const bool rok = gy >= 0 && gy < Ny;
+        const int gyc = min(max(gy, 0), Ny - 1);
+        float r[4];
+        if (vec) {
+          const float4 z = *reinterpret_cast<const float4 *>(
+              reinterpret_cast<const float *>(in_c) + gyc * isy + gx);
+          r[0] = z.x; r[1] = z.y; r[2] = z.z; r[3] = z.w;
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int x = gx + j;
+            const float z = power(in_c[gyc * isy + min(max(x, 0), Nx - 1) * isx]);
+            r[j] = (x >= 0 && x < Nx) ? z : 0.f;
+          }
+        }
+        v[ly] = rok ? make_float4(r[0], r[1], r[2], r[3]) : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int ly = 0; ly < LH; ++ly) *reinterpret_cast<float4 *>(&tile[ly * LW + tcol]) = v[ly];
     }
-    tile[e] = v;
   }
   __syncthreads();
 
-  const int tx = threadIdx.x % 64, ty = threadIdx.x / 64;
-  const int cx0 = tx * CF_CPT;        // first tile column of this thread
-  const int ry0 = ty * CF_RPT;        // first tile row of this thread
-  const int ex = a.ex, ey = a.ey;
-
-  // horizontal partial sums for RPT + 2HY input rows
-  float full[CF_RPT + 2 * HY][CF_CPT], lr[CF_RPT + 2 * HY][CF_CPT];
+  const int gx0 = x0 + CF_CPL * lane;  // first column of this lane
+  if (gx0 >= Nx) return;
+  // in-bounds cell count = ro*co - rh*chh: co/chh per column, ro/rh per row
+  int co[CF_CPL], chh[CF_CPL];
+  float thr_int[CF_CPL];
 #pragma unroll
-  for (int rr = 0; rr < CF_RPT + 2 * HY; ++rr) {
-    const float *row = tile + (ry0 + rr) * LW + HALO + cx0;
-    float v[CF_CPT + 2 * HX];
-#pragma unroll
-    for (int i = 0; i < CF_CPT + 2 * HX; ++i) v[i] = row[i - HX];
-#pragma unroll
-    for (int j = 0; j < CF_CPT; ++j) {
-      float l = 0.f, m = 0.f, r = 0.f;
-#pragma unroll
-      for (int d = -HX; d <= HX; ++d) {
-        const float w = v[j + HX + d];
-        if (d < -ex) l += w;
-        else if (d <= ex) m += w;
-        else r += w;
-      }
-      lr[rr][j] = l + r;
-      full[rr][j] = (l + m) + r;
-    }
+  for (int j = 0; j < CF_CPL; ++j) {
+    const int gx = gx0 + j;
+    co[j] = min(gx + HX, Nx - 1) - max(gx - HX, 0) + 1;
+    chh[j] = min(gx + EX, Nx - 1) - max(gx - EX, 0) + 1;
+    thr_int[j] = thr_tab[min(max((2 * HY + 1) * co[j] - (2 * EY + 1) * chh[j], 0), NCELL)];
   }
+  const bool vec_out = a.vec_dets && gx0 + 3 < Nx;
+  const int dsy = (int)a.ds_y, dsx = (int)a.ds_x;
+  int *const dets_c = a.dets + (int64_t)c * a.ds_c;
 
+  float full[RING][CF_CPL], lr[RING][CF_CPL], ctr[RING][CF_CPL];
 #pragma unroll
-  for (int rr = 0; rr < CF_RPT; ++rr) {
-    const int gy = y0 + ry0 + rr;
-    if (gy >= a.Ny) break;
-    const int ro = min(gy + HY, a.Ny - 1) - max(gy - HY, 0) + 1;
-    const int rh = min(gy + ey, a.Ny - 1) - max(gy - ey, 0) + 1;
+  for (int ly = 0; ly < LH; ++ly) {
+    // values of columns gx0-HALO .. gx0+3+HALO of this row
+    float u[NV];
+    const float4 *row4 = reinterpret_cast<const float4 *>(&tile[ly * LW + CF_CPL * lane]);
 #pragma unroll
-    for (int j = 0; j < CF_CPT; ++j) {
-      const int gx = x0 + cx0 + j;
-      if (gx >= a.Nx) continue;
+    for (int i = 0; i < NV / 4; ++i) {
+      const float4 z = row4[i];
+      u[4 * i] = z.x; u[4 * i + 1] = z.y; u[4 * i + 2] = z.z; u[4 * i + 3] = z.w;
+    }
+    // u[HALO + j + d] is column gx0 + j + d
+    float L[4], Mh[4], Rt[4];
+    win4<HX - EX>(u + HALO - HX, L);
+    win4<2 * EX + 1>(u + HALO - EX, Mh);
+    win4<HX - EX>(u + HALO + EX + 1, Rt);
+    const int k = ly % RING;
+#pragma unroll
+    for (int j = 0; j < CF_CPL; ++j) {
+      lr[k][j] = L[j] + Rt[j];
+      full[k][j] = lr[k][j] + Mh[j];
+      ctr[k][j] = u[HALO + j];
+    }
+    if (ly < 2 * HY) continue;
+    const int oy = ly - HY;  // tile row of the cells under test
+    const int gy = y0 + oy - HY;
+    if (gy >= Ny) break;
+    float thr[CF_CPL];
+    int nrm[CF_CPL];
+    const bool edge = gy < HY || gy + HY >= Ny;
+    const int ro = min(gy + HY, Ny - 1) - max(gy - HY, 0) + 1;
+    const int rh = min(gy + EY, Ny - 1) - max(gy - EY, 0) + 1;
+#pragma unroll
+    for (int j = 0; j < CF_CPL; ++j) {
+      nrm[j] = ro * co[j] - rh * chh[j];
+      // columns past Nx (never stored) can give a meaningless count: clamp
+      // the table index
+      thr[j] = edge ? thr_tab[min(max(nrm[j], 0), NCELL)] : thr_int[j];
+    }
+    int det[CF_CPL];
+#pragma unroll
+    for (int j = 0; j < CF_CPL; ++j) {
       float s = 0.f;
 #pragma unroll
       for (int dy = -HY; dy <= HY; ++dy) {
-        const bool hole_row = dy >= -ey && dy <= ey;
-        s += hole_row ? lr[rr + HY + dy][j] : full[rr + HY + dy][j];
+        const int kk = (oy + dy) % RING;
+        const float t = (dy >= -EY && dy <= EY) ? lr[kk][j] : full[kk][j];
+        s = dy == -HY ? t : s + t;
       }
-      const int co = min(gx + HX, a.Nx - 1) - max(gx - HX, 0) + 1;
-      const int chh = min(gx + ex, a.Nx - 1) - max(gx - ex, 0) + 1;
-      const int nrm = ro * co - rh * chh;
-      const float normf = (float)nrm;
-      const float b = s / normf;
-      const float p = tile[(ry0 + rr + HY) * LW + HALO + cx0 + j];
-      const int det = p > alpha_tab[nrm] * b ? 1 : 0;
-      a.dets[(int64_t)c * a.ds_c + (int64_t)gy * a.ds_y + (int64_t)gx * a.ds_x] = det;
-      if (a.ba) a.ba[(int64_t)c * a.bs_c + (int64_t)gy * a.bs_y + (int64_t)gx * a.bs_x] = b;
-      if (a.pwo) a.pwo[(int64_t)c * a.os_c + (int64_t)gy * a.os_y + (int64_t)gx * a.os_x] = p;
+      const float p = ctr[oy % RING][j];
+      if constexpr (EXACT) {
+        const float b = s / (float)nrm[j];
+        det[j] = p > thr[j] * b ? 1 : 0;
+        if (gx0 + j < Nx)
+          a.ba[(int64_t)c * a.bs_c + (int64_t)gy * a.bs_y + (int64_t)(gx0 + j) * a.bs_x] = b;
+      } else {
+        det[j] = p > thr[j] * s ? 1 : 0;
+      }
+      if constexpr (WPOW) {
+        if (gx0 + j < Nx)
+          a.pwo[(int64_t)c * a.os_c + (int64_t)gy * a.os_y + (int64_t)(gx0 + j) * a.os_x] = p;
+      }
+    }
+    int *drow = dets_c + gy * dsy;
+    if (vec_out) {
+      *reinterpret_cast<int4 *>(drow + gx0) = make_int4(det[0], det[1], det[2], det[3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < CF_CPL; ++j)
+        if (gx0 + j < Nx) drow[(gx0 + j) * dsx] = det[j];
     }
   }
 }
 
-template <int HY, int HX>
+template <int HY, int HX, int EY, int EX, bool CIN, bool EXACT, bool WPOW>
+static void launch_cfar3(const CfarArgs &a, dim3 grid, hipStream_t s) {
+  cfar_kernel<HY, HX, EY, EX, CIN, EXACT, WPOW><<<grid, dim3(CF_NT), 0, s>>>(a);
+}
+
+template <int HY, int HX, int EY, int EX>
 static mxr_status launch_cfar(const CfarArgs &a, hipStream_t s) {
+  static_assert(HX <= 16 && HY <= 10, "window");
   dim3 grid((a.Nx + CF_TX - 1) / CF_TX, (a.Ny + CF_TY - 1) / CF_TY, a.C);
-  cfar_kernel<HY, HX><<<grid, dim3(CF_NT), 0, s>>>(a);
+  {
+    ProfScope ps("cfar_kernel", s);
+    const bool ex = a.ba != nullptr, wp = a.pwo != nullptr;
+    if (a.in_c64) {
+      if (ex) wp ? launch_cfar3<HY, HX, EY, EX, true, true, true>(a, grid, s)
+              : launch_cfar3<HY, HX, EY, EX, true, true, false>(a, grid, s);
+      else wp ? launch_cfar3<HY, HX, EY, EX, true, false, true>(a, grid, s)
+              : launch_cfar3<HY, HX, EY, EX, true, false, false>(a, grid, s);
+    } else {
+      if (ex) wp ? launch_cfar3<HY, HX, EY, EX, false, true, true>(a, grid, s)
+              : launch_cfar3<HY, HX, EY, EX, false, true, false>(a, grid, s);
+      else wp ? launch_cfar3<HY, HX, EY, EX, false, false, true>(a, grid, s)
+              : launch_cfar3<HY, HX, EY, EX, false, false, false>(a, grid, s);
+    }
+  }
   MXR_HIP_TRY(hipGetLastError());
   return MXR_SUCCESS;
 }
@@ -171,6 +300,11 @@ mxr_status cfar_impl(const mxr_tensor *dets, const mxr_tensor *ba_out, const mxr
   if (C == 0 || Ny == 0 || Nx == 0) return MXR_SUCCESS;
   MXR_REQUIRE(C <= 65535 && Ny < (1 << 30) && Nx < (1 << 30), MXR_INVALID_SIZE,
               "cfar: shape too large");
+  auto chan_span = [&](const mxr_tensor *t) {
+    return std::llabs(t->stride[1]) * (Ny - 1) + std::llabs(t->stride[2]) * (Nx - 1);
+  };
+  MXR_REQUIRE(chan_span(in) < (1ll << 31) && chan_span(dets) < (1ll << 31), MXR_INVALID_SIZE,
+              "cfar: one channel of the input or detections spans more than 2^31 elements");
   CfarArgs a{};
   a.in = in->data;
   a.in_c64 = in->dtype == MXR_C64;
@@ -196,24 +330,32 @@ mxr_status cfar_impl(const mxr_tensor *dets, const mxr_tensor *ba_out, const mxr
   a.C = (int)C;
   a.Ny = (int)Ny;
   a.Nx = (int)Nx;
-  a.ey = gy / 2;
-  a.ex = gx / 2;
   a.pfa = pp->pfa;
-  const int hy = wy / 2, hx = wx / 2;
-#define MXR_CFAR_CASE(HY_, HX_) \
-  if (hy == HY_ && hx == HX_) return launch_cfar<HY_, HX_>(a, stream);
-  MXR_CFAR_CASE(2, 6)   // 5 x 13 (MatX mask)
-  MXR_CFAR_CASE(1, 1)
-  MXR_CFAR_CASE(1, 2)
-  MXR_CFAR_CASE(2, 2)
-  MXR_CFAR_CASE(2, 4)
-  MXR_CFAR_CASE(3, 3)
-  MXR_CFAR_CASE(3, 10)  // 7 x 21
-  MXR_CFAR_CASE(10, 3)  // 21 x 7
-  MXR_CFAR_CASE(4, 8)
+  auto aligned16 = [](const mxr_tensor *t) {
+    return ((uintptr_t)t->data % 16 == 0) && t->stride[2] == 1 && t->stride[1] % 4 == 0 &&
+           t->stride[0] % 4 == 0;
+  };
+  a.vec_in = in->dtype == MXR_F32 && aligned16(in);
+  a.vec_dets = aligned16(dets);
+  const int hy = wy / 2, hx = wx / 2, ey = gy / 2, ex = gx / 2;
+#define MXR_CFAR_CASE(HY_, HX_, EY_, EX_) \
+  if (hy == HY_ && hx == HX_ && ey == EY_ && ex == EX_) return launch_cfar<HY_, HX_, EY_, EX_>(a, stream);
+  MXR_CFAR_CASE(2, 6, 1, 1)   // 5 x 13 window, 3 x 3 hole (MatX mask)
+  MXR_CFAR_CASE(2, 6, 1, 2)   // 5 x 13 window, 3 x 5 hole
+  MXR_CFAR_CASE(2, 6, 0, 0)
+  MXR_CFAR_CASE(1, 1, 0, 0)
+  MXR_CFAR_CASE(1, 2, 0, 1)
+  MXR_CFAR_CASE(2, 2, 1, 1)
+  MXR_CFAR_CASE(2, 4, 1, 1)
+  MXR_CFAR_CASE(3, 3, 1, 1)
+  MXR_CFAR_CASE(3, 10, 1, 2)  // 7 x 21 window, 3 x 5 hole
+  MXR_CFAR_CASE(3, 10, 1, 1)
+  MXR_CFAR_CASE(10, 3, 2, 1)  // 21 x 7 window, 5 x 3 hole
+  MXR_CFAR_CASE(4, 8, 1, 1)
 #undef MXR_CFAR_CASE
   set_error(MXR_NOT_SUPPORTED,
-            "cfar: window %dx%d has no fused kernel instance (use conv2d + threshold)", wy, wx);
+            "cfar: window %dx%d / guard %dx%d has no fused kernel instance (use conv2d + "
+            "threshold)", wy, wx, gy, gx);
   return MXR_NOT_SUPPORTED;
 }
 

@@ -10,6 +10,7 @@
 #include <algorithm>
 
 #include "mxr_internal.hpp"
+#include "mxr_cache.hpp"
 
 namespace mxr {
 
@@ -172,7 +173,10 @@ static mxr_status conv_common(const char *fn, const mxr_tensor *out, const mxr_t
   if (a.total == 0) return MXR_SUCCESS;
   const int64_t blocks = (a.total + 255) / 256;
   MXR_REQUIRE(blocks <= 0x7fffffff, MXR_INVALID_SIZE, "%s: too large", fn);
-  conv_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(a);
+  {
+    ProfScope ps(two_d ? "conv2d_kernel" : "conv1d_kernel", stream);
+    conv_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(a);
+  }
   MXR_HIP_TRY(hipGetLastError());
   return MXR_SUCCESS;
 }

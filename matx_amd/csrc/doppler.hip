@@ -11,6 +11,7 @@
 // writes the range-Doppler map and/or its power with TR-wide coalesced rows.
 // HBM traffic per tile: P*TR*8 bytes in, nd*TR*(4 and/or 8) bytes out.
 #include "fft_engine.hpp"
+#include "mxr_cache.hpp"
 
 namespace mxr {
 
@@ -32,7 +33,9 @@ struct DopArgs {
 template <int ND>
 struct DopCfg {
   static constexpr int NT = 256;
-  static constexpr int TR = ND <= 128 ? 64 : 8192 / ND;  // 64 KiB of LDS at most
+  // 32 range bins (256-byte rows) per tile up to 128 Doppler bins, 32 KiB of
+  // LDS for 128 pulses: four workgroups per CU keep enough loads in flight
+  static constexpr int TR = ND <= 128 ? 32 : 4096 / ND;
 };
 
 template <int ND>
@@ -52,12 +55,28 @@ __global__ __launch_bounds__(256) void doppler_kernel(DopArgs a, int rows_buf) {
     if (i < a.rows_t) h = i < a.win_rows ? hamming_f32(i, a.win_rows) : 1.0f;
     win[i] = h;
   }
-  // coalesced tile load: rows of TR range bins
+  // coalesced tile load: rows of TR range bins, 16 loads in flight per lane
   const float2 *yc = a.y + (int64_t)c * a.ys_c;
-  for (int e = threadIdx.x; e < a.P * TR; e += NT) {
-    const int p = e / TR, t = e % TR;
+  {
+    constexpr int RPI = NT / TR;  // pulses covered by one sweep of the workgroup
+    constexpr int BATCH = 16;
+    const int t = threadIdx.x % TR, p0 = threadIdx.x / TR;
     const int r = r0 + t;
-    lds[p * TR + t] = r < a.R ? yc[(int64_t)p * a.ys_p + (int64_t)r * a.ys_r] : make_float2(0.f, 0.f);
+    const bool rok = r < a.R;
+    const float2 *col = yc + (int64_t)r * a.ys_r;
+    for (int pb = 0; pb < a.P; pb += RPI * BATCH) {
+      float2 v[BATCH];
+#pragma unroll
+      for (int u = 0; u < BATCH; ++u) {
+        const int p = pb + p0 + u * RPI;
+        v[u] = (rok && p < a.P) ? col[(int64_t)p * a.ys_p] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < BATCH; ++u) {
+        const int p = pb + p0 + u * RPI;
+        if (p < a.P) lds[p * TR + t] = v[u];
+      }
+    }
   }
   wg_barrier();
 
@@ -110,7 +129,10 @@ static mxr_status launch_doppler(const DopArgs &a, hipStream_t s) {
     MXR_HIP_TRY(hipFuncSetAttribute((const void *)doppler_kernel<ND>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   dim3 grid((a.R + TR - 1) / TR, a.C);
-  doppler_kernel<ND><<<grid, dim3(DopCfg<ND>::NT), lds, s>>>(a, rows_buf);
+  {
+    ProfScope ps("doppler_kernel", s);
+    doppler_kernel<ND><<<grid, dim3(DopCfg<ND>::NT), lds, s>>>(a, rows_buf);
+  }
   MXR_HIP_TRY(hipGetLastError());
   return MXR_SUCCESS;
 }

@@ -12,6 +12,7 @@
 #include <cmath>
 
 #include "fft_engine.hpp"
+#include "mxr_cache.hpp"
 
 namespace mxr {
 
@@ -99,6 +100,7 @@ static mxr_status launch_generic(const FftArgs &a, bool inv, hipStream_t s) {
   const int64_t grid = (a.nbatch + C::TB - 1) / C::TB;
   if (grid == 0) return MXR_SUCCESS;
   MXR_REQUIRE(grid <= 0x7fffffff, MXR_INVALID_SIZE, "fft: batch too large");
+  ProfScope ps("fft_generic_kernel", s);
   if (inv) fft_generic_kernel<N, true><<<dim3((unsigned)grid), dim3(C::NT), 0, s>>>(a);
   else fft_generic_kernel<N, false><<<dim3((unsigned)grid), dim3(C::NT), 0, s>>>(a);
   MXR_HIP_TRY(hipGetLastError());

@@ -8,4 +8,15 @@ void register_cache_clear(void (*fn)());
 struct CacheClearRegistrar {
   explicit CacheClearRegistrar(void (*fn)()) { register_cache_clear(fn); }
 };
+// Brackets one kernel launch with profiling events when profiling is on.
+#include <hip/hip_runtime.h>
+class ProfScope {
+ public:
+  ProfScope(const char *kernel, hipStream_t s);
+  ~ProfScope();
+
+ private:
+  int slot_ = -1;
+  hipStream_t s_;
+};
 }  // namespace mxr

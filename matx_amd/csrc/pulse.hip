@@ -220,6 +220,9 @@ mxr_status plan_pc(int64_t seff, int64_t L, int64_t n, int64_t nout, bool allow_
     } else {
       if (M < L || M > n) return;
       p.V = M - L + 1;
+      // keep whole 64-lag groups per block so every block's stores start on
+      // a 512-byte boundary of an aligned row
+      if (p.V >= 64) p.V = (p.V / 64) * 64;
       p.nblk = (nout + p.V - 1) / p.V;
       if (in_place) {
         // a block that wraps past n would read lags already overwritten
@@ -280,9 +283,13 @@ CacheClearRegistrar g_reg_wc(clear_wc);
 
 template <int M>
 mxr_status launch_pc(const WfArgs &wa, const PcArgs &pa, hipStream_t s) {
-  wfspec_kernel<M><<<dim3(1), dim3(PcCfg<M>::NT), 0, s>>>(wa);
+  {
+    ProfScope ps("wfspec_kernel", s);
+    wfspec_kernel<M><<<dim3(1), dim3(PcCfg<M>::NT), 0, s>>>(wa);
+  }
   MXR_HIP_TRY(hipGetLastError());
   if (pa.groups > 0) {
+    ProfScope ps("pc_ols_kernel", s);
     if (pa.bpw > 1)
       pc_ols_kernel<M, true><<<dim3((unsigned)pa.groups), dim3(PcCfg<M>::NT), 0, s>>>(pa);
     else

@@ -150,6 +150,40 @@ const float2 *twiddles_device(int N, bool rev) {
   return d;
 }
 
+// ------------------------------------------------------------ profiling
+namespace {
+struct ProfRec {
+  std::string name;
+  hipEvent_t a = nullptr, b = nullptr;
+  bool used = false;
+};
+std::mutex g_prof_mu;
+std::vector<ProfRec> g_prof;  // pool; records reused after reset
+size_t g_prof_n = 0;
+bool g_prof_on = false;
+}  // namespace
+
+ProfScope::ProfScope(const char *kernel, hipStream_t s) : s_(s) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  if (!g_prof_on) return;
+  if (g_prof_n == g_prof.size()) {
+    ProfRec r;
+    if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
+    g_prof.push_back(r);
+  }
+  ProfRec &r = g_prof[g_prof_n];
+  r.name = kernel;
+  r.used = true;
+  if (hipEventRecord(r.a, s) != hipSuccess) return;
+  slot_ = (int)g_prof_n++;
+}
+
+ProfScope::~ProfScope() {
+  if (slot_ < 0) return;
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  (void)hipEventRecord(g_prof[slot_].b, s_);
+}
+
 static void clear_twiddles() {
   std::lock_guard<std::mutex> g(g_tw_mu);
   for (auto &kv : g_tw) {
@@ -169,6 +203,40 @@ extern "C" {
 const char *mxr_last_error(void) { return mxr::g_last_error.c_str(); }
 
 int mxr_version(void) { return 100; }
+
+mxr_status mxr_profile_enable(int on) {
+  std::lock_guard<std::mutex> g(mxr::g_prof_mu);
+  mxr::g_prof_on = on != 0;
+  return MXR_SUCCESS;
+}
+
+mxr_status mxr_profile_reset(void) {
+  std::lock_guard<std::mutex> g(mxr::g_prof_mu);
+  for (size_t i = 0; i < mxr::g_prof_n; ++i) mxr::g_prof[i].used = false;
+  mxr::g_prof_n = 0;
+  return MXR_SUCCESS;
+}
+
+mxr_status mxr_profile_query(const char *name, int64_t *count, double *total_ms) {
+  return mxr::guarded("mxr_profile_query", [&]() -> mxr_status {
+    MXR_REQUIRE(name && count && total_ms, MXR_INVALID_PARAMETER, "profile_query: null argument");
+    std::lock_guard<std::mutex> g(mxr::g_prof_mu);
+    int64_t n = 0;
+    double t = 0.0;
+    for (size_t i = 0; i < mxr::g_prof_n; ++i) {
+      auto &r = mxr::g_prof[i];
+      if (r.name.find(name) == std::string::npos) continue;
+      MXR_HIP_TRY(hipEventSynchronize(r.b));
+      float ms = 0.f;
+      MXR_HIP_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+      t += ms;
+      ++n;
+    }
+    *count = n;
+    *total_ms = t;
+    return MXR_SUCCESS;
+  });
+}
 
 mxr_status mxr_clear_caches(void) {
   return mxr::guarded("mxr_clear_caches", [&]() -> mxr_status {

@@ -144,7 +144,9 @@ class RadarPipeline:
 
 
 class FusedRadarPipeline:
-    """Three-pass throughput form.  x: [C, P, S] resident input cube.
+    """Three-pass throughput form.  x: [C, P, >=S] resident input cube (only
+    the first S samples of each pulse are read, e.g. MatX's inputView
+    {C, P, N}).
 
     tpc="valid", nd=P (rounded up) is the BASELINE workload (126 canceller
     rows, 128-point Doppler); tpc="same", nd=next_pow2(P, strict) is MatX's.
@@ -162,9 +164,13 @@ class FusedRadarPipeline:
         self.win, self.guard, self.pfa = win, guard, pfa
         self.stream = stream
         kw = dict(device=device)
-        self.y = torch.empty((C, P, self.R), dtype=torch.complex64, **kw)
-        self.pw = torch.empty((C, self.nd, self.R), dtype=torch.float32, **kw)
-        self.dets = torch.empty((C, self.nd, self.R), dtype=torch.int32, **kw)
+        # intermediate rows padded to 64 elements: every row starts on a
+        # 256/512-byte boundary, so tile reads and writes cover whole lines
+        self.ld = (self.R + 63) // 64 * 64
+        R = self.R
+        self.y = torch.empty((C, P, self.ld), dtype=torch.complex64, **kw)[:, :, :R]
+        self.pw = torch.empty((C, self.nd, self.ld), dtype=torch.float32, **kw)[:, :, :R]
+        self.dets = torch.empty((C, self.nd, self.ld), dtype=torch.int32, **kw)[:, :, :R]
 
     def step(self, x, w):
         s = self.stream
