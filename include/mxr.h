@@ -165,6 +165,44 @@ mxr_status mxr_cfar(const mxr_tensor *dets, const mxr_tensor *ba_out,
                     const mxr_tensor *pow_out, const mxr_tensor *in,
                     const mxr_cfar_params *params, hipStream_t stream);
 
+/* ----------------------------------------------------------- elementwise */
+/* Fused elementwise expression: out[i] = f(in_0[i], ..., in_{n-1}[i], i) for
+ * every index i of out, in one kernel -- the (lhs = expr).run(exec) path of
+ * cudaExecutor::Exec over matxOpT*Kernel (include/matx/executors/cuda.h:
+ * 84-230, include/matx/executors/kernel.h:41-222).  f is a postfix program
+ * over complex float values (real inputs enter with a zero imaginary part;
+ * a f32 output keeps the real part, an i32 output the truncated real part).
+ * Every input has out's rank; broadcasting is a zero stride (MatX Clone,
+ * include/matx/core/tensor.h:1101-1114).  Up to 8 inputs, 48 instructions,
+ * 16 constants, stack depth 8. */
+typedef enum mxr_ew_opcode {
+  MXR_EW_LOAD = 1,     /* push in[a] at the current index                  */
+  MXR_EW_CONST = 2,    /* push constant a (consts[2a] + i*consts[2a+1])     */
+  MXR_EW_HAMMING = 3,  /* push 0.54-0.46cos(2 pi i_a/(b-1)) in float,
+                          include/matx/generators/hamming.h:96-99           */
+  MXR_EW_INDEX = 4,    /* push the index along dim a                        */
+  MXR_EW_ADD = 10, MXR_EW_SUB = 11, MXR_EW_MUL = 12, MXR_EW_DIV = 13,
+  MXR_EW_POW = 14,     /* real part: x^y (cuda::std::pow of calcDets)       */
+  MXR_EW_GT = 15, MXR_EW_LT = 16, MXR_EW_GE = 17, MXR_EW_LE = 18,
+  MXR_EW_EQ = 19,      /* real-part comparisons -> 1 or 0                   */
+  MXR_EW_MAX = 20, MXR_EW_MIN = 21,
+  MXR_EW_CONJ = 30, MXR_EW_ABS2 = 31, MXR_EW_ABS = 32, MXR_EW_SQRT = 33,
+  MXR_EW_NEG = 34, MXR_EW_REAL = 35, MXR_EW_IMAG = 36, MXR_EW_EXP = 37,
+  MXR_EW_LOG = 38, MXR_EW_COS = 39, MXR_EW_SIN = 40,
+  MXR_EW_SELECT = 50   /* pop c, y, x: push (re c != 0) ? x : y             */
+} mxr_ew_opcode;
+typedef struct mxr_ew_instr {
+  int32_t op, a, b;
+} mxr_ew_instr;
+mxr_status mxr_ew_eval(const mxr_tensor *out, const mxr_tensor *const *ins, int nin,
+                       const mxr_ew_instr *prog, int nprog, const float *consts,
+                       int nconst, hipStream_t stream);
+
+/* Sum of all elements of `in` (c64 or f32) into the one-element `out` (same
+ * dtype), in float with a two-level tree (replaces sum_impl / cub_sum,
+ * include/matx/transforms/reduce.h:646-655, transforms/cub.h:1828-1860). */
+mxr_status mxr_sum(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stream);
+
 /* --------------------------------------------------------- convolution */
 /* Direct 1-D convolution along the last axis, batched over the leading
  * dims (filter rank 1 is broadcast, as conv1d_impl clones it,

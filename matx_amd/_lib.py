@@ -24,7 +24,7 @@ PC_WINDOW, PC_NORMALIZE, PC_WRITEBACK = 1, 2, 4
 EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_enable",
            "mxr_profile_reset", "mxr_profile_query", "mxr_fft_c2c",
            "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar",
-           "mxr_conv1d", "mxr_conv2d")
+           "mxr_conv1d", "mxr_conv2d", "mxr_ew_eval", "mxr_sum")
 
 STATUS_NAMES = {0: "SUCCESS", 1: "NOT_SUPPORTED", 2: "INVALID_PARAMETER",
                 3: "INVALID_DIM", 4: "INVALID_SIZE", 5: "OUT_OF_MEMORY",
@@ -88,6 +88,8 @@ def lib():
         "mxr_cfar": [T, T, T, T, ctypes.POINTER(mxr_cfar_params), V],
         "mxr_conv1d": [T, T, T, I, V],
         "mxr_conv2d": [T, T, T, I, V],
+        "mxr_ew_eval": [T, ctypes.c_void_p, I, ctypes.c_void_p, I, ctypes.c_void_p, I, V],
+        "mxr_sum": [T, T, V],
     }
     for name, argt in sigs.items():
         fn = getattr(L, name)

@@ -15,11 +15,13 @@
 // in float as calcDets evaluates it, det = pow > alpha*ba.
 //
 // Tiling: a one-wave workgroup owns 256 consecutive columns x 16 rows of one
-// channel.  It stages the (16 + 2HY) x (256 + 2*HALO) input cells in LDS with
-// 16-byte loads (zero outside the map); each lane then walks down its four
-// adjacent columns, forming the horizontal window sums of the four columns
-// with shared partial sums, keeping the last 2HY+1 rows in registers, and
-// writes each detection row as one 16-byte store per lane.
+// channel; each lane owns four adjacent columns.  Walking down the rows, a
+// lane reads its columns plus the window halo straight from memory (16-byte
+// accesses; the overlap between neighbouring lanes is served by the cache,
+// so HBM sees each cell about once), forms the horizontal window sums of its
+// four columns with shared partial sums, keeps the last 2HY+1 rows in
+// registers, and writes each detection row as one 16-byte store.  No LDS
+// staging and no barriers: residency is bounded by registers only.
 #include <cmath>
 #include <type_traits>
 
@@ -86,19 +88,18 @@ __device__ __forceinline__ void win4(const float *u, float (&o)[4]) {
 // threshold up to a few ulp of float rounding).  WPOW: also write pow.
 template <int HY, int HX, int EY, int EX, bool CIN, bool EXACT, bool WPOW>
 __global__ __launch_bounds__(CF_NT) void cfar_kernel(CfarArgs a) {
-  constexpr int HALO = (HX + 3) & ~3;  // keeps LDS rows and reads 16-byte aligned
-  constexpr int LW = CF_TX + 2 * HALO;
+  constexpr int HALO = (HX + 3) & ~3;  // whole quads of halo on each side
+  constexpr int NQ = 1 + 2 * HALO / 4; // quads a lane reads per row
+  constexpr int NV = 4 * NQ;
   constexpr int LH = CF_TY + 2 * HY;
   constexpr int NCELL = (2 * HY + 1) * (2 * HX + 1);
   constexpr int RING = 2 * HY + 1;
-  constexpr int NV = CF_CPL + 2 * HALO;  // values a lane reads per row
-  __shared__ __attribute__((aligned(16))) float tile[LH * LW];
   __shared__ float thr_tab[NCELL + 1];
 
   const int c = blockIdx.z;
   const int y0 = blockIdx.y * CF_TY;
-  const int x0 = blockIdx.x * CF_TX;
   const int lane = threadIdx.x;
+  const int gx0 = blockIdx.x * CF_TX + CF_CPL * lane;  // first column of this lane
   const int Nx = a.Nx, Ny = a.Ny;
 
   for (int n = lane; n <= NCELL; n += CF_NT) {
@@ -106,58 +107,48 @@ __global__ __launch_bounds__(CF_NT) void cfar_kernel(CfarArgs a) {
     const float alpha = nf * (powf(a.pfa, -1.0f / nf) - 1.0f);
     thr_tab[n] = EXACT ? alpha : alpha / nf;
   }
+  __syncthreads();
 
   using T = typename std::conditional<CIN, float2, float>::type;
   const T *const in_c = reinterpret_cast<const T *>(a.in) + (int64_t)c * a.is_c;
-  auto power = [](T z) -> float {
-    if constexpr (CIN) return z.x * z.x + z.y * z.y;
-    else return z;
-  };
   const int isy = (int)a.is_y, isx = (int)a.is_x;
-  // Stage rows y0-HY .. y0+TY+HY-1, columns x0-HALO .. x0+TX+HALO-1 (zero
-  // outside the map).  Lane l owns tile columns 4l..4l+3; lanes < HALO/2
-  // also fetch the halo quads.  Every load is issued before the LDS writes.
-  {
-    constexpr int NQ = CF_NT + HALO / 2;  // quads per row (body + halo)
-    for (int q = lane; q < NQ; q += CF_NT) {
-      // quad q < 64: body columns 4q..; else halo quads left then right
-      int tcol;
-      if (q < CF_NT) tcol = HALO + 4 * q;
-      else {
-        const int h = q - CF_NT;
-        tcol = h < HALO / 4 ? 4 * h : HALO + CF_TX + 4 * (h - HALO / 4);
-      }
-      const int gx = x0 - HALO + tcol;
-      const bool vec = a.vec_in && gx >= 0 && gx + 3 < Nx;
-      float4 v[LH];
+  // Row ly of the tile -> the NV values at columns gx0-HALO .. gx0+3+HALO,
+  // zero outside the map.  Quads wholly inside a row with unit stride load
+  // as one 16-byte access; the few quads crossing the map edge go per cell.
+  // Neighbouring lanes' quads overlap and are merged by the cache.
+  auto load_row = [&](int ly, float (&u)[NV]) {
+    const int gy = y0 - HY + ly;
+    if (gy < 0 || gy >= Ny) {
 #pragma unroll
-      for (int ly = 0; ly < LH; ++ly) {
-        const int gy = y0 - HY + ly;
-        const bool rok = gy >= 0 && gy < Ny;
-        const int gyc = min(max(gy, 0), Ny - 1);
-        float r[4];
-        if (vec) {
-          const float4 z = *reinterpret_cast<const float4 *>(
-              reinterpret_cast<const float *>(in_c) + gyc * isy + gx);
-          r[0] = z.x; r[1] = z.y; r[2] = z.z; r[3] = z.w;
-        } else {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int x = gx + j;
-            const float z = power(in_c[gyc * isy + min(max(x, 0), Nx - 1) * isx]);
-            r[j] = (x >= 0 && x < Nx) ? z : 0.f;
-          }
-        }
-        v[ly] = rok ? make_float4(r[0], r[1], r[2], r[3]) : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-#pragma unroll
-      for (int ly = 0; ly < LH; ++ly) *reinterpret_cast<float4 *>(&tile[ly * LW + tcol]) = v[ly];
+      for (int i = 0; i < NV; ++i) u[i] = 0.f;
+      return;
     }
-  }
-  __syncthreads();
+    const T *rp = in_c + gy * isy;
+#pragma unroll
+    for (int qd = 0; qd < NQ; ++qd) {
+      const int x = gx0 - HALO + 4 * qd;
+      if (!CIN && a.vec_in && x >= 0 && x + 3 < Nx) {
+        const float4 z = *reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(rp) + x);
+        u[4 * qd] = z.x;
+        u[4 * qd + 1] = z.y;
+        u[4 * qd + 2] = z.z;
+        u[4 * qd + 3] = z.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int xe = x + e;
+          float val = 0.f;
+          if (xe >= 0 && xe < Nx) {
+            const T z = rp[xe * isx];
+            if constexpr (CIN) val = z.x * z.x + z.y * z.y;
+            else val = z;
+          }
+          u[4 * qd + e] = val;
+        }
+      }
+    }
+  };
 
-  const int gx0 = x0 + CF_CPL * lane;  // first column of this lane
-  if (gx0 >= Nx) return;
   // in-bounds cell count = ro*co - rh*chh: co/chh per column, ro/rh per row
   int co[CF_CPL], chh[CF_CPL];
   float thr_int[CF_CPL];
@@ -168,20 +159,23 @@ __global__ __launch_bounds__(CF_NT) void cfar_kernel(CfarArgs a) {
     chh[j] = min(gx + EX, Nx - 1) - max(gx - EX, 0) + 1;
     thr_int[j] = thr_tab[min(max((2 * HY + 1) * co[j] - (2 * EY + 1) * chh[j], 0), NCELL)];
   }
+  const bool lane_live = gx0 < Nx;
   const bool vec_out = a.vec_dets && gx0 + 3 < Nx;
   const int dsy = (int)a.ds_y, dsx = (int)a.ds_x;
   int *const dets_c = a.dets + (int64_t)c * a.ds_c;
 
   float full[RING][CF_CPL], lr[RING][CF_CPL], ctr[RING][CF_CPL];
+  float ua[NV], ub[NV];  // two rows in flight
+  load_row(0, ua);
+  if (LH > 1) load_row(1, ub);
 #pragma unroll
   for (int ly = 0; ly < LH; ++ly) {
-    // values of columns gx0-HALO .. gx0+3+HALO of this row
     float u[NV];
-    const float4 *row4 = reinterpret_cast<const float4 *>(&tile[ly * LW + CF_CPL * lane]);
 #pragma unroll
-    for (int i = 0; i < NV / 4; ++i) {
-      const float4 z = row4[i];
-      u[4 * i] = z.x; u[4 * i + 1] = z.y; u[4 * i + 2] = z.z; u[4 * i + 3] = z.w;
+    for (int i = 0; i < NV; ++i) u[i] = (ly & 1) ? ub[i] : ua[i];
+    if (ly + 2 < LH) {
+      if (ly & 1) load_row(ly + 2, ub);
+      else load_row(ly + 2, ua);
     }
     // u[HALO + j + d] is column gx0 + j + d
     float L[4], Mh[4], Rt[4];
@@ -199,6 +193,7 @@ __global__ __launch_bounds__(CF_NT) void cfar_kernel(CfarArgs a) {
     const int oy = ly - HY;  // tile row of the cells under test
     const int gy = y0 + oy - HY;
     if (gy >= Ny) break;
+    if (!lane_live) continue;
     float thr[CF_CPL];
     int nrm[CF_CPL];
     const bool edge = gy < HY || gy + HY >= Ny;

@@ -1,0 +1,409 @@
+// Fused elementwise expressions and the scalar sum.
+//
+// MatX fuses an elementwise expression tree into one templated kernel per
+// (lhs = expr).run(exec) at compile time (include/matx/executors/kernel.h).
+// Behind a C ABI the expression cannot be a C++ type, so the front end lowers
+// it to a short postfix program and one precompiled kernel interprets it per
+// element.  The program is the same for every lane, so its decode and
+// branches run on the scalar unit; only the arithmetic is per lane.  Each
+// lane handles EW_EPT consecutive elements of the innermost dimension, so the
+// index decomposition is paid once per run of elements.
+#include <cmath>
+#include <map>
+#include <mutex>
+
+#include "mxr_cache.hpp"
+#include "mxr_internal.hpp"
+
+namespace mxr {
+
+constexpr int EW_MAXIN = 8, EW_MAXPROG = 48, EW_MAXCONST = 16, EW_STACK = 8;
+constexpr int EW_EPT = 4;
+
+struct EwIn {
+  const void *p;
+  int dtype;
+  int64_t st[MXR_MAX_RANK];
+};
+struct EwArgs {
+  void *out;
+  int odtype;
+  int rank;
+  int64_t shape[MXR_MAX_RANK];
+  int64_t ost[MXR_MAX_RANK];
+  int64_t total_runs;  // runs of EW_EPT along the innermost dim
+  int64_t runs_per_row;
+  int nin, nprog;
+  EwIn in[EW_MAXIN];
+  mxr_ew_instr prog[EW_MAXPROG];
+  float2 cst[EW_MAXCONST];
+};
+
+__device__ __forceinline__ float2 ew_load(const EwIn &in, int64_t off) {
+  switch (in.dtype) {
+    case MXR_C64: return reinterpret_cast<const float2 *>(in.p)[off];
+    case MXR_F32: return make_float2(reinterpret_cast<const float *>(in.p)[off], 0.f);
+    default: return make_float2((float)reinterpret_cast<const int *>(in.p)[off], 0.f);
+  }
+}
+
+__device__ __forceinline__ float2 cdiv(float2 a, float2 b) {
+  // MatX simplified complex division (core/defines.h:50-58 semantics)
+  const float d = b.x * b.x + b.y * b.y;
+  return make_float2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+
+__global__ __launch_bounds__(256) void ew_kernel(const EwArgs *__restrict__ ap) {
+  const EwArgs &a = *ap;
+  const int64_t run = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (run >= a.total_runs) return;
+  const int r = a.rank;
+  // coordinates of the first element of this run
+  int64_t idx[MXR_MAX_RANK];
+  int64_t rem = run / a.runs_per_row;
+  idx[r - 1] = (run % a.runs_per_row) * EW_EPT;
+  for (int d = r - 2; d >= 0; --d) {
+    idx[d] = rem % a.shape[d];
+    rem /= a.shape[d];
+  }
+  int64_t ooff = 0;
+  int64_t ioff[EW_MAXIN];
+  for (int k = 0; k < a.nin; ++k) ioff[k] = 0;
+  for (int d = 0; d < r; ++d) {
+    ooff += idx[d] * a.ost[d];
+    for (int k = 0; k < a.nin; ++k) ioff[k] += idx[d] * a.in[k].st[d];
+  }
+  const int64_t last = idx[r - 1];
+  const int cnt = (int)min((int64_t)EW_EPT, a.shape[r - 1] - last);
+  for (int e = 0; e < cnt; ++e) {
+    float2 stk[EW_STACK];
+    int sp = 0;
+    for (int pc = 0; pc < a.nprog; ++pc) {
+      const mxr_ew_instr ins = a.prog[pc];
+      switch (ins.op) {
+        case MXR_EW_LOAD:
+          stk[sp++] = ew_load(a.in[ins.a], ioff[ins.a] + e * a.in[ins.a].st[r - 1]);
+          break;
+        case MXR_EW_CONST: stk[sp++] = a.cst[ins.a]; break;
+        case MXR_EW_HAMMING: {
+          const int64_t i = ins.a == r - 1 ? last + e : idx[ins.a];
+          stk[sp++] = make_float2(hamming_f32((int)i, ins.b), 0.f);
+          break;
+        }
+        case MXR_EW_INDEX: {
+          const int64_t i = ins.a == r - 1 ? last + e : idx[ins.a];
+          stk[sp++] = make_float2((float)i, 0.f);
+          break;
+        }
+        case MXR_EW_SELECT: {
+          const float2 cnd = stk[--sp], y = stk[--sp], x = stk[--sp];
+          stk[sp++] = cnd.x != 0.f ? x : y;
+          break;
+        }
+        default: {
+          if (ins.op >= MXR_EW_ADD && ins.op < MXR_EW_CONJ) {
+            const float2 y = stk[--sp], x = stk[--sp];
+            float2 z;
+            switch (ins.op) {
+              case MXR_EW_ADD: z = cadd(x, y); break;
+              case MXR_EW_SUB: z = csub(x, y); break;
+              case MXR_EW_MUL: z = cmul(x, y); break;
+              case MXR_EW_DIV: z = (y.y == 0.f) ? make_float2(x.x / y.x, x.y / y.x) : cdiv(x, y); break;
+              case MXR_EW_POW: z = make_float2(powf(x.x, y.x), 0.f); break;
+              case MXR_EW_GT: z = make_float2(x.x > y.x ? 1.f : 0.f, 0.f); break;
+              case MXR_EW_LT: z = make_float2(x.x < y.x ? 1.f : 0.f, 0.f); break;
+              case MXR_EW_GE: z = make_float2(x.x >= y.x ? 1.f : 0.f, 0.f); break;
+              case MXR_EW_LE: z = make_float2(x.x <= y.x ? 1.f : 0.f, 0.f); break;
+              case MXR_EW_EQ: z = make_float2((x.x == y.x && x.y == y.y) ? 1.f : 0.f, 0.f); break;
+              case MXR_EW_MAX: z = make_float2(fmaxf(x.x, y.x), 0.f); break;
+              default: z = make_float2(fminf(x.x, y.x), 0.f); break;
+            }
+            stk[sp++] = z;
+          } else {
+            const float2 x = stk[--sp];
+            float2 z;
+            switch (ins.op) {
+              case MXR_EW_CONJ: z = make_float2(x.x, -x.y); break;
+              case MXR_EW_ABS2: z = make_float2(x.x * x.x + x.y * x.y, 0.f); break;
+              case MXR_EW_ABS: z = make_float2(sqrtf(x.x * x.x + x.y * x.y), 0.f); break;
+              case MXR_EW_SQRT: z = make_float2(sqrtf(x.x), 0.f); break;
+              case MXR_EW_NEG: z = make_float2(-x.x, -x.y); break;
+              case MXR_EW_REAL: z = make_float2(x.x, 0.f); break;
+              case MXR_EW_IMAG: z = make_float2(x.y, 0.f); break;
+              case MXR_EW_EXP: {
+                const float m = expf(x.x);
+                z = make_float2(m * cosf(x.y), m * sinf(x.y));
+                break;
+              }
+              case MXR_EW_LOG: z = make_float2(logf(sqrtf(x.x * x.x + x.y * x.y)), atan2f(x.y, x.x)); break;
+              case MXR_EW_COS: z = make_float2(cosf(x.x), 0.f); break;
+              default: z = make_float2(sinf(x.x), 0.f); break;
+            }
+            stk[sp++] = z;
+          }
+        }
+      }
+    }
+    const float2 v = stk[0];
+    const int64_t o = ooff + e * a.ost[r - 1];
+    switch (a.odtype) {
+      case MXR_C64: reinterpret_cast<float2 *>(a.out)[o] = v; break;
+      case MXR_F32: reinterpret_cast<float *>(a.out)[o] = v.x; break;
+      default: reinterpret_cast<int *>(a.out)[o] = (int)v.x; break;
+    }
+  }
+}
+
+// Kernel arguments live in a small per-stream device ring so a launch never
+// depends on host memory staying alive.
+namespace {
+struct ArgRing {
+  EwArgs *dev = nullptr;
+  int slots = 0, next = 0;
+};
+std::mutex g_ring_mu;
+std::map<std::pair<int, uintptr_t>, ArgRing> g_rings;
+}  // namespace
+
+static mxr_status ew_validate_program(const mxr_ew_instr *prog, int nprog, int nin, int nconst,
+                                      int rank) {
+  MXR_REQUIRE(nprog >= 1 && nprog <= EW_MAXPROG, MXR_INVALID_PARAMETER,
+              "ew: program length %d not in [1, %d]", nprog, EW_MAXPROG);
+  int sp = 0;
+  for (int pc = 0; pc < nprog; ++pc) {
+    const mxr_ew_instr &i = prog[pc];
+    int pops = 0, pushes = 1;
+    switch (i.op) {
+      case MXR_EW_LOAD:
+        MXR_REQUIRE(i.a >= 0 && i.a < nin, MXR_INVALID_PARAMETER, "ew: LOAD of input %d", i.a);
+        break;
+      case MXR_EW_CONST:
+        MXR_REQUIRE(i.a >= 0 && i.a < nconst, MXR_INVALID_PARAMETER, "ew: CONST %d", i.a);
+        break;
+      case MXR_EW_HAMMING:
+      case MXR_EW_INDEX:
+        MXR_REQUIRE(i.a >= 0 && i.a < rank, MXR_INVALID_PARAMETER, "ew: dim %d", i.a);
+        break;
+      case MXR_EW_SELECT: pops = 3; break;
+      default:
+        if (i.op >= MXR_EW_ADD && i.op <= MXR_EW_MIN) pops = 2;
+        else if (i.op >= MXR_EW_CONJ && i.op <= MXR_EW_SIN) pops = 1;
+        else {
+          set_error(MXR_INVALID_PARAMETER, "ew: unknown opcode %d", i.op);
+          return MXR_INVALID_PARAMETER;
+        }
+    }
+    MXR_REQUIRE(sp >= pops, MXR_INVALID_PARAMETER, "ew: stack underflow at instruction %d", pc);
+    sp += pushes - pops;
+    MXR_REQUIRE(sp <= EW_STACK, MXR_INVALID_PARAMETER, "ew: stack deeper than %d", EW_STACK);
+  }
+  MXR_REQUIRE(sp == 1, MXR_INVALID_PARAMETER, "ew: program leaves %d values", sp);
+  return MXR_SUCCESS;
+}
+
+mxr_status ew_impl(const mxr_tensor *out, const mxr_tensor *const *ins, int nin,
+                   const mxr_ew_instr *prog, int nprog, const float *consts, int nconst,
+                   hipStream_t stream) {
+  mxr_status st;
+  if ((st = check_tensor(out, "ew output", 0, 1, MXR_MAX_RANK))) return st;
+  MXR_REQUIRE(nin >= 0 && nin <= EW_MAXIN, MXR_INVALID_PARAMETER, "ew: %d inputs", nin);
+  MXR_REQUIRE(nconst >= 0 && nconst <= EW_MAXCONST, MXR_INVALID_PARAMETER, "ew: %d constants",
+              nconst);
+  MXR_REQUIRE(prog != nullptr, MXR_INVALID_PARAMETER, "ew: null program");
+  if ((st = ew_validate_program(prog, nprog, nin, nconst, out->rank))) return st;
+  EwArgs a{};
+  a.out = out->data;
+  a.odtype = out->dtype;
+  a.rank = out->rank;
+  for (int d = 0; d < out->rank; ++d) {
+    a.shape[d] = out->shape[d];
+    a.ost[d] = out->stride[d];
+  }
+  a.nin = nin;
+  for (int k = 0; k < nin; ++k) {
+    if ((st = check_tensor(ins[k], "ew input", 0, out->rank, out->rank))) return st;
+    for (int d = 0; d < out->rank; ++d)
+      MXR_REQUIRE(ins[k]->shape[d] == out->shape[d] || ins[k]->stride[d] == 0, MXR_INVALID_SIZE,
+                  "ew: input %d dim %d extent %lld does not match the output %lld", k, d,
+                  (long long)ins[k]->shape[d], (long long)out->shape[d]);
+    a.in[k].p = ins[k]->data;
+    a.in[k].dtype = ins[k]->dtype;
+    for (int d = 0; d < out->rank; ++d) a.in[k].st[d] = ins[k]->stride[d];
+  }
+  a.nprog = nprog;
+  for (int i = 0; i < nprog; ++i) a.prog[i] = prog[i];
+  for (int i = 0; i < nconst; ++i) a.cst[i] = make_float2(consts[2 * i], consts[2 * i + 1]);
+  const int64_t n_last = out->shape[out->rank - 1];
+  if (numel(*out) == 0) return MXR_SUCCESS;
+  a.runs_per_row = (n_last + EW_EPT - 1) / EW_EPT;
+  a.total_runs = (numel(*out) / n_last) * a.runs_per_row;
+  // the program and views travel through a per-stream device ring of
+  // argument slots (an in-order stream never reuses a slot before the launch
+  // that read it has run, as long as fewer than `slots` launches are queued)
+  EwArgs *slot = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_ring_mu);
+    ArgRing &ring = g_rings[{device_id(), (uintptr_t)stream}];
+    if (!ring.dev) {
+      ring.slots = 1024;
+      MXR_HIP_TRY(hipMalloc(&ring.dev, sizeof(EwArgs) * ring.slots));
+    }
+    slot = ring.dev + ring.next;
+    ring.next = (ring.next + 1) % ring.slots;
+  }
+  MXR_HIP_TRY(hipMemcpyAsync(slot, &a, sizeof(EwArgs), hipMemcpyHostToDevice, stream));
+  const int64_t blocks = (a.total_runs + 255) / 256;
+  MXR_REQUIRE(blocks <= 0x7fffffff, MXR_INVALID_SIZE, "ew: too large");
+  {
+    ProfScope ps("ew_kernel", stream);
+    ew_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(slot);
+  }
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
+// ------------------------------------------------------------------ sum
+__global__ __launch_bounds__(256) void sum_kernel(const void *in, int is_c, int rank,
+                                                  const int64_t *shape_stride, int64_t n,
+                                                  float2 *partial, void *out, int out_c,
+                                                  int stage) {
+  __shared__ double sr[256], si[256];
+  double ar = 0.0, ai = 0.0;
+  if (stage == 0) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+      int64_t rem = e, off = 0;
+      for (int d = rank - 1; d >= 0; --d) {
+        const int64_t s = shape_stride[d];
+        off += (rem % s) * shape_stride[MXR_MAX_RANK + d];
+        rem /= s;
+      }
+      if (is_c) {
+        const float2 z = reinterpret_cast<const float2 *>(in)[off];
+        ar += z.x;
+        ai += z.y;
+      } else {
+        ar += reinterpret_cast<const float *>(in)[off];
+      }
+    }
+  } else {
+    for (int e = threadIdx.x; e < (int)n; e += 256) {
+      ar += partial[e].x;
+      ai += partial[e].y;
+    }
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (stage == 0) {
+      partial[blockIdx.x] = make_float2((float)sr[0], (float)si[0]);
+    } else if (out_c) {
+      *reinterpret_cast<float2 *>(out) = make_float2((float)sr[0], (float)si[0]);
+    } else {
+      *reinterpret_cast<float *>(out) = (float)sr[0];
+    }
+  }
+}
+
+namespace {
+struct SumWs {
+  float2 *partial = nullptr;
+  int64_t *meta = nullptr;  // 1024 shape/stride slots
+  int next = 0;
+};
+std::mutex g_sum_mu;
+std::map<std::pair<int, uintptr_t>, SumWs> g_sum;
+
+void clear_ew() {
+  {
+    std::lock_guard<std::mutex> g(g_ring_mu);
+    for (auto &kv : g_rings) {
+      int cur = 0;
+      (void)hipGetDevice(&cur);
+      (void)hipSetDevice(kv.first.first);
+      (void)hipFree(kv.second.dev);
+      (void)hipSetDevice(cur);
+    }
+    g_rings.clear();
+  }
+  std::lock_guard<std::mutex> g(g_sum_mu);
+  for (auto &kv : g_sum) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(kv.first.first);
+    (void)hipFree(kv.second.partial);
+    (void)hipFree(kv.second.meta);
+    (void)hipSetDevice(cur);
+  }
+  g_sum.clear();
+}
+CacheClearRegistrar g_reg_ew(clear_ew);
+}  // namespace
+
+mxr_status sum_impl(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stream) {
+  mxr_status st;
+  if ((st = check_tensor(in, "sum input", 0, 1, MXR_MAX_RANK))) return st;
+  if ((st = check_tensor(out, "sum output", 0, 0, MXR_MAX_RANK))) return st;
+  MXR_REQUIRE(in->dtype == MXR_C64 || in->dtype == MXR_F32, MXR_INVALID_TYPE,
+              "sum: input must be c64 or f32");
+  MXR_REQUIRE(out->dtype == in->dtype, MXR_INVALID_TYPE, "sum: output dtype must match input");
+  MXR_REQUIRE(numel(*out) == 1, MXR_INVALID_SIZE, "sum: output must hold one element");
+  constexpr int NBLK = 512;
+  const int64_t n = numel(*in);
+  SumWs *ws;
+  int64_t *meta;
+  {
+    std::lock_guard<std::mutex> g(g_sum_mu);
+    ws = &g_sum[{device_id(), (uintptr_t)stream}];
+    if (!ws->partial) {
+      MXR_HIP_TRY(hipMalloc(&ws->partial, sizeof(float2) * NBLK));
+      MXR_HIP_TRY(hipMalloc(&ws->meta, sizeof(int64_t) * 2 * MXR_MAX_RANK * 64));
+    }
+    meta = ws->meta + 2 * MXR_MAX_RANK * ws->next;
+    ws->next = (ws->next + 1) % 64;
+  }
+  int64_t hm[2 * MXR_MAX_RANK] = {0};
+  for (int d = 0; d < in->rank; ++d) {
+    hm[d] = in->shape[d];
+    hm[MXR_MAX_RANK + d] = in->stride[d];
+  }
+  MXR_HIP_TRY(hipMemcpyAsync(meta, hm, sizeof(hm), hipMemcpyHostToDevice, stream));
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > NBLK) blocks = NBLK;
+  if (blocks < 1) blocks = 1;
+  {
+    ProfScope ps("sum_kernel", stream);
+    sum_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(in->data, in->dtype == MXR_C64,
+                                                                  in->rank, meta, n, ws->partial,
+                                                                  nullptr, 0, 0);
+    sum_kernel<<<dim3(1), dim3(256), 0, stream>>>(nullptr, 0, 0, nullptr, blocks, ws->partial,
+                                                  out->data, out->dtype == MXR_C64, 1);
+  }
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
+}  // namespace mxr
+
+extern "C" {
+
+mxr_status mxr_ew_eval(const mxr_tensor *out, const mxr_tensor *const *ins, int nin,
+                       const mxr_ew_instr *prog, int nprog, const float *consts, int nconst,
+                       hipStream_t stream) {
+  return mxr::guarded("mxr_ew_eval", [&] {
+    return mxr::ew_impl(out, ins, nin, prog, nprog, consts, nconst, stream);
+  });
+}
+
+mxr_status mxr_sum(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stream) {
+  return mxr::guarded("mxr_sum", [&] { return mxr::sum_impl(out, in, stream); });
+}
+
+}  // extern "C"

@@ -204,6 +204,11 @@ void build_twiddles(int N, RadixSeq s, float2 *out);
 // device pointer owned by the library.
 const float2 *twiddles_device(int N, bool reversed_seq);
 
+// Per-device cache of radix-16 power-base rows for the pass with sub-length
+// Ls of an N-point transform: row k (k < Ls) holds W^k, W^2k, W^4k, W^8k with
+// W = e^{-2 pi i / (16 Ls)}, each from double precision rounded once.
+const float4 *pow_base_device(int N, int Ls);
+
 // Workgroup barrier that is also a compiler fence: no load or store is moved
 // across it, so twiddle and spectrum loads stay inside their pass instead of
 // being hoisted (and held in VGPRs) across the whole kernel.

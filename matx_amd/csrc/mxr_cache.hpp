@@ -1,5 +1,8 @@
-// Registration of per-module cache-release functions run by mxr_clear_caches.
+// Registration of per-module cache-release functions run by mxr_clear_caches,
+// and the kernel-timing scope used around every launch.
 #pragma once
+
+#include <hip/hip_runtime.h>
 
 namespace mxr {
 void register_cache_clear(void (*fn)());
@@ -8,8 +11,8 @@ void register_cache_clear(void (*fn)());
 struct CacheClearRegistrar {
   explicit CacheClearRegistrar(void (*fn)()) { register_cache_clear(fn); }
 };
+
 // Brackets one kernel launch with profiling events when profiling is on.
-#include <hip/hip_runtime.h>
 class ProfScope {
  public:
   ProfScope(const char *kernel, hipStream_t s);

@@ -128,6 +128,117 @@ void pc_ols_kernel(PcArgs a) {
   }
 }
 
+// Dedicated M = 4096 block kernel (plan radix 16 x 16 x 16, 256 threads, one
+// radix-16 butterfly per thread per pass).  Twiddles are built in registers
+// from four base powers W^k, W^2k, W^4k, W^8k per pass (one 32-byte row of a
+// compact table, W = e^{-2 pi i / (16 Ls)}): W^{mk} for the other m is a
+// product of at most four of them (at most three extra roundings).  The base
+// rows are fetched before each LDS exchange, so their latency hides behind
+// the exchange and its barriers instead of stalling the butterfly after it;
+// the inverse passes reuse the forward rows conjugated.
+struct Pc4096Args {
+  PcArgs a;
+  const float4 *base1;  // [16][2] float4: pass Ls = 16
+  const float4 *base2;  // [256][2] float4: pass Ls = 256
+};
+
+__device__ __forceinline__ void tw_from_base(float2 (&v)[16], float4 b0, float4 b1, bool inv) {
+  float2 w1 = make_float2(b0.x, b0.y), w2 = make_float2(b0.z, b0.w);
+  float2 w4 = make_float2(b1.x, b1.y), w8 = make_float2(b1.z, b1.w);
+  if (inv) {
+    w1.y = -w1.y;
+    w2.y = -w2.y;
+    w4.y = -w4.y;
+    w8.y = -w8.y;
+  }
+  float2 w[16];
+  w[1] = w1;
+  w[2] = w2;
+  w[4] = w4;
+  w[8] = w8;
+  w[3] = cmul(w2, w1);
+  w[5] = cmul(w4, w1);
+  w[6] = cmul(w4, w2);
+  w[7] = cmul(w[6], w1);
+  w[9] = cmul(w8, w1);
+  w[10] = cmul(w8, w2);
+  w[11] = cmul(w[10], w1);
+  w[12] = cmul(w8, w4);
+  w[13] = cmul(w[12], w1);
+  w[14] = cmul(w[12], w2);
+  w[15] = cmul(w[14], w1);
+#pragma unroll
+  for (int m = 1; m < 16; ++m) v[m] = cmul(v[m], w[m]);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void pc4096_kernel(Pc4096Args pa) {
+  constexpr int M = 4096, NT = 256;
+  using LY = Layout<M, 1, false>;
+  __shared__ float2 lds[LY::LDS_ELEMS];
+  const PcArgs &a = pa.a;
+  const int j = threadIdx.x;
+  const int64_t g = xcd_remap(blockIdx.x, a.groups);
+  const int64_t row = g / a.nblk;
+  const int lo = (int)((g % a.nblk) * a.V);
+  int64_t ox, oy;
+  batch_offsets(a.bi, row, &ox, &oy);
+  const float2 *__restrict__ xr = a.x + ox;
+  float2 *__restrict__ yr = a.y + oy;
+  const int n32 = (int)a.n, seff32 = (int)a.seff;
+
+  float2 v[16];
+  // Stockham exchange: write this pass's outputs, read the next pass's inputs
+  auto xchg = [&](int ls_out) {
+    const int k = j & (ls_out - 1), base = (j - k) * 16 + k;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) lds[LY::addr(0, base + q * ls_out)] = v[q];
+    wg_barrier();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = lds[LY::addr(0, j + m * NT)];
+    wg_barrier();
+  };
+  const float4 *b1p = pa.base1 + (j & 15) * 2;
+  const float4 *b2p = pa.base2 + j * 2;
+
+  // forward pass 0 (Ls = 1) straight from HBM (zero padding and wrap implicit)
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    int gi = lo + j + m * NT;
+    if (gi >= n32) gi -= n32;
+    v[m] = gi < seff32 ? xr[gi] : make_float2(0.f, 0.f);
+  }
+  dft16<false>(v);
+  float4 t0 = b1p[0], t1 = b1p[1];
+  xchg(1);
+  tw_from_base(v, t0, t1, false);  // forward pass 1 (Ls = 16)
+  dft16<false>(v);
+  t0 = b2p[0];
+  t1 = b2p[1];
+  xchg(16);
+  tw_from_base(v, t0, t1, false);  // forward pass 2 (Ls = 256)
+  dft16<false>(v);
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], a.wc[j + q * 256]);
+  dft16<true>(v);                   // inverse pass 0 (Ls = 1)
+  t0 = b1p[0];
+  t1 = b1p[1];
+  xchg(1);
+  tw_from_base(v, t0, t1, true);   // inverse pass 1 (Ls = 16)
+  dft16<true>(v);
+  t0 = b2p[0];
+  t1 = b2p[1];
+  xchg(16);
+  tw_from_base(v, t0, t1, true);   // inverse pass 2 (Ls = 256)
+  dft16<true>(v);
+  const int lim = (int)((a.nout - lo) < a.V ? (a.nout - lo) : a.V);
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = j + q * 256;
+    if (i < lim) yr[lo + i] = cscale(v[q], a.scale);
+  }
+}
+
 // Waveform spectrum: Wc = conj?(fft_M(pad_M(w2))), w2 = w1/||w1|| (optional),
 // w1 = w .* hamming(L) (optional); optionally writes w2 back to w.
 struct WfArgs {
@@ -288,6 +399,16 @@ mxr_status launch_pc(const WfArgs &wa, const PcArgs &pa, hipStream_t s) {
     wfspec_kernel<M><<<dim3(1), dim3(PcCfg<M>::NT), 0, s>>>(wa);
   }
   MXR_HIP_TRY(hipGetLastError());
+  if constexpr (M == 4096) {
+    if (pa.groups > 0 && pa.bpw == 1) {
+      Pc4096Args p4{pa, pow_base_device(4096, 16), pow_base_device(4096, 256)};
+      MXR_REQUIRE(p4.base1 && p4.base2, MXR_OUT_OF_MEMORY, "pulse compression: twiddle table allocation failed");
+      ProfScope ps("pc_ols_kernel", s);
+      pc4096_kernel<<<dim3((unsigned)pa.groups), dim3(256), 0, s>>>(p4);
+      MXR_HIP_TRY(hipGetLastError());
+      return MXR_SUCCESS;
+    }
+  }
   if (pa.groups > 0) {
     ProfScope ps("pc_ols_kernel", s);
     if (pa.bpw > 1)

@@ -184,6 +184,40 @@ ProfScope::~ProfScope() {
   (void)hipEventRecord(g_prof[slot_].b, s_);
 }
 
+namespace {
+std::map<TwKey, float4 *> g_pb;
+}
+
+const float4 *pow_base_device(int N, int Ls) {
+  const int dev = device_id();
+  std::lock_guard<std::mutex> g(g_tw_mu);
+  TwKey key{dev, N, Ls};
+  auto it = g_pb.find(key);
+  if (it != g_pb.end()) return it->second;
+  const double two_pi = 6.283185307179586476925286766559;
+  const int64_t len = (int64_t)16 * Ls;
+  std::vector<float4> host((size_t)Ls * 2);
+  for (int64_t k = 0; k < Ls; ++k) {
+    float2 w[4];
+    const int pw[4] = {1, 2, 4, 8};
+    for (int i = 0; i < 4; ++i) {
+      const int64_t e = (pw[i] * k) % len;
+      const double ang = -two_pi * (double)e / (double)len;
+      w[i] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    host[2 * k] = make_float4(w[0].x, w[0].y, w[1].x, w[1].y);
+    host[2 * k + 1] = make_float4(w[2].x, w[2].y, w[3].x, w[3].y);
+  }
+  float4 *d = nullptr;
+  if (hipMalloc(&d, sizeof(float4) * host.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, host.data(), sizeof(float4) * host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  g_pb[key] = d;
+  return d;
+}
+
 static void clear_twiddles() {
   std::lock_guard<std::mutex> g(g_tw_mu);
   for (auto &kv : g_tw) {
@@ -194,6 +228,14 @@ static void clear_twiddles() {
     (void)hipSetDevice(cur);
   }
   g_tw.clear();
+  for (auto &kv : g_pb) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(kv.first.dev);
+    (void)hipFree(kv.second);
+    (void)hipSetDevice(cur);
+  }
+  g_pb.clear();
 }
 
 }  // namespace mxr

@@ -284,3 +284,48 @@ def test_matx_stage_api(mx, orc, matx_inputs, golden):
         assert np.array_equal(np.argwhere(dets[c] == 1), golden["matx_det_idx"])
     nt = host(p.GetnormT())[0, 2:258, 6:8007]
     assert np.array_equal(nt, st["norm"][0])
+
+
+# ------------------------------------------------------- elementwise / sum
+def test_ew_expressions(mx):
+    from matx_amd import expr as E
+    rng = np.random.default_rng(20)
+    a = crandn(rng, 4, 126, 300)
+    b = rng.standard_normal(300).astype(np.float32)
+    ad, bd = dev(a), dev(b, torch.float32)
+    # xc = xc * hamming<1>  (simple_radar_pipeline.h:353-355), in place
+    out = ad.clone()
+    E.run(out, E.wrap(out) * E.hamming(1, 126))
+    assert rel(host(out), a * np.hamming(126)[None, :, None]) < RTOL
+    # broadcast real vector, conj, division by a scalar constant
+    o2 = torch.empty_like(ad)
+    E.run(o2, E.conj(ad) * bd / 3.0 + 1.5)
+    assert rel(host(o2), np.conj(a) * b / 3.0 + 1.5) < RTOL
+    # abs2 into a real output, comparisons into int32
+    p = torch.empty(ad.shape, dtype=torch.float32, device="cuda")
+    E.run(p, E.abs2(ad))
+    assert rel(host(p), np.abs(a) ** 2) < RTOL
+    m = torch.empty(ad.shape, dtype=torch.int32, device="cuda")
+    E.run(m, E.abs2(ad) > 2.0)
+    assert np.array_equal(host(m), (np.abs(a.astype(complex)) ** 2 > 2.0).astype(np.int32))
+    # calcDets body: xpow > norm*(pfa^(-1/norm)-1)*ba  (simple_radar_pipeline.h:86-98)
+    nrm = dev(rng.integers(20, 57, (4, 126, 300)).astype(np.float32), torch.float32)
+    ba = dev(rng.exponential(1.0, (4, 126, 300)).astype(np.float32), torch.float32)
+    d2 = torch.empty(ad.shape, dtype=torch.int32, device="cuda")
+    E.run(d2, E.abs2(ad) > E.wrap(nrm) * (E.pow(1e-5, -1.0 / E.wrap(nrm)) - 1.0) * ba)
+    n_ = host(nrm).astype(np.float64)
+    thr = n_ * (np.power(1e-5, -1.0 / n_) - 1) * host(ba)
+    pw = np.abs(a.astype(complex)) ** 2
+    near = np.abs(pw - thr) / thr < 1e-5
+    assert ((host(d2) != (pw > thr)) & ~near).sum() == 0
+
+
+def test_sum(mx):
+    from matx_amd import expr as E
+    rng = np.random.default_rng(21)
+    w = crandn(rng, 1000)
+    s = host(E.sum(dev(w)))
+    assert abs(s - w.astype(complex).sum()) / abs(w.astype(complex).sum()) < 1e-5
+    r = rng.standard_normal((7, 333)).astype(np.float32)
+    sr = host(E.sum(dev(r, torch.float32)[:, ::2]))
+    assert abs(sr - r[:, ::2].astype(np.float64).sum()) < 1e-4 * np.abs(r).sum()
