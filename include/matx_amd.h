@@ -1,0 +1,629 @@
+// matx_amd.h -- C++ front end with MatX's operator API over libmxr.so.
+//
+// Host-only, header-only C++17: a MatX-style program (tensor_t, make_tensor,
+// slice / Permute / Clone views, (lhs = op).run(exec), fft / ifft / fft_conv /
+// conv1d / conv2d / sum, elementwise operators and generators) compiles with
+// g++ against this header and links libmxr.so; every operation reaches the
+// gfx950 kernels through the C ABI in mxr.h.  The executor is HIPExecutor (a
+// hipStream_t), the analogue of cudaExecutor
+// (include/matx/executors/cuda.h:57-231 in MatX).
+//
+// Differences from MatX, by design:
+//  * elementwise expressions are lowered at run() time to one postfix program
+//    evaluated by one kernel (mxr_ew_eval) instead of a compile-time template;
+//  * a transform whose input is an expression materialises it into a
+//    temporary first (MatX allocates the same temporaries,
+//    include/matx/operators/fft.h:493-505);
+//  * errors throw matxException carrying the mxr_status and message.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <array>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <initializer_list>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "mxr.h"
+
+namespace matx {
+
+using index_t = int64_t;
+using cfloat = std::complex<float>;  // interleaved re,im: the mxr C64 layout
+
+constexpr index_t matxKeepDim = std::numeric_limits<index_t>::max();
+constexpr index_t matxEnd = std::numeric_limits<index_t>::max() - 1;
+constexpr index_t matxDropDim = std::numeric_limits<index_t>::max() - 2;
+
+enum matxConvCorrMode_t { MATX_C_MODE_FULL = 0, MATX_C_MODE_SAME = 1, MATX_C_MODE_VALID = 2 };
+enum class FFTNorm { BACKWARD = 0, FORWARD = 1, ORTHO = 2 };
+enum matxMemorySpace_t { MATX_MANAGED_MEMORY, MATX_DEVICE_MEMORY, MATX_HOST_MALLOC_MEMORY };
+
+class matxException : public std::runtime_error {
+ public:
+  matxException(int code, const std::string &msg) : std::runtime_error(msg), code_(code) {}
+  int code() const { return code_; }
+
+ private:
+  int code_;
+};
+
+namespace detail {
+inline void check(mxr_status s) {
+  if (s != MXR_SUCCESS) throw matxException((int)s, mxr_last_error());
+}
+inline void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw matxException(MXR_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+template <typename T>
+struct dtype_of;
+template <>
+struct dtype_of<cfloat> {
+  static constexpr int value = MXR_C64;
+};
+template <>
+struct dtype_of<float> {
+  static constexpr int value = MXR_F32;
+};
+template <>
+struct dtype_of<int> {
+  static constexpr int value = MXR_I32;
+};
+}  // namespace detail
+
+// ---------------------------------------------------------------- executor
+class HIPExecutor {
+ public:
+  using hip_executor = bool;
+  using matx_executor = bool;
+  explicit HIPExecutor(hipStream_t s = nullptr) : stream_(s) {}
+  hipStream_t getStream() const { return stream_; }
+  void sync() const { detail::hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+  // executor timers (include/matx/executors/cuda_executor_common.h:142-169)
+  void start_timer() {
+    ensure_events();
+    detail::hip_check(hipEventRecord(ev_.get()[0], stream_), "hipEventRecord");
+  }
+  void stop_timer() {
+    ensure_events();
+    detail::hip_check(hipEventRecord(ev_.get()[1], stream_), "hipEventRecord");
+  }
+  float get_time_ms() {
+    float ms = 0.f;
+    detail::hip_check(hipEventSynchronize(ev_.get()[1]), "hipEventSynchronize");
+    detail::hip_check(hipEventElapsedTime(&ms, ev_.get()[0], ev_.get()[1]), "hipEventElapsedTime");
+    return ms;
+  }
+
+ private:
+  void ensure_events() {
+    if (!ev_) {
+      ev_ = std::shared_ptr<hipEvent_t>(new hipEvent_t[2], [](hipEvent_t *e) {
+        (void)hipEventDestroy(e[0]);
+        (void)hipEventDestroy(e[1]);
+        delete[] e;
+      });
+      detail::hip_check(hipEventCreate(&ev_.get()[0]), "hipEventCreate");
+      detail::hip_check(hipEventCreate(&ev_.get()[1]), "hipEventCreate");
+    }
+  }
+  hipStream_t stream_;
+  std::shared_ptr<hipEvent_t> ev_;
+};
+
+// ----------------------------------------------------------------- tensors
+template <typename T, int RANK>
+class tensor_t {
+  static_assert(RANK >= 0 && RANK <= MXR_MAX_RANK, "rank");
+
+ public:
+  using value_type = T;
+  using type = T;
+  using scalar_type = T;
+  static constexpr int Rank() { return RANK; }
+
+  tensor_t() = default;
+  // Allocates a contiguous tensor (managed memory by default, as MatX's
+  // make_tensor, include/matx/core/make_tensor.h:57-58).
+  explicit tensor_t(const std::array<index_t, RANK> &shape, matxMemorySpace_t space = MATX_MANAGED_MEMORY) {
+    shape_ = shape;
+    index_t s = 1;
+    for (int d = RANK - 1; d >= 0; --d) {
+      stride_[d] = s;
+      s *= shape_[d];
+    }
+    const size_t bytes = sizeof(T) * (size_t)(s > 0 ? s : 1);
+    void *p = nullptr;
+    if (space == MATX_MANAGED_MEMORY) detail::hip_check(hipMallocManaged(&p, bytes), "hipMallocManaged");
+    else if (space == MATX_DEVICE_MEMORY) detail::hip_check(hipMalloc(&p, bytes), "hipMalloc");
+    else detail::hip_check(hipHostMalloc(&p, bytes), "hipHostMalloc");
+    const bool host = space == MATX_HOST_MALLOC_MEMORY;
+    storage_ = std::shared_ptr<void>(p, [host](void *q) {
+      if (host) (void)hipHostFree(q);
+      else (void)hipFree(q);
+    });
+    data_ = static_cast<T *>(p);
+    space_ = space;
+  }
+  tensor_t(std::shared_ptr<void> st, T *data, const std::array<index_t, RANK> &shape,
+           const std::array<index_t, RANK> &stride, matxMemorySpace_t space)
+      : storage_(std::move(st)), data_(data), shape_(shape), stride_(stride), space_(space) {}
+
+  index_t Size(int d) const { return shape_[d]; }
+  index_t Stride(int d) const { return stride_[d]; }
+  T *Data() const { return data_; }
+  const std::array<index_t, RANK> &Shape() const { return shape_; }
+  const std::array<index_t, RANK> &Strides() const { return stride_; }
+  const std::shared_ptr<void> &Storage() const { return storage_; }
+  matxMemorySpace_t Space() const { return space_; }
+  index_t TotalSize() const {
+    index_t n = 1;
+    for (int d = 0; d < RANK; ++d) n *= shape_[d];
+    return n;
+  }
+  bool IsContiguous() const {
+    index_t s = 1;
+    for (int d = RANK - 1; d >= 0; --d) {
+      if (shape_[d] != 1 && stride_[d] != s) return false;
+      s *= shape_[d];
+    }
+    return true;
+  }
+  // host element access (managed or host memory; synchronise first)
+  template <typename... I>
+  T &operator()(I... idx) const {
+    static_assert(sizeof...(I) == RANK, "index count");
+    const index_t ix[] = {(index_t)idx..., 0};
+    index_t off = 0;
+    for (int d = 0; d < RANK; ++d) off += ix[d] * stride_[d];
+    return data_[off];
+  }
+  // view with permuted dimensions (include/matx/core/tensor.h:894-921)
+  tensor_t Permute(const std::array<int, RANK> &dims) const {
+    std::array<index_t, RANK> sh, st;
+    for (int d = 0; d < RANK; ++d) {
+      sh[d] = shape_[dims[d]];
+      st[d] = stride_[dims[d]];
+    }
+    return tensor_t(storage_, data_, sh, st, space_);
+  }
+  // broadcast view of higher rank: matxKeepDim keeps this tensor's dims in
+  // order, other entries are new zero-stride dims (tensor.h:1101-1114)
+  template <int N>
+  tensor_t<T, N> Clone(const std::array<index_t, N> &shape) const {
+    std::array<index_t, N> sh, st;
+    int k = 0;
+    for (int d = 0; d < N; ++d) {
+      if (shape[d] == matxKeepDim) {
+        sh[d] = shape_[k];
+        st[d] = stride_[k];
+        ++k;
+      } else {
+        sh[d] = shape[d];
+        st[d] = 0;
+      }
+    }
+    if (k != RANK) throw matxException(MXR_INVALID_DIM, "Clone: matxKeepDim count != rank");
+    return tensor_t<T, N>(storage_, data_, sh, st, space_);
+  }
+  void SetVals(std::initializer_list<T> v) {
+    static_assert(RANK == 1, "SetVals(list) on rank 1");
+    std::vector<T> h(v);
+    put_host(h);
+  }
+  void SetVals(std::initializer_list<std::initializer_list<T>> v) {
+    static_assert(RANK == 2, "SetVals(list of lists) on rank 2");
+    std::vector<T> h;
+    for (auto &r : v) h.insert(h.end(), r.begin(), r.end());
+    put_host(h);
+  }
+  void PrefetchDevice(hipStream_t s) const {
+    if (space_ != MATX_MANAGED_MEMORY) return;
+    int dev = 0;
+    detail::hip_check(hipGetDevice(&dev), "hipGetDevice");
+    (void)hipMemPrefetchAsync(data_, sizeof(T) * (size_t)TotalSize(), dev, s);
+  }
+  mxr_tensor desc() const {
+    mxr_tensor t{};
+    t.data = (void *)data_;
+    t.dtype = detail::dtype_of<T>::value;
+    if (RANK == 0) {  // a scalar travels as a one-element vector
+      t.rank = 1;
+      t.shape[0] = 1;
+      t.stride[0] = 1;
+      return t;
+    }
+    t.rank = RANK;
+    for (int d = 0; d < RANK; ++d) {
+      t.shape[d] = shape_[d];
+      t.stride[d] = stride_[d];
+    }
+    return t;
+  }
+
+  template <typename Op>
+  auto operator=(const Op &op) const;
+
+ private:
+  void put_host(const std::vector<T> &h) {
+    if ((index_t)h.size() != TotalSize() || !IsContiguous())
+      throw matxException(MXR_INVALID_SIZE, "SetVals: value count does not match the tensor");
+    detail::hip_check(hipMemcpy(data_, h.data(), sizeof(T) * h.size(), hipMemcpyDefault), "hipMemcpy");
+  }
+  std::shared_ptr<void> storage_;
+  T *data_ = nullptr;
+  std::array<index_t, RANK> shape_{}, stride_{};
+  matxMemorySpace_t space_ = MATX_MANAGED_MEMORY;
+};
+
+template <typename T, int RANK>
+void make_tensor(tensor_t<T, RANK> &t, const std::array<index_t, RANK> &shape,
+                 matxMemorySpace_t space = MATX_MANAGED_MEMORY) {
+  t = tensor_t<T, RANK>(shape, space);
+}
+template <typename T>
+void make_tensor(tensor_t<T, 0> &t, matxMemorySpace_t space = MATX_MANAGED_MEMORY) {
+  t = tensor_t<T, 0>(std::array<index_t, 0>{}, space);
+}
+template <typename T, int RANK>
+tensor_t<T, RANK> make_tensor(const std::array<index_t, RANK> &shape,
+                              matxMemorySpace_t space = MATX_MANAGED_MEMORY) {
+  return tensor_t<T, RANK>(shape, space);
+}
+
+// slice(t, starts, ends): matxEnd means "to the end", matxDropDim in ends
+// removes the dimension (OutRank < RANK) (include/matx/operators/slice.h).
+template <int OUT = -1, typename T, int RANK>
+auto slice(const tensor_t<T, RANK> &t, const std::array<index_t, RANK> &starts,
+           const std::array<index_t, RANK> &ends) {
+  constexpr int R = OUT < 0 ? RANK : OUT;
+  std::array<index_t, R> sh{}, st{};
+  index_t off = 0;
+  int k = 0;
+  for (int d = 0; d < RANK; ++d) {
+    const index_t b = starts[d];
+    off += b * t.Stride(d);
+    if (ends[d] == matxDropDim) continue;
+    const index_t e = ends[d] == matxEnd ? t.Size(d) : ends[d];
+    if (b < 0 || e > t.Size(d) || e < b) throw matxException(MXR_INVALID_SIZE, "slice out of range");
+    if (k >= R) throw matxException(MXR_INVALID_DIM, "slice: output rank mismatch");
+    sh[k] = e - b;
+    st[k] = t.Stride(d);
+    ++k;
+  }
+  if (k != R) throw matxException(MXR_INVALID_DIM, "slice: output rank mismatch");
+  return tensor_t<T, R>(t.Storage(), t.Data() + off, sh, st, t.Space());
+}
+
+// --------------------------------------------------- elementwise expressions
+namespace detail {
+struct Node;
+using NodeP = std::shared_ptr<const Node>;
+struct Node {
+  enum Kind { TENSOR, CONST, GEN, OP } kind;
+  mxr_tensor t{};                 // TENSOR
+  std::shared_ptr<void> keep;     // TENSOR storage lifetime
+  cfloat c{};                     // CONST
+  int op = 0, a = 0, b = 0;       // GEN (op, dim a, length b) or OP
+  std::vector<NodeP> args;        // OP
+};
+}  // namespace detail
+
+// An elementwise expression (rank given by the output it is assigned to).
+struct Expr {
+  detail::NodeP n;
+};
+
+namespace detail {
+template <typename X>
+Expr to_expr(const X &x) {
+  if constexpr (std::is_same_v<X, Expr>) {
+    return x;
+  } else if constexpr (std::is_arithmetic_v<X>) {
+    auto n = std::make_shared<Node>();
+    n->kind = Node::CONST;
+    n->c = cfloat((float)x, 0.f);
+    return Expr{n};
+  } else if constexpr (std::is_same_v<X, cfloat>) {
+    auto n = std::make_shared<Node>();
+    n->kind = Node::CONST;
+    n->c = x;
+    return Expr{n};
+  } else {
+    auto n = std::make_shared<Node>();
+    n->kind = Node::TENSOR;
+    n->t = x.desc();
+    n->keep = x.Storage();
+    return Expr{n};
+  }
+}
+template <typename X>
+struct is_exprish : std::false_type {};
+template <>
+struct is_exprish<Expr> : std::true_type {};
+template <typename T, int R>
+struct is_exprish<tensor_t<T, R>> : std::true_type {};
+template <typename A, typename B>
+constexpr bool binop_ok = (is_exprish<A>::value || is_exprish<B>::value) &&
+                          (is_exprish<A>::value || std::is_arithmetic_v<A> || std::is_same_v<A, cfloat>) &&
+                          (is_exprish<B>::value || std::is_arithmetic_v<B> || std::is_same_v<B, cfloat>);
+inline Expr mk(int op, std::vector<Expr> a) {
+  auto n = std::make_shared<Node>();
+  n->kind = Node::OP;
+  n->op = op;
+  for (auto &e : a) n->args.push_back(e.n);
+  return Expr{n};
+}
+}  // namespace detail
+
+#define MATX_AMD_BINOP(OPSYM, CODE)                                                      \
+  template <typename A, typename B, std::enable_if_t<detail::binop_ok<A, B>, int> = 0> \
+  Expr operator OPSYM(const A &a, const B &b) {                                        \
+    return detail::mk(CODE, {detail::to_expr(a), detail::to_expr(b)});                 \
+  }
+MATX_AMD_BINOP(+, MXR_EW_ADD)
+MATX_AMD_BINOP(-, MXR_EW_SUB)
+MATX_AMD_BINOP(*, MXR_EW_MUL)
+MATX_AMD_BINOP(/, MXR_EW_DIV)
+MATX_AMD_BINOP(>, MXR_EW_GT)
+MATX_AMD_BINOP(<, MXR_EW_LT)
+MATX_AMD_BINOP(>=, MXR_EW_GE)
+MATX_AMD_BINOP(<=, MXR_EW_LE)
+#undef MATX_AMD_BINOP
+
+template <typename A>
+Expr conj(const A &a) { return detail::mk(MXR_EW_CONJ, {detail::to_expr(a)}); }
+template <typename A>
+Expr abs2(const A &a) { return detail::mk(MXR_EW_ABS2, {detail::to_expr(a)}); }
+template <typename A>
+Expr abs(const A &a) { return detail::mk(MXR_EW_ABS, {detail::to_expr(a)}); }
+template <typename A>
+Expr sqrt(const A &a) { return detail::mk(MXR_EW_SQRT, {detail::to_expr(a)}); }
+template <typename A>
+Expr exp(const A &a) { return detail::mk(MXR_EW_EXP, {detail::to_expr(a)}); }
+template <typename A, typename B>
+Expr pow(const A &a, const B &b) { return detail::mk(MXR_EW_POW, {detail::to_expr(a), detail::to_expr(b)}); }
+// IF(cond) a else b
+template <typename C, typename A, typename B>
+Expr select(const C &c, const A &a, const B &b) {
+  return detail::mk(MXR_EW_SELECT, {detail::to_expr(a), detail::to_expr(b), detail::to_expr(c)});
+}
+// hamming<Dim>(shape): 0.54 - 0.46 cos(2 pi i / (n-1)) along Dim, n = shape[Dim]
+// (include/matx/generators/hamming.h:146-178)
+template <int Dim, int Rank = 1, typename T = float>
+Expr hamming(const std::array<index_t, Rank> &shape) {
+  auto n = std::make_shared<detail::Node>();
+  n->kind = detail::Node::GEN;
+  n->op = MXR_EW_HAMMING;
+  n->a = Dim;
+  n->b = (int)shape[Dim];
+  return Expr{n};
+}
+// ones(shape) (include/matx/generators/ones.h)
+template <typename T = float, int Rank = 1>
+Expr ones(const std::array<index_t, Rank> &) { return detail::to_expr(1.0f); }
+
+// --------------------------------------------------------- transform ops
+namespace detail {
+struct FFTSpec {
+  Expr in;
+  std::vector<index_t> in_shape;  // shape of `in` when it must be materialised
+  index_t n = 0;
+  int dir = MXR_FFT_FORWARD;
+  int norm = (int)FFTNorm::BACKWARD;
+  int axis = -1;
+};
+}  // namespace detail
+
+struct FFTOp {
+  detail::FFTSpec s;
+};
+struct FFTConvOp {
+  Expr x, w;
+  index_t n;
+  int conj_w;
+};
+struct Conv1DOp {
+  Expr in, f;
+  int mode;
+};
+struct Conv2DOp {
+  Expr in, f;
+  int mode;
+};
+struct SumOp {
+  Expr in;
+  std::vector<index_t> in_shape;
+};
+
+template <typename T, int R>
+FFTOp fft(const tensor_t<T, R> &a, index_t n = 0, FFTNorm norm = FFTNorm::BACKWARD) {
+  return FFTOp{{detail::to_expr(a), {}, n, MXR_FFT_FORWARD, (int)norm, R - 1}};
+}
+template <typename T, int R>
+FFTOp ifft(const tensor_t<T, R> &a, index_t n = 0, FFTNorm norm = FFTNorm::BACKWARD) {
+  return FFTOp{{detail::to_expr(a), {}, n, MXR_FFT_INVERSE, (int)norm, R - 1}};
+}
+// fft along an explicit axis: fft(a, {axis}) (include/matx/operators/fft.h:565-577)
+template <typename T, int R>
+FFTOp fft(const tensor_t<T, R> &a, std::array<int, 1> axis, index_t n = 0, FFTNorm norm = FFTNorm::BACKWARD) {
+  return FFTOp{{detail::to_expr(a), {}, n, MXR_FFT_FORWARD, (int)norm, axis[0]}};
+}
+template <typename T, int R>
+FFTOp ifft(const tensor_t<T, R> &a, std::array<int, 1> axis, index_t n = 0, FFTNorm norm = FFTNorm::BACKWARD) {
+  return FFTOp{{detail::to_expr(a), {}, n, MXR_FFT_INVERSE, (int)norm, axis[0]}};
+}
+// out = ifft(fft(x, n) * conj(fft(w, n)))  (circular cross-correlation)
+template <typename X, typename W>
+FFTConvOp fft_conv(const X &x, const W &w, index_t n, bool conj_w = true) {
+  return FFTConvOp{detail::to_expr(x), detail::to_expr(w), n, conj_w ? 1 : 0};
+}
+template <typename X, typename F>
+Conv1DOp conv1d(const X &x, const F &f, matxConvCorrMode_t mode) {
+  return Conv1DOp{detail::to_expr(x), detail::to_expr(f), (int)mode};
+}
+template <typename X, typename F>
+Conv2DOp conv2d(const X &x, const F &f, matxConvCorrMode_t mode) {
+  return Conv2DOp{detail::to_expr(x), detail::to_expr(f), (int)mode};
+}
+template <typename T, int R>
+SumOp sum(const tensor_t<T, R> &a) {
+  return SumOp{detail::to_expr(a), std::vector<index_t>(a.Shape().begin(), a.Shape().end())};
+}
+// sum(expr) needs the expression's shape: pass it explicitly
+inline SumOp sum(const Expr &e, std::vector<index_t> shape) { return SumOp{e, std::move(shape)}; }
+
+// ---------------------------------------------------------------- run()
+namespace detail {
+struct Lowered {
+  std::vector<mxr_ew_instr> prog;
+  std::vector<mxr_tensor> ins;
+  std::vector<float> consts;
+};
+// Aligns an input view to the output's trailing dims (zero strides elsewhere).
+inline mxr_tensor broadcast_to(const mxr_tensor &in, const mxr_tensor &out) {
+  mxr_tensor t = in;
+  t.rank = out.rank;
+  const int pad = out.rank - in.rank;
+  if (pad < 0) throw matxException(MXR_INVALID_DIM, "expression input has a higher rank than its output");
+  for (int d = out.rank - 1; d >= 0; --d) {
+    const int s = d - pad;
+    if (s >= 0 && in.shape[s] == out.shape[d]) {
+      t.shape[d] = in.shape[s];
+      t.stride[d] = in.stride[s];
+    } else if (s < 0 || in.shape[s] == 1) {
+      t.shape[d] = out.shape[d];
+      t.stride[d] = 0;
+    } else {
+      throw matxException(MXR_INVALID_SIZE, "expression input extent does not match the output");
+    }
+  }
+  return t;
+}
+inline void lower(const NodeP &n, const mxr_tensor &out, Lowered &L) {
+  switch (n->kind) {
+    case Node::TENSOR: {
+      L.ins.push_back(broadcast_to(n->t, out));
+      L.prog.push_back({MXR_EW_LOAD, (int)L.ins.size() - 1, 0});
+      break;
+    }
+    case Node::CONST:
+      L.consts.push_back(n->c.real());
+      L.consts.push_back(n->c.imag());
+      L.prog.push_back({MXR_EW_CONST, (int)L.consts.size() / 2 - 1, 0});
+      break;
+    case Node::GEN: {
+      // generator dims count from the output's leading dimension
+      L.prog.push_back({n->op, n->a, n->b});
+      break;
+    }
+    case Node::OP:
+      for (auto &a : n->args) lower(a, out, L);
+      L.prog.push_back({n->op, 0, 0});
+      break;
+  }
+}
+inline void eval_into(const mxr_tensor &out, const Expr &e, hipStream_t s) {
+  Lowered L;
+  lower(e.n, out, L);
+  std::vector<const mxr_tensor *> ptrs;
+  for (auto &t : L.ins) ptrs.push_back(&t);
+  check(mxr_ew_eval(&out, ptrs.data(), (int)ptrs.size(), L.prog.data(), (int)L.prog.size(),
+                    L.consts.empty() ? nullptr : L.consts.data(), (int)L.consts.size() / 2, s));
+}
+// A transform operand: a tensor view as is, or an expression materialised
+// into a temporary of the given shape and dtype.
+struct Operand {
+  mxr_tensor t{};
+  std::shared_ptr<void> tmp;
+};
+inline Operand operand(const Expr &e, const std::vector<index_t> &shape, int dtype, hipStream_t s) {
+  Operand o;
+  if (e.n->kind == Node::TENSOR) {
+    o.t = e.n->t;
+    return o;
+  }
+  if (shape.empty()) throw matxException(MXR_INVALID_PARAMETER, "expression operand needs an explicit shape");
+  index_t total = 1;
+  mxr_tensor t{};
+  t.rank = (int)shape.size();
+  t.dtype = dtype;
+  for (int d = t.rank - 1; d >= 0; --d) {
+    t.shape[d] = shape[d];
+    t.stride[d] = total;
+    total *= shape[d];
+  }
+  void *p = nullptr;
+  hip_check(hipMallocAsync(&p, (dtype == MXR_C64 ? 8 : 4) * (size_t)total, s), "hipMallocAsync");
+  o.tmp = std::shared_ptr<void>(p, [s](void *q) { (void)hipFreeAsync(q, s); });
+  t.data = p;
+  eval_into(t, e, s);
+  o.t = t;
+  return o;
+}
+}  // namespace detail
+
+template <typename L, typename Op>
+struct SetOp {
+  L lhs;
+  Op op;
+  void run(const HIPExecutor &ex) const { run(ex.getStream()); }
+  void run(hipStream_t s = nullptr) const {
+    const mxr_tensor out = lhs.desc();
+    using namespace detail;
+    if constexpr (std::is_same_v<Op, Expr>) {
+      eval_into(out, op, s);
+    } else if constexpr (std::is_same_v<Op, FFTOp>) {
+      auto in = operand(op.s.in, op.s.in_shape, MXR_C64, s);
+      const int axis = op.s.axis < 0 ? out.rank - 1 : op.s.axis;
+      check(mxr_fft_c2c(&out, &in.t, op.s.n, op.s.dir, op.s.norm, axis, s));
+    } else if constexpr (std::is_same_v<Op, FFTConvOp>) {
+      auto x = operand(op.x, {}, MXR_C64, s);
+      auto w = operand(op.w, {}, MXR_C64, s);
+      check(mxr_fft_conv(&out, &x.t, &w.t, op.n, op.conj_w, s));
+    } else if constexpr (std::is_same_v<Op, Conv1DOp>) {
+      auto x = operand(op.in, {}, MXR_C64, s);
+      auto f = operand(op.f, {}, MXR_F32, s);
+      check(mxr_conv1d(&out, &x.t, &f.t, op.mode, s));
+    } else if constexpr (std::is_same_v<Op, Conv2DOp>) {
+      auto x = operand(op.in, {}, MXR_F32, s);
+      auto f = operand(op.f, {}, MXR_F32, s);
+      check(mxr_conv2d(&out, &x.t, &f.t, op.mode, s));
+    } else if constexpr (std::is_same_v<Op, SumOp>) {
+      auto x = operand(op.in, op.in_shape, out.dtype, s);
+      check(mxr_sum(&out, &x.t, s));
+    }
+  }
+};
+
+template <typename T, int RANK>
+template <typename Op>
+auto tensor_t<T, RANK>::operator=(const Op &op) const {
+  if constexpr (std::is_same_v<Op, FFTOp> || std::is_same_v<Op, FFTConvOp> ||
+                std::is_same_v<Op, Conv1DOp> || std::is_same_v<Op, Conv2DOp> ||
+                std::is_same_v<Op, SumOp>) {
+    return SetOp<tensor_t, Op>{*this, op};
+  } else {
+    return SetOp<tensor_t, Expr>{*this, detail::to_expr(op)};
+  }
+}
+
+// --------------------------------------------------- fused radar stages
+// One-call stage entries (pulse compression in place, canceller + window +
+// Doppler FFT, CA-CFAR) for pipelines that do not need MatX's stage-by-stage
+// intermediate buffers.
+template <typename TY, typename TX, typename TW>
+void pulse_compress(const TY &y, const TX &x, const TW &w, index_t n, int flags, const HIPExecutor &ex) {
+  const mxr_tensor yd = y.desc(), xd = x.desc(), wd = w.desc();
+  detail::check(mxr_pulse_compress(&yd, &xd, &wd, n, flags, ex.getStream()));
+}
+
+}  // namespace matx

@@ -40,6 +40,30 @@ def _compile(src: str) -> str:
     return obj
 
 
+CPP_PROGRAMS = {
+    # C++ front-end programs (host only, g++), linked against libmxr.so
+    "simple_radar_pipeline": os.path.join(HERE, "examples", "simple_radar_pipeline.cpp"),
+    "radar_stage_test": os.path.join(ROOT, "tests", "cpp", "radar_stage_test.cpp"),
+}
+BIN = os.path.join(BUILD, "bin")
+
+
+def build_cpp(verbose: bool = True) -> None:
+    os.makedirs(BIN, exist_ok=True)
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    for name, src in CPP_PROGRAMS.items():
+        out = os.path.join(BIN, name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+               "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(rocm, "include"),
+               src, "-o", out, "-L" + HERE, "-lmxr", "-L" + os.path.join(rocm, "lib"),
+               "-lamdhip64", "-Wl,-rpath,$ORIGIN/../..", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stderr}")
+    if verbose:
+        print(f"built {len(CPP_PROGRAMS)} C++ front-end programs in {BIN}")
+
+
 def build(verbose: bool = True, jobs: int | None = None) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
@@ -54,6 +78,7 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
+    build_cpp(verbose)
     return LIB
 
 
