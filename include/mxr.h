@@ -123,6 +123,28 @@ mxr_status mxr_pulse_compress(const mxr_tensor *y, const mxr_tensor *x,
                               const mxr_tensor *w, int64_t n, int flags,
                               hipStream_t stream);
 
+/* Pulse-compression plan: the waveform preparation of PulseCompression
+ * (examples/simple_radar_pipeline.h:276-296: window, unit-energy scale,
+ * conj(fft)) done once per waveform instead of once per cube, as a library
+ * FFT plan is made once per shape (cf. the cuFFT plan cache,
+ * include/matx/transforms/fft/fft_cuda.h:285-291).
+ * create: rows of S samples (x's last axis), waveforms of L samples,
+ *   transform size n (power of two), nout lags kept, flags MXR_PC_WINDOW |
+ *   MXR_PC_NORMALIZE.  Picks the block size and allocates the spectrum on
+ *   the current device.
+ * set_waveform: computes the spectrum of w ([L] c64) on `stream`; later
+ *   executes on other streams must be ordered after it by the caller.
+ * execute: y[..., r] = mxr_pulse_compress's result for r < nout, from the
+ *   stored spectrum.  x: [..., S] c64, y: [..., nout] c64, no overlap.
+ * destroy: frees the plan (synchronising the device). */
+typedef struct mxr_pc_plan_s *mxr_pc_plan;
+mxr_status mxr_pc_plan_create(mxr_pc_plan *plan, int64_t S, int64_t L, int64_t n, int64_t nout,
+                              int flags);
+mxr_status mxr_pc_plan_set_waveform(mxr_pc_plan plan, const mxr_tensor *w, hipStream_t stream);
+mxr_status mxr_pc_plan_execute(mxr_pc_plan plan, const mxr_tensor *y, const mxr_tensor *x,
+                               hipStream_t stream);
+mxr_status mxr_pc_plan_destroy(mxr_pc_plan plan);
+
 /* ------------------------------------------------------------ pulse axis */
 /* tpc_mode value for mxr_doppler: y already holds the canceller output; all
  * of its rows feed the FFT (MatX's DopplerProcessing stage on tpcView,
@@ -164,6 +186,20 @@ typedef struct mxr_cfar_params {
 mxr_status mxr_cfar(const mxr_tensor *dets, const mxr_tensor *ba_out,
                     const mxr_tensor *pow_out, const mxr_tensor *in,
                     const mxr_cfar_params *params, hipStream_t stream);
+
+/* Pulse-axis pass and CA-CFAR fused: canceller (tpc_mode as mxr_doppler),
+ * window, nd-point Doppler FFT, |.|^2 and the CFAR of mxr_cfar in one
+ * launch; the power map stays on chip unless pow_out is given.
+ * y: [C, P, R] c64.  dets: [C, nd, R] i32; ba_out, pow_out: [C, nd, R] f32
+ * or NULL.  Replaces ThreePulseCanceller + DopplerProcessing + CFARDetections
+ * (examples/simple_radar_pipeline.h:321-357,394-436).  Fused for nd <= 128,
+ * unit range strides and the window shapes instantiated in the library;
+ * other cases run mxr_doppler then mxr_cfar with identical results (through
+ * a stream-ordered temporary when pow_out is NULL). */
+mxr_status mxr_doppler_cfar(const mxr_tensor *dets, const mxr_tensor *ba_out,
+                            const mxr_tensor *pow_out, const mxr_tensor *y, int tpc_mode,
+                            int64_t win_rows, int64_t nd, const mxr_cfar_params *params,
+                            hipStream_t stream);
 
 /* ----------------------------------------------------------- elementwise */
 /* Fused elementwise expression: out[i] = f(in_0[i], ..., in_{n-1}[i], i) for

@@ -23,8 +23,9 @@ PC_WINDOW, PC_NORMALIZE, PC_WRITEBACK = 1, 2, 4
 # Every entry point include/mxr.h declares.
 EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_enable",
            "mxr_profile_reset", "mxr_profile_query", "mxr_fft_c2c",
-           "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar",
-           "mxr_conv1d", "mxr_conv2d", "mxr_ew_eval", "mxr_sum")
+           "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar", "mxr_doppler_cfar",
+           "mxr_conv1d", "mxr_conv2d", "mxr_ew_eval", "mxr_sum", "mxr_pc_plan_create",
+           "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy")
 
 STATUS_NAMES = {0: "SUCCESS", 1: "NOT_SUPPORTED", 2: "INVALID_PARAMETER",
                 3: "INVALID_DIM", 4: "INVALID_SIZE", 5: "OUT_OF_MEMORY",
@@ -86,10 +87,15 @@ def lib():
         "mxr_pulse_compress": [T, T, T, I64, I, V],
         "mxr_doppler": [T, T, T, I, I64, I64, V],
         "mxr_cfar": [T, T, T, T, ctypes.POINTER(mxr_cfar_params), V],
+        "mxr_doppler_cfar": [T, T, T, T, I, I64, I64, ctypes.POINTER(mxr_cfar_params), V],
         "mxr_conv1d": [T, T, T, I, V],
         "mxr_conv2d": [T, T, T, I, V],
         "mxr_ew_eval": [T, ctypes.c_void_p, I, ctypes.c_void_p, I, ctypes.c_void_p, I, V],
         "mxr_sum": [T, T, V],
+        "mxr_pc_plan_create": [ctypes.POINTER(V), I64, I64, I64, I64, I],
+        "mxr_pc_plan_set_waveform": [V, T, V],
+        "mxr_pc_plan_execute": [V, T, T, V],
+        "mxr_pc_plan_destroy": [V],
     }
     for name, argt in sigs.items():
         fn = getattr(L, name)

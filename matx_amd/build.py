@@ -30,8 +30,12 @@ CFLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if incremental and os.path.exists(obj):
+        deps = [src] + glob.glob(os.path.join(CSRC, "*.hpp")) + [os.path.join(ROOT, "include", "mxr.h")]
+        if os.path.getmtime(obj) >= max(os.path.getmtime(d) for d in deps):
+            return obj
     cmd = [HIPCC, "-c", src, "-o", obj] + CFLAGS
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
@@ -64,12 +68,15 @@ def build_cpp(verbose: bool = True) -> None:
         print(f"built {len(CPP_PROGRAMS)} C++ front-end programs in {BIN}")
 
 
-def build(verbose: bool = True, jobs: int | None = None) -> str:
+def build(verbose: bool = True, jobs: int | None = None, incremental: bool = False) -> str:
+    """Compiles every csrc/*.hip (all of them unless `incremental`, which
+    skips objects newer than their source and the shared headers) and links
+    libmxr.so, then the C++ front-end programs."""
     os.makedirs(BUILD, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(lambda f: _compile(f, incremental), srcs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -83,5 +90,5 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -392,35 +392,38 @@ void clear_wc() {
 }
 CacheClearRegistrar g_reg_wc(clear_wc);
 
+// Either part may be skipped: wa == nullptr keeps the spectrum already in
+// the buffer (a plan's), pa == nullptr only (re)computes the spectrum.
 template <int M>
-mxr_status launch_pc(const WfArgs &wa, const PcArgs &pa, hipStream_t s) {
-  {
+mxr_status launch_pc(const WfArgs *wa, const PcArgs *pa, hipStream_t s) {
+  if (wa) {
     ProfScope ps("wfspec_kernel", s);
-    wfspec_kernel<M><<<dim3(1), dim3(PcCfg<M>::NT), 0, s>>>(wa);
+    wfspec_kernel<M><<<dim3(1), dim3(PcCfg<M>::NT), 0, s>>>(*wa);
   }
   MXR_HIP_TRY(hipGetLastError());
+  if (!pa || pa->groups <= 0) return MXR_SUCCESS;
   if constexpr (M == 4096) {
-    if (pa.groups > 0 && pa.bpw == 1) {
-      Pc4096Args p4{pa, pow_base_device(4096, 16), pow_base_device(4096, 256)};
+    if (pa->bpw == 1) {
+      Pc4096Args p4{*pa, pow_base_device(4096, 16), pow_base_device(4096, 256)};
       MXR_REQUIRE(p4.base1 && p4.base2, MXR_OUT_OF_MEMORY, "pulse compression: twiddle table allocation failed");
-      ProfScope ps("pc_ols_kernel", s);
-      pc4096_kernel<<<dim3((unsigned)pa.groups), dim3(256), 0, s>>>(p4);
+      ProfScope ps("pc4096_kernel", s);
+      pc4096_kernel<<<dim3((unsigned)pa->groups), dim3(256), 0, s>>>(p4);
       MXR_HIP_TRY(hipGetLastError());
       return MXR_SUCCESS;
     }
   }
-  if (pa.groups > 0) {
+  {
     ProfScope ps("pc_ols_kernel", s);
-    if (pa.bpw > 1)
-      pc_ols_kernel<M, true><<<dim3((unsigned)pa.groups), dim3(PcCfg<M>::NT), 0, s>>>(pa);
+    if (pa->bpw > 1)
+      pc_ols_kernel<M, true><<<dim3((unsigned)pa->groups), dim3(PcCfg<M>::NT), 0, s>>>(*pa);
     else
-      pc_ols_kernel<M, false><<<dim3((unsigned)pa.groups), dim3(PcCfg<M>::NT), 0, s>>>(pa);
-    MXR_HIP_TRY(hipGetLastError());
+      pc_ols_kernel<M, false><<<dim3((unsigned)pa->groups), dim3(PcCfg<M>::NT), 0, s>>>(*pa);
   }
+  MXR_HIP_TRY(hipGetLastError());
   return MXR_SUCCESS;
 }
 
-mxr_status dispatch_pc(int M, const WfArgs &wa, const PcArgs &pa, hipStream_t s) {
+mxr_status dispatch_pc(int M, const WfArgs *wa, const PcArgs *pa, hipStream_t s) {
   switch (M) {
     case 1024: return launch_pc<1024>(wa, pa, s);
     case 2048: return launch_pc<2048>(wa, pa, s);
@@ -435,14 +438,13 @@ mxr_status dispatch_pc(int M, const WfArgs &wa, const PcArgs &pa, hipStream_t s)
 
 }  // namespace
 
-// Shared body of mxr_pulse_compress and mxr_fft_conv.
-mxr_status pc_common(const char *fn, const mxr_tensor *y, const mxr_tensor *x, const mxr_tensor *w,
-                     int64_t n, int window, int normalize, int writeback, int conj_w,
-                     hipStream_t stream) {
+// Validates the signal/output pair of a pulse-compression call and fills
+// the row fields of PcArgs (x, y, rows, batch strides, seff, n, nout).
+static mxr_status pc_rows(const char *fn, const mxr_tensor *y, const mxr_tensor *x, int64_t n,
+                          PcArgs *pa, bool *in_place) {
   mxr_status st;
   if ((st = check_tensor(y, "output", MXR_C64, 1, MXR_MAX_RANK))) return st;
   if ((st = check_tensor(x, "input", MXR_C64, 1, MXR_MAX_RANK))) return st;
-  if ((st = check_tensor(w, "waveform", MXR_C64, 1, 1))) return st;
   MXR_REQUIRE(x->rank == y->rank, MXR_INVALID_DIM, "%s: input rank %d != output rank %d", fn,
               x->rank, y->rank);
   const int rank = x->rank;
@@ -450,28 +452,68 @@ mxr_status pc_common(const char *fn, const mxr_tensor *y, const mxr_tensor *x, c
     MXR_REQUIRE(x->shape[d] == y->shape[d], MXR_INVALID_SIZE, "%s: batch dim %d differs", fn, d);
   MXR_REQUIRE(x->stride[rank - 1] == 1 && y->stride[rank - 1] == 1, MXR_INVALID_PARAMETER,
               "%s: the sample axis must have unit stride", fn);
-  const int64_t S = x->shape[rank - 1];
-  const int64_t L = w->shape[0];
-  const int64_t nout = y->shape[rank - 1];
   MXR_REQUIRE(n > 0 && (n & (n - 1)) == 0, MXR_NOT_SUPPORTED,
               "%s: transform size %lld must be a power of two", fn, (long long)n);
-  MXR_REQUIRE(L >= 1 && L <= n, MXR_INVALID_SIZE, "%s: waveform length %lld not in [1, n=%lld]", fn,
-              (long long)L, (long long)n);
+  const int64_t S = x->shape[rank - 1];
+  const int64_t nout = y->shape[rank - 1];
   MXR_REQUIRE(nout <= n, MXR_INVALID_SIZE, "%s: %lld output lags exceed the transform size %lld", fn,
               (long long)nout, (long long)n);
-  MXR_REQUIRE(!views_overlap(*w, *x) && !views_overlap(*w, *y), MXR_ALIAS_ERROR,
-              "%s: waveform aliases the signal", fn);
-  bool in_place = false;
+  *in_place = false;
   if (views_overlap(*x, *y)) {
     bool same_rows = x->data == y->data;
     for (int d = 0; d < rank - 1; ++d) same_rows = same_rows && x->stride[d] == y->stride[d];
     MXR_REQUIRE(same_rows, MXR_ALIAS_ERROR,
                 "%s: output overlaps input without being the same rows (in place)", fn);
-    in_place = true;
+    *in_place = true;
   }
-  const int64_t seff = S < n ? S : n;
+  pa->x = (const float2 *)x->data;
+  pa->y = (float2 *)y->data;
+  pa->rows = 1;
+  pa->bi.nd = rank - 1;
+  for (int d = 0; d < rank - 1; ++d) {
+    pa->bi.shape[d] = x->shape[d];
+    pa->bi.sa[d] = x->stride[d];
+    pa->bi.sb[d] = y->stride[d];
+    pa->rows *= x->shape[d];
+  }
+  pa->seff = S < n ? S : n;
+  pa->n = n;
+  pa->nout = nout;
+  return MXR_SUCCESS;
+}
+
+// Block fields of PcArgs from a plan.
+static mxr_status pc_blocks(const char *fn, const PcPlan &plan, bool in_place, const float2 *wc,
+                            const float2 *twf, const float2 *twi, PcArgs *pa) {
+  pa->V = plan.V;
+  pa->nblk = plan.nblk;
+  pa->bpw = in_place ? plan.nblk : 1;
+  const int64_t gpr = (pa->nblk + pa->bpw - 1) / pa->bpw;
+  pa->groups = pa->nout > 0 ? pa->rows * gpr : 0;
+  MXR_REQUIRE(pa->groups <= 0x7fffffff, MXR_INVALID_SIZE, "%s: too many rows", fn);
+  pa->wc = wc;
+  pa->twf = twf;
+  pa->twi = twi;
+  pa->scale = (float)(1.0 / (double)plan.M);
+  return MXR_SUCCESS;
+}
+
+// Shared body of mxr_pulse_compress and mxr_fft_conv.
+mxr_status pc_common(const char *fn, const mxr_tensor *y, const mxr_tensor *x, const mxr_tensor *w,
+                     int64_t n, int window, int normalize, int writeback, int conj_w,
+                     hipStream_t stream) {
+  mxr_status st;
+  if ((st = check_tensor(w, "waveform", MXR_C64, 1, 1))) return st;
+  PcArgs pa{};
+  bool in_place = false;
+  if ((st = pc_rows(fn, y, x, n, &pa, &in_place))) return st;
+  const int64_t L = w->shape[0];
+  MXR_REQUIRE(L >= 1 && L <= n, MXR_INVALID_SIZE, "%s: waveform length %lld not in [1, n=%lld]", fn,
+              (long long)L, (long long)n);
+  MXR_REQUIRE(!views_overlap(*w, *x) && !views_overlap(*w, *y), MXR_ALIAS_ERROR,
+              "%s: waveform aliases the signal", fn);
   PcPlan plan;
-  if ((st = plan_pc(seff, L, n, nout, conj_w != 0, in_place, &plan))) return st;
+  if ((st = plan_pc(pa.seff, L, n, pa.nout, conj_w != 0, in_place, &plan))) return st;
   MXR_REQUIRE(L <= plan.M, MXR_NOT_SUPPORTED, "%s: waveform longer than the block", fn);
 
   float2 *wc = wc_buffer(plan.M, stream);
@@ -490,32 +532,107 @@ mxr_status pc_common(const char *fn, const mxr_tensor *y, const mxr_tensor *x, c
   wa.conj = conj_w;
   wa.wc = wc;
   wa.tw = twf;
+  if ((st = pc_blocks(fn, plan, in_place, wc, twf, twi, &pa))) return st;
+  return dispatch_pc(plan.M, &wa, &pa, stream);
+}
 
+// --------------------------------------------------------------- plans
+// A plan fixes (S, L, n, nout, flags), picks the block size once and owns
+// the M-point spectrum of one waveform, so a stream of cubes filtered by the
+// same waveform runs only the block kernel per cube.
+struct PcPlanObj {
+  int dev;
+  int64_t S, L, n, nout;
+  int flags;
+  PcPlan plan;
+  float2 *wc;
+  const float2 *twf, *twi;
+};
+
+static mxr_status pc_plan_create(mxr_pc_plan *out, int64_t S, int64_t L, int64_t n, int64_t nout,
+                                 int flags) {
+  MXR_REQUIRE(out != nullptr, MXR_INVALID_PARAMETER, "pc_plan_create: null plan pointer");
+  *out = nullptr;
+  MXR_REQUIRE(n > 0 && (n & (n - 1)) == 0, MXR_NOT_SUPPORTED,
+              "pc_plan_create: transform size %lld must be a power of two", (long long)n);
+  MXR_REQUIRE(S >= 1 && L >= 1 && L <= n && nout >= 0 && nout <= n, MXR_INVALID_SIZE,
+              "pc_plan_create: need S >= 1, 1 <= L <= n and 0 <= nout <= n (S=%lld L=%lld "
+              "n=%lld nout=%lld)", (long long)S, (long long)L, (long long)n, (long long)nout);
+  MXR_REQUIRE((flags & ~(MXR_PC_WINDOW | MXR_PC_NORMALIZE)) == 0, MXR_INVALID_PARAMETER,
+              "pc_plan_create: flags may hold only MXR_PC_WINDOW | MXR_PC_NORMALIZE");
+  PcPlan plan;
+  mxr_status st;
+  if ((st = plan_pc(S < n ? S : n, L, n, nout, true, false, &plan))) return st;
+  MXR_REQUIRE(L <= plan.M, MXR_NOT_SUPPORTED, "pc_plan_create: waveform longer than the block");
+  const float2 *twf = twiddles_device(plan.M, false);
+  const float2 *twi = twiddles_device(plan.M, true);
+  MXR_REQUIRE(twf && twi, MXR_OUT_OF_MEMORY, "pc_plan_create: twiddle table allocation failed");
+  if (plan.M == 4096)
+    MXR_REQUIRE(pow_base_device(4096, 16) && pow_base_device(4096, 256), MXR_OUT_OF_MEMORY,
+                "pc_plan_create: twiddle table allocation failed");
+  float2 *wc = nullptr;
+  MXR_HIP_TRY(hipMalloc(&wc, sizeof(float2) * plan.M));
+  *out = reinterpret_cast<mxr_pc_plan>(new PcPlanObj{device_id(), S, L, n, nout, flags, plan, wc, twf, twi});
+  return MXR_SUCCESS;
+}
+
+static mxr_status plan_on_device(const PcPlanObj *p, const char *fn) {
+  MXR_REQUIRE(p != nullptr, MXR_INVALID_PARAMETER, "%s: null plan", fn);
+  MXR_REQUIRE(device_id() == p->dev, MXR_INVALID_PARAMETER,
+              "%s: plan was made on device %d, current device is %d", fn, p->dev, device_id());
+  return MXR_SUCCESS;
+}
+
+static mxr_status pc_plan_set_waveform(mxr_pc_plan h, const mxr_tensor *w, hipStream_t s) {
+  const PcPlanObj *p = reinterpret_cast<const PcPlanObj *>(h);
+  mxr_status st;
+  if ((st = plan_on_device(p, "pc_plan_set_waveform"))) return st;
+  if ((st = check_tensor(w, "waveform", MXR_C64, 1, 1))) return st;
+  MXR_REQUIRE(w->shape[0] == p->L, MXR_INVALID_SIZE,
+              "pc_plan_set_waveform: waveform has %lld samples, the plan %lld",
+              (long long)w->shape[0], (long long)p->L);
+  WfArgs wa{};
+  wa.w = (float2 *)w->data;
+  wa.wstride = w->stride[0];
+  wa.L = p->L;
+  wa.window = (p->flags & MXR_PC_WINDOW) != 0;
+  wa.normalize = (p->flags & MXR_PC_NORMALIZE) != 0;
+  wa.writeback = 0;
+  wa.conj = 1;
+  wa.wc = p->wc;
+  wa.tw = p->twf;
+  return dispatch_pc(p->plan.M, &wa, nullptr, s);
+}
+
+static mxr_status pc_plan_execute(mxr_pc_plan h, const mxr_tensor *y, const mxr_tensor *x,
+                                  hipStream_t s) {
+  const PcPlanObj *p = reinterpret_cast<const PcPlanObj *>(h);
+  mxr_status st;
+  if ((st = plan_on_device(p, "pc_plan_execute"))) return st;
   PcArgs pa{};
-  pa.x = (const float2 *)x->data;
-  pa.y = (float2 *)y->data;
-  pa.rows = 1;
-  pa.bi.nd = rank - 1;
-  for (int d = 0; d < rank - 1; ++d) {
-    pa.bi.shape[d] = x->shape[d];
-    pa.bi.sa[d] = x->stride[d];
-    pa.bi.sb[d] = y->stride[d];
-    pa.rows *= x->shape[d];
-  }
-  pa.seff = seff;
-  pa.n = n;
-  pa.nout = nout;
-  pa.V = plan.V;
-  pa.nblk = plan.nblk;
-  pa.bpw = in_place ? plan.nblk : 1;
-  const int64_t gpr = (pa.nblk + pa.bpw - 1) / pa.bpw;
-  pa.groups = nout > 0 ? pa.rows * gpr : 0;
-  MXR_REQUIRE(pa.groups <= 0x7fffffff, MXR_INVALID_SIZE, "%s: too many rows", fn);
-  pa.wc = wc;
-  pa.twf = twf;
-  pa.twi = twi;
-  pa.scale = (float)(1.0 / (double)plan.M);
-  return dispatch_pc(plan.M, wa, pa, stream);
+  bool in_place = false;
+  if ((st = pc_rows("pc_plan_execute", y, x, p->n, &pa, &in_place))) return st;
+  MXR_REQUIRE(!in_place, MXR_ALIAS_ERROR,
+              "pc_plan_execute: output overlaps the input (use mxr_pulse_compress in place)");
+  MXR_REQUIRE(x->shape[x->rank - 1] == p->S && pa.nout == p->nout, MXR_INVALID_SIZE,
+              "pc_plan_execute: rows of %lld samples -> %lld lags, the plan has %lld -> %lld",
+              (long long)x->shape[x->rank - 1], (long long)pa.nout, (long long)p->S,
+              (long long)p->nout);
+  if ((st = pc_blocks("pc_plan_execute", p->plan, false, p->wc, p->twf, p->twi, &pa))) return st;
+  return dispatch_pc(p->plan.M, nullptr, &pa, s);
+}
+
+static mxr_status pc_plan_destroy(mxr_pc_plan h) {
+  PcPlanObj *p = reinterpret_cast<PcPlanObj *>(h);
+  if (!p) return MXR_SUCCESS;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(p->dev);
+  const hipError_t e = hipFree(p->wc);
+  (void)hipSetDevice(cur);
+  delete p;
+  MXR_HIP_TRY(e);
+  return MXR_SUCCESS;
 }
 
 }  // namespace mxr
@@ -536,6 +653,27 @@ mxr_status mxr_fft_conv(const mxr_tensor *out, const mxr_tensor *x, const mxr_te
   return mxr::guarded("mxr_fft_conv", [&] {
     return mxr::pc_common("fft_conv", out, x, w, n, 0, 0, 0, conj_w ? 1 : 0, stream);
   });
+}
+
+mxr_status mxr_pc_plan_create(mxr_pc_plan *plan, int64_t S, int64_t L, int64_t n, int64_t nout,
+                              int flags) {
+  return mxr::guarded("mxr_pc_plan_create",
+                      [&] { return mxr::pc_plan_create(plan, S, L, n, nout, flags); });
+}
+
+mxr_status mxr_pc_plan_set_waveform(mxr_pc_plan plan, const mxr_tensor *w, hipStream_t stream) {
+  return mxr::guarded("mxr_pc_plan_set_waveform",
+                      [&] { return mxr::pc_plan_set_waveform(plan, w, stream); });
+}
+
+mxr_status mxr_pc_plan_execute(mxr_pc_plan plan, const mxr_tensor *y, const mxr_tensor *x,
+                               hipStream_t stream) {
+  return mxr::guarded("mxr_pc_plan_execute",
+                      [&] { return mxr::pc_plan_execute(plan, y, x, stream); });
+}
+
+mxr_status mxr_pc_plan_destroy(mxr_pc_plan plan) {
+  return mxr::guarded("mxr_pc_plan_destroy", [&] { return mxr::pc_plan_destroy(plan); });
 }
 
 }  // extern "C"

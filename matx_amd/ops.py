@@ -12,6 +12,8 @@ import torch
 
 from . import _lib as L
 
+L_ = L  # alias for scopes where L is a waveform length
+
 _MODES = {"full": L.FULL, "same": L.SAME, "valid": L.VALID}
 _NORMS = {"backward": L.NORM_BACKWARD, "forward": L.NORM_FORWARD, "ortho": L.NORM_ORTHO}
 
@@ -71,6 +73,46 @@ def pulse_compress(x, w, n: int, nout: int | None = None, window: bool = True,
     return out
 
 
+class PulseCompressionPlan:
+    """Pulse compression with the waveform prepared once (mxr_pc_plan_*):
+    rows of S samples, waveforms of L samples, transform size n, nout lags
+    kept.  set_waveform(w) computes the windowed, unit-energy conj spectrum
+    (examples/simple_radar_pipeline.h:276-296); calling the plan filters a
+    cube with it, giving mxr_pulse_compress's result."""
+
+    def __init__(self, S: int, L: int, n: int, nout: int | None = None, window: bool = True,
+                 normalize: bool = True):
+        self.S, self.L, self.n = S, L, n
+        self.nout = S - L + 1 if nout is None else nout
+        flags = (L_.PC_WINDOW if window else 0) | (L_.PC_NORMALIZE if normalize else 0)
+        h = L_.ctypes.c_void_p()
+        L_.check(L_.lib().mxr_pc_plan_create(L_.ctypes.byref(h), S, L, n, self.nout, flags))
+        self._h = h
+
+    def set_waveform(self, w, stream=None):
+        L_.check(L_.lib().mxr_pc_plan_set_waveform(self._h, L_.ref(L_.desc(w)),
+                                                   L_.stream_handle(stream)))
+        self._w = w  # keep the waveform alive until the spectrum is computed
+
+    def __call__(self, x, out=None, stream=None):
+        if out is None:
+            out = torch.empty(x.shape[:-1] + (self.nout,), dtype=torch.complex64, device=x.device)
+        L_.check(L_.lib().mxr_pc_plan_execute(self._h, L_.ref(L_.desc(out)), L_.ref(L_.desc(x)),
+                                              L_.stream_handle(stream)))
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            L_.check(L_.lib().mxr_pc_plan_destroy(self._h))
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def doppler(y, nd: int, tpc: str = "same", win_rows: int | None = None, rd=None, pw=None,
             want_rd: bool = False, want_pow: bool = True, stream=None):
     """Three-pulse canceller + Hamming window + nd-point Doppler FFT (+ |.|^2).
@@ -102,6 +144,30 @@ def cfar(inp, win=(5, 13), guard=(3, 3), pfa: float = 1e-5, dets=None, ba=None, 
     p = L.mxr_cfar_params(win[0], win[1], guard[0], guard[1], float(pfa))
     L.check(L.lib().mxr_cfar(L.ref(L.desc(dets)), L.ref(L.desc(ba)), L.ref(L.desc(pw)),
                              L.ref(L.desc(inp)), L.ctypes.byref(p), L.stream_handle(stream)))
+    return dets, ba, pw
+
+
+def doppler_cfar(y, nd: int, tpc: str = "same", win_rows: int | None = None, win=(5, 13),
+                 guard=(3, 3), pfa: float = 1e-5, dets=None, ba=None, pw=None,
+                 want_ba: bool = False, want_pow: bool = False, stream=None):
+    """Canceller + window + nd-point Doppler FFT + |.|^2 + CA-CFAR in one
+    pass (ThreePulseCanceller + DopplerProcessing + CFARDetections,
+    examples/simple_radar_pipeline.h:321-357,394-436).  Returns (dets, ba,
+    pow); the power map is written only when requested."""
+    C, P, R = y.shape
+    mode = {"same": L.SAME, "valid": L.VALID, "none": L.TPC_NONE}[tpc]
+    rows = P - 2 if mode == L.VALID else P
+    win_rows = (P - 2 if mode != L.TPC_NONE else rows) if win_rows is None else win_rows
+    if dets is None:
+        dets = torch.empty((C, nd, R), dtype=torch.int32, device=y.device)
+    if ba is None and want_ba:
+        ba = torch.empty((C, nd, R), dtype=torch.float32, device=y.device)
+    if pw is None and want_pow:
+        pw = torch.empty((C, nd, R), dtype=torch.float32, device=y.device)
+    p = L.mxr_cfar_params(win[0], win[1], guard[0], guard[1], float(pfa))
+    L.check(L.lib().mxr_doppler_cfar(L.ref(L.desc(dets)), L.ref(L.desc(ba)), L.ref(L.desc(pw)),
+                                     L.ref(L.desc(y)), mode, int(win_rows), int(nd),
+                                     L.ctypes.byref(p), L.stream_handle(stream)))
     return dets, ba, pw
 
 

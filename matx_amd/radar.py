@@ -144,39 +144,64 @@ class RadarPipeline:
 
 
 class FusedRadarPipeline:
-    """Three-pass throughput form.  x: [C, P, >=S] resident input cube (only
-    the first S samples of each pulse are read, e.g. MatX's inputView
-    {C, P, N}).
+    """Two-kernel throughput form of the radar chain.  x: [C, P, S] resident
+    input cube (a view of MatX's zero-padded inputView {C, P, N} works: only
+    the first S samples of each pulse are read).
 
-    tpc="valid", nd=P (rounded up) is the BASELINE workload (126 canceller
-    rows, 128-point Doppler); tpc="same", nd=next_pow2(P, strict) is MatX's.
+    Per cube: the pulse-compression plan's block kernel (overlap-save,
+    spectrum prepared once per waveform) writes the compressed cube; the
+    fused pulse-axis pass (canceller, window, Doppler FFT, |.|^2, CA-CFAR)
+    reads it once and writes the detection mask.  The power map and the
+    background averages are written only when asked (want_pow / want_ba).
+
+    tpc="valid", nd=128 is the BASELINE workload (126 canceller rows);
+    tpc="same", nd=next_pow2(P, strict) is MatX's stage dataflow.
     """
 
     def __init__(self, C: int, P: int, S: int, L: int, nd: int | None = None,
                  tpc: str = "valid", n: int | None = None, win=CFAR_WIN, guard=CFAR_GUARD,
-                 pfa: float = PFA, device="cuda", stream=None):
+                 pfa: float = PFA, want_pow: bool = False, want_ba: bool = False,
+                 device="cuda", stream=None):
         self.C, self.P, self.S, self.L = C, P, S, L
         self.R = S - L + 1
         self.n = next_pow2(S) if n is None else n
         self.tpc = tpc
         rows = P - 2 if tpc == "valid" else P
         self.nd = next_pow2(rows) if nd is None else nd
+        self.win_rows = P - 2 if tpc != "none" else P
         self.win, self.guard, self.pfa = win, guard, pfa
         self.stream = stream
         kw = dict(device=device)
-        # intermediate rows padded to 64 elements: every row starts on a
-        # 256/512-byte boundary, so tile reads and writes cover whole lines
+        # rows padded to 64 elements: every row starts on a 256/512-byte
+        # boundary, so tile reads and writes cover whole cache lines
         self.ld = (self.R + 63) // 64 * 64
         R = self.R
+        self.plan = ops.PulseCompressionPlan(S, L, self.n, R)
         self.y = torch.empty((C, P, self.ld), dtype=torch.complex64, **kw)[:, :, :R]
-        self.pw = torch.empty((C, self.nd, self.ld), dtype=torch.float32, **kw)[:, :, :R]
         self.dets = torch.empty((C, self.nd, self.ld), dtype=torch.int32, **kw)[:, :, :R]
+        self.pw = self.ba = None
+        self.set_outputs(want_pow, want_ba)
 
-    def step(self, x, w):
+    def set_outputs(self, want_pow: bool, want_ba: bool):
+        """Choose whether each cube also writes the power map / averages."""
+        kw = dict(dtype=torch.float32, device=self.dets.device)
+        shape = (self.C, self.nd, self.ld)
+        self.pw = torch.empty(shape, **kw)[:, :, :self.R] if want_pow else None
+        self.ba = torch.empty(shape, **kw)[:, :, :self.R] if want_ba else None
+
+    def set_waveform(self, w):
+        self.plan.set_waveform(w, stream=self.stream)
+
+    def step(self, x, w=None):
+        """One cube.  Passing w re-prepares the waveform first (MatX's
+        per-call form); otherwise the spectrum from set_waveform is used."""
         s = self.stream
-        ops.pulse_compress(x, w, self.n, nout=self.R, out=self.y, stream=s)
-        ops.doppler(self.y, self.nd, tpc=self.tpc, win_rows=self.P - 2, pw=self.pw, stream=s)
-        ops.cfar(self.pw, self.win, self.guard, self.pfa, dets=self.dets, stream=s)
+        if w is not None:
+            self.set_waveform(w)
+        self.plan(x, out=self.y, stream=s)
+        ops.doppler_cfar(self.y, self.nd, tpc=self.tpc, win_rows=self.win_rows, win=self.win,
+                         guard=self.guard, pfa=self.pfa, dets=self.dets, ba=self.ba, pw=self.pw,
+                         stream=s)
         return self.dets
 
     def algorithmic_bytes(self) -> int:

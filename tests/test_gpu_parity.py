@@ -241,7 +241,7 @@ def test_model_chain_17_detections(mx, orc, matx_inputs, golden):
     x, w = matx_inputs
     C = 16
     xd = dev(x).unsqueeze(0).expand(C, -1, -1).contiguous()
-    pipe = mx.FusedRadarPipeline(C, 128, 9000, 1000, nd=256, tpc="valid")
+    pipe = mx.FusedRadarPipeline(C, 128, 9000, 1000, nd=256, tpc="valid", want_pow=True)
     dets = host(pipe.step(xd, dev(w)))
     assert dets.sum() == 17 * C
     for c in range(C):

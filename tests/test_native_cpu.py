@@ -135,6 +135,33 @@ def test_pipeline_stage_checks(L):
     assert r == 1                                # n not a power of two
 
 
+def test_doppler_cfar_checks(L):
+    lib = L.lib()
+    y = fake(L, [2, 8, 100], L.C64)
+    dets = fake(L, [2, 8, 100], L.I32, addr=0x900000)
+    p = L.mxr_cfar_params(5, 13, 3, 3, 1e-5)
+    # a 4-point Doppler FFT cannot hold 8 canceller rows (dets shaped for nd=4)
+    d4 = fake(L, [2, 4, 100], L.I32, addr=0x900000)
+    r, msg = status(L, lib.mxr_doppler_cfar, ctypes.byref(d4), None, None, ctypes.byref(y), 1, 6,
+                    4, ctypes.byref(p), None)
+    assert r == 4 and "shorter" in msg
+    r, _ = status(L, lib.mxr_doppler_cfar, ctypes.byref(dets), None, None, ctypes.byref(y), 1, 6,
+                  8, None, None)
+    assert r == 2                                # null params
+    bad = L.mxr_cfar_params(5, 13, 3, 4, 1e-5)   # even guard
+    r, _ = status(L, lib.mxr_doppler_cfar, ctypes.byref(dets), None, None, ctypes.byref(y), 1, 6,
+                  8, ctypes.byref(bad), None)
+    assert r == 2
+    d_wrong = fake(L, [2, 8, 99], L.I32, addr=0x900000)
+    r, _ = status(L, lib.mxr_doppler_cfar, ctypes.byref(d_wrong), None, None, ctypes.byref(y), 1,
+                  6, 8, ctypes.byref(p), None)
+    assert r == 4                                # output shape
+    d_alias = fake(L, [2, 8, 100], L.I32)        # same address as y
+    r, _ = status(L, lib.mxr_doppler_cfar, ctypes.byref(d_alias), None, None, ctypes.byref(y), 1,
+                  6, 8, ctypes.byref(p), None)
+    assert r == 8
+
+
 def test_header_matches_python_constants(L):
     text = open(os.path.join(ROOT, "include", "mxr.h")).read()
     assert "MXR_TPC_NONE 3" in text and L.TPC_NONE == 3
