@@ -250,47 +250,72 @@ struct WfArgs {
   const float2 *tw;
 };
 
+// One workgroup of M/16 threads.  Thread j loads samples j + m*M/16,
+// m < 16 (all loads in flight at once), which are exactly the inputs of
+// butterfly j of the first radix-16 Stockham pass, so the window, the
+// energy and the scaling are applied in registers and pass 0 starts without
+// an LDS round trip; the energy is reduced in double (wave shuffles, then
+// one LDS slot per wave); the last pass stores the (conjugated) spectrum
+// straight to memory.
 template <int M>
 __global__ __launch_bounds__(PcCfg<M>::NT) void wfspec_kernel(WfArgs a) {
   constexpr int NT = PcCfg<M>::NT;
+  constexpr int NWV = NT / 64;
   using LY = Layout<M, 1, false>;
   using SF = Seq<M, false>;
+  constexpr int K = SF::n;
+  static_assert(SF::template R<0> == 16 && M / 16 == NT, "first pass must be radix 16 over NT butterflies");
   __shared__ float2 lds[LY::LDS_ELEMS];
-  __shared__ double red[NT];
+  __shared__ double red[NWV];
+  const int tid = threadIdx.x;
 
+  Pass<LY, 16, M, NT, 1> p0;
   double acc = 0.0;
-  for (int i = threadIdx.x; i < M; i += NT) {
-    float2 v = make_float2(0.f, 0.f);
-    if (i < a.L) {
-      v = a.w[(int64_t)i * a.wstride];
-      if (a.window) v = cscale(v, hamming_f32(i, (int)a.L));
-      acc += (double)v.x * v.x + (double)v.y * v.y;
-    }
-    lds[LY::addr(0, i)] = v;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int i = tid + m * NT;
+    p0.v[0][m] = i < a.L ? a.w[(int64_t)i * a.wstride] : make_float2(0.f, 0.f);
   }
-  red[threadIdx.x] = acc;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int i = tid + m * NT;
+    float2 v = p0.v[0][m];
+    if (a.window && i < a.L) v = cscale(v, hamming_f32(i, (int)a.L));
+    acc += (double)v.x * v.x + (double)v.y * v.y;
+    p0.v[0][m] = v;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((tid & 63) == 0) red[tid >> 6] = acc;
   __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
+  double tot = 0.0;
+#pragma unroll
+  for (int q = 0; q < NWV; ++q) tot += red[q];
   if (a.normalize) {
-    const float inv = (float)(1.0 / sqrt(red[0]));
-    for (int i = threadIdx.x; i < M; i += NT) {
-      float2 v = cscale(lds[LY::addr(0, i)], inv);
-      lds[LY::addr(0, i)] = v;
-      if (a.writeback && i < a.L) a.w[(int64_t)i * a.wstride] = v;
+    const float inv = (float)(1.0 / sqrt(tot));
+#pragma unroll
+    for (int m = 0; m < 16; ++m) p0.v[0][m] = cscale(p0.v[0][m], inv);
+  }
+  if (a.writeback) {
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int i = tid + m * NT;
+      if (i < a.L) a.w[(int64_t)i * a.wstride] = p0.v[0][m];
     }
-  } else if (a.writeback) {
-    for (int i = threadIdx.x; i < a.L; i += NT) a.w[(int64_t)i * a.wstride] = lds[LY::addr(0, i)];
   }
-  __syncthreads();
-  lds_passes<LY, SF, M, NT, 1, false, 0, SF::n>(lds, a.tw);
-  for (int i = threadIdx.x; i < M; i += NT) {
-    float2 v = lds[LY::addr(0, i)];
-    if (a.conj) v.y = -v.y;
-    a.wc[i] = v;
-  }
+  p0.template compute<1, false>(nullptr);
+  p0.template store<1>([&](int t, int i, float2 x) { lds[LY::addr(t, i)] = x; });
+  wg_barrier();
+  if constexpr (K > 2) lds_passes<LY, SF, M, NT, 1, false, 1, K - 1>(lds, a.tw);
+  constexpr int RL = SF::template R<K - 1>;
+  constexpr int LSL = SF::template LS<K - 1>;
+  Pass<LY, RL, M, NT, 1> pl;
+  pl.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
+  pl.template compute<LSL, false>(a.tw + SF::template TWOFF<K - 1>);
+  pl.template store<LSL>([&](int t, int i, float2 x) {
+    if (a.conj) x.y = -x.y;
+    a.wc[i] = x;
+  });
 }
 
 // ------------------------------------------------------------------ host
