@@ -46,16 +46,17 @@ def parse():
                     help="wall budget of the CPU baseline leg (0 disables it)")
     ap.add_argument("--no-profile", action="store_true",
                     help="time without per-kernel events (roofline omitted)")
-    ap.add_argument("--workload", default="radar", choices=["radar", "fftconv"],
+    ap.add_argument("--workload", default="radar", choices=["radar", "fftconv", "large"],
                     help="radar: the 16x128x9000 pipeline (default); fftconv: fft_conv on "
-                         "4x32x2048 with a 256-sample chirp at N=4096 (us per call)")
+                         "4x32x2048 with a 256-sample chirp at N=4096 (us per call); large: "
+                         "the 64x512x32768 pipeline (8.6 GB cube)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 512 MiB device copies (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
 
 
-def synth_cube(C, P, S, L, nd, n_targets, snr_db, seed, device):
+def synth_cube(C, P, S, L, nd, n_targets, snr_db, seed, device, bw=0.4):
     """Noise (E|n|^2 = 1) plus point-target echoes of an unwindowed LFM."""
     import torch
     g = torch.Generator(device="cpu").manual_seed(seed)
@@ -69,7 +70,6 @@ def synth_cube(C, P, S, L, nd, n_targets, snr_db, seed, device):
     x.copy_(torch.randn((C, P, S), dtype=torch.complex64, device=device,
                         generator=torch.Generator(device=device).manual_seed(seed)))
     n = torch.arange(L, dtype=torch.float64)
-    bw = 0.4
     ph = 2 * math.pi * (-bw / 2 * n + 0.5 * bw / L * n * n)
     lfm = torch.polar(torch.ones(L, dtype=torch.float64), ph)
     targets = []
@@ -124,8 +124,9 @@ def cpu_baseline_leg(x0, w, pow0, dets0, nd, budget_s, C, mask):
     return cpu, parity
 
 
-def load_traffic(kernel):
-    p = os.path.join(ROOT, "profiles", f"{kernel}_radar16x128x9000_traffic.json")
+def load_traffic(kernel, workload="radar"):
+    tag = {"radar": "radar16x128x9000", "large": "radar64x512x32768"}[workload]
+    p = os.path.join(ROOT, "profiles", f"{kernel}_{tag}_traffic.json")
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f).get("hbm_bytes_per_launch")
@@ -248,17 +249,46 @@ def main():
     if args.workload == "fftconv":
         bench_fftconv(args)
         return
+    bench_pipeline(args, WORKLOADS[args.workload])
+
+
+# (Doppler, range) window and guard sizes.  radar: 13 range x 5 Doppler
+# window with 2 x 1 guard cells around the cell under test; large: 21 x 7.
+WORKLOADS = {
+    "radar": dict(C=16, P=128, S=9000, L=1000, nd=128, bw=0.4, win=(5, 13), guard=(3, 5),
+                  targets=20, snr=(0.0, 20.0), steps=None,
+                  metric="cubes/s (16x128x9000 c64 radar pipeline)",
+                  desc="radar pipeline 16ch x 128 pulses x 9000 samples c64, L=1000 LFM (0.4 fs), "
+                       "N=16384 pulse compression, [1,-2,1] canceller (126 rows), Hamming + "
+                       "128-pt Doppler FFT, CA-CFAR 13x5 window / 2x1 guard cells (5x3 hole), "
+                       "Pfa 1e-5; waveform spectrum prepared once per waveform"),
+    "large": dict(C=64, P=512, S=32768, L=4096, nd=512, bw=0.5, win=(7, 21), guard=(3, 5),
+                  targets=200, snr=(0.0, 25.0), steps=20,
+                  metric="cubes/s (64x512x32768 c64 radar pipeline)",
+                  desc="radar pipeline 64ch x 512 pulses x 32768 samples c64 (8.6 GB), L=4096 "
+                       "LFM (0.5 fs), N=32768 pulse compression, [1,-2,1] canceller (510 rows), "
+                       "Hamming + 512-pt Doppler FFT, CA-CFAR 21x7 window / 2x1 guard cells, "
+                       "Pfa 1e-5; waveform spectrum prepared once per waveform"),
+}
+
+
+def bench_pipeline(args, cfg):
+    import torch
     import matx_amd
     from matx_amd import _lib
 
     dev = "cuda:0"
-    C, P, S, L = 16, 128, 9000, 1000
-    nd = 128
-    win, guard = (5, 13), (3, 5)   # (Doppler, range): 13 x 5 window, 2 x 1 guard cells
-    x, w, targets = synth_cube(C, P, S, L, nd, 20, (0.0, 20.0), args.seed, dev)
-    pipe = matx_amd.FusedRadarPipeline(C, P, S, L, nd=nd, tpc="valid", win=win, guard=guard)
+    C, P, S, L, nd = cfg["C"], cfg["P"], cfg["S"], cfg["L"], cfg["nd"]
+    win, guard = cfg["win"], cfg["guard"]
+    steps = args.steps if cfg["steps"] is None else min(args.steps, cfg["steps"])
+    warmup = args.warmup if cfg["steps"] is None else min(args.warmup, 3)
+    x, w, targets = synth_cube(C, P, S, L, nd, cfg["targets"], cfg["snr"], args.seed, dev,
+                               bw=cfg["bw"])
+    fused = nd <= 128
+    pipe = matx_amd.FusedRadarPipeline(C, P, S, L, nd=nd, tpc="valid", win=win, guard=guard,
+                                       want_pow=not fused)
     pipe.set_waveform(w)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         pipe.step(x)
     torch.cuda.synchronize()
 
@@ -271,37 +301,42 @@ def main():
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    t_wall = time.perf_counter()
     e0.record(stream)
-    for _ in range(args.steps):
+    for _ in range(steps):
         pipe.step(x)
     e1.record(stream)
     torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall
-    ms = e0.elapsed_time(e1) / args.steps
+    ms = e0.elapsed_time(e1) / steps
     if prof:
         _lib.check(lib.mxr_profile_enable(0))
 
     B = pipe.algorithmic_bytes()
     value = 1000.0 / ms
+    R = S - L + 1
+    # algorithmic bytes per launch of each kernel: pulse compression reads x
+    # and writes the compressed cube; the fused pulse-axis + CFAR pass reads
+    # the compressed cube and writes the int32 detections; unfused, the
+    # pulse-axis pass writes the power map and the CFAR kernel reads it
+    kbytes = {"pc4096_kernel": 8 * C * P * S + 8 * C * P * R,
+              "pc_ols_kernel": 8 * C * P * S + 8 * C * P * R,
+              "dopcfar_kernel": 8 * C * P * R + 4 * C * nd * R,
+              "doppler_kernel": 8 * C * P * R + 4 * C * nd * R,
+              "cfar_kernel": 4 * C * nd * R + 4 * C * nd * R}
+    if fused:
+        kbytes.pop("doppler_kernel")
+        kbytes.pop("cfar_kernel")   # the name also matches dopcfar_kernel
     kern = {}
     if prof:
-        for k in ("wfspec_kernel", "pc4096_kernel", "pc_ols_kernel", "dopcfar_kernel"):
+        for k in ["wfspec_kernel"] + list(kbytes):
             n, tot = _lib.profile_query(k)
             kern[k] = {"launches": n, "avg_us": 1000.0 * tot / max(n, 1)}
-    R = S - L + 1
-    # algorithmic bytes per launch: pulse compression reads x and writes the
-    # compressed cube; the fused pulse-axis + CFAR pass reads the compressed
-    # cube and writes the int32 detections
-    kbytes = {"pc4096_kernel": 8 * C * P * S + 8 * C * P * R,
-              "dopcfar_kernel": 8 * C * P * R + 4 * C * nd * R}
     rooflines = {}
     for k, b in kbytes.items():
         if prof and kern.get(k, {}).get("launches"):
             ach = b / (kern[k]["avg_us"] * 1e-6) / 1e9
             rooflines[k] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                            "traffic": load_traffic(k), "kernel": k,
+                            "traffic": load_traffic(k, args.workload), "kernel": k,
                             "algorithmic_bytes_per_launch": b,
                             "avg_us": round(kern[k]["avg_us"], 2)}
     dominant = max(rooflines, key=lambda k: kern[k]["avg_us"]) if rooflines else None
@@ -313,35 +348,29 @@ def main():
     torch.cuda.synchronize()
     dets = pipe.dets
     det_cells = int(dets.sum().item())
-    # every injected target with a Doppler bin away from zero survives the
-    # canceller; count how many land on a detection at (Doppler, range)
-    d_host = dets.cpu()
-    hit = sum(int(d_host[:, dop, rb].any()) for rb, dop, _ in targets)
+    # count the injected targets that land on a detection at (Doppler, range)
+    hit = sum(int(dets[:, dop, rb].any().item()) for rb, dop, _ in targets)
 
     cpu, parity = None, None
     if args.cpu_seconds > 0:
         cpu, parity = cpu_baseline_leg(x[0].cpu().numpy(), w.cpu().numpy(),
-                                       pipe.pw[0].cpu().numpy(), d_host[0].numpy(), nd,
+                                       pipe.pw[0].cpu().numpy(), dets[0].cpu().numpy(), nd,
                                        args.cpu_seconds, C, cfar_mask(win, guard))
 
     out = {
-        "metric": "cubes/s (16x128x9000 c64 radar pipeline)",
+        "metric": cfg["metric"],
         "value": round(value, 2),
         "unit": "cubes/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(ms, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64",
-        "data": "synthetic (noise + 20 LFM point targets, seeded)",
-        "config": {"workload": "radar pipeline 16ch x 128 pulses x 9000 samples c64, "
-                               "L=1000 LFM, N=16384 pulse compression, [1,-2,1] canceller "
-                               "(126 rows), Hamming + 128-pt Doppler FFT, CA-CFAR 13x5 "
-                               "window / 2x1 guard cells (5x3 hole), Pfa 1e-5; waveform "
-                               "spectrum prepared once per waveform",
+        "data": f"synthetic (noise + {len(targets)} LFM point targets, seeded)",
+        "config": {"workload": cfg["desc"],
                    "channels": C, "pulses": P, "samples": S, "waveform": L, "doppler": nd,
                    "algorithmic_bytes_per_cube": B,
                    "pipeline_hbm_frac": round(B * value / (HBM_PEAK_GBS * 1e9), 4),

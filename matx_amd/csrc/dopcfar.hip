@@ -27,6 +27,7 @@
 //     writes one 16-byte detection quad per row.  Sums never subtract, so a
 //     strong cell in the hole cannot cancel (the same decomposition as the
 //     stand-alone CFAR kernel).
+#include <cmath>
 #include <cstdlib>
 
 #include "fft_engine.hpp"
@@ -34,15 +35,21 @@
 
 namespace mxr {
 
+constexpr int DC_MAXCELL = 255;  // largest window area with a fused instance
+
 struct DcArgs {
   const float2 *y;  // [C, P, R], unit range stride
-  int64_t ys_c, ys_p;
+  int64_t ys_c;
+  int ys_p;
   int *dets;  // [C, nd, R], unit range stride
-  int64_t ds_c, ds_k;
+  int64_t ds_c;
+  int ds_k;
   float *ba;  // optional, unit range stride
-  int64_t bs_c, bs_k;
+  int64_t bs_c;
+  int bs_k;
   float *pw;  // optional power output, unit range stride
-  int64_t ps_c, ps_k;
+  int64_t ps_c;
+  int ps_k;
   int C, P, R;
   int rows_t;    // canceller rows fed to the FFT
   int p_off;     // canceller row i is centred on pulse i + p_off
@@ -50,9 +57,22 @@ struct DcArgs {
   int no_tpc;    // y already holds the canceller output
   int tiles;     // range tiles per channel
   int vec_out;   // every output row is 16-byte aligned
-  float pfa;
   const float2 *tw;
+  // threshold factor per in-bounds cell count n: alpha(n) (exact form) or
+  // alpha(n)/n, alpha(n) = n*(pfa^(-1/n) - 1) evaluated in float on the host
+  float thr[DC_MAXCELL + 1];
 };
+
+// Element access at a 32-bit byte offset from a wave-uniform base (lets the
+// compiler use the scalar-base + 32-bit vector-offset addressing form).
+template <class T>
+__device__ __forceinline__ T ld32(const T *base, uint32_t byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+template <class T>
+__device__ __forceinline__ void st32(T *base, uint32_t byte_off, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off) = v;
+}
 
 constexpr int DC_TB = 64;   // computed range columns per tile
 constexpr int DC_NT = 512;  // threads per workgroup
@@ -91,71 +111,77 @@ __device__ __forceinline__ void dc_win4(const float *u, float (&o)[4]) {
   }
 }
 
+// Two 512-thread workgroups per CU (LDS bound): 4 waves per SIMD, so at most
+// 128 VGPRs.
 template <int ND, int HY, int HX, int EY, int EX, bool EXACT>
-__global__ __launch_bounds__(DC_NT) void dopcfar_kernel(DcArgs a) {
+__global__ __launch_bounds__(DC_NT) __attribute__((amdgpu_waves_per_eu(4)))
+void dopcfar_kernel(DcArgs a) {
   constexpr int TB = DC_TB, NT = DC_NT, NW = NT / 64;
   constexpr int HXA = DcGeom<HX>::HXA, OUT = DcGeom<HX>::OUT;
   static_assert(OUT >= 16, "window too wide for a 64-column tile");
   static_assert(ND >= NW, "Doppler size below one row per wave");
+  static_assert(2 * HY <= ND, "power tile and its zero rows must fit the complex tile");
   constexpr int SR = ND / NW;  // Doppler-input rows per wave in the load phase
   using LY = Layout<ND, TB, true>;
   using SQ = Seq<ND, false>;
   constexpr int K = SQ::n;
   constexpr int NCELL = (2 * HY + 1) * (2 * HX + 1);
+  static_assert(NCELL <= DC_MAXCELL, "window area");
   __shared__ __align__(16) float2 lds[ND * TB];
-  __shared__ float win[ND];
+  __shared__ float2 coef[ND];  // Doppler-input row i = coef.x*(y[p-1]+y[p+1]) + coef.y*y[p]
   __shared__ float thr_tab[NCELL + 1];
 
   const int G = a.tiles * a.C;
   const int g = (int)xcd_remap(blockIdx.x, G);
   const int c = g / a.tiles, tile = g - c * a.tiles;
   const int r0 = tile * OUT - HXA;  // range bin of tile column 0
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
+  // canceller [1,-2,1] and window folded into two coefficients per row:
+  // rows past the canceller output are zero, NONE passes y[p] windowed
   for (int i = threadIdx.x; i < ND; i += NT) {
     float h = 0.f;
     if (i < a.rows_t) h = i < a.win_rows ? hamming_f32(i, a.win_rows) : 1.0f;
-    win[i] = h;
+    coef[i] = a.no_tpc ? make_float2(0.f, h) : make_float2(h, -2.0f * h);
   }
-  for (int n = threadIdx.x; n <= NCELL; n += NT) {
-    const float nf = (float)n;
-    const float alpha = nf * (powf(a.pfa, -1.0f / nf) - 1.0f);
-    thr_tab[n] = EXACT ? alpha : alpha / nf;
-  }
+  for (int n = threadIdx.x; n <= NCELL; n += NT) thr_tab[n] = a.thr[n];
   __syncthreads();
 
-  // ---- 1. canceller + window in registers, one LDS write per element
+  // ---- 1. canceller + window in registers, one LDS write per element.
+  // Indices are clamped into the cube so every load is unconditional; the
+  // values outside it are replaced by zeros afterwards.
   {
     const int r = r0 + lane;
     const bool rok = r >= 0 && r < a.R;
-    const float2 *col = a.y + (int64_t)c * a.ys_c + (rok ? r : 0);
+    const int rc = min(max(r, 0), a.R - 1);
+    const float2 *chan = a.y + (int64_t)c * a.ys_c;
     const int i0 = wv * SR;
     const int pb = i0 + a.p_off - 1;  // pulse held in v[0]
     float2 v[SR + 2];
 #pragma unroll
     for (int u = 0; u < SR + 2; ++u) {
       const int p = pb + u;
-      v[u] = (rok && p >= 0 && p < a.P) ? col[(int64_t)p * a.ys_p] : make_float2(0.f, 0.f);
+      const int pc = min(max(p, 0), a.P - 1);
+      const float2 t = ld32(chan, (uint32_t)(rc + pc * a.ys_p) * 8u);
+      const bool in = rok && p >= 0 && p < a.P;
+      v[u] = make_float2(in ? t.x : 0.f, in ? t.y : 0.f);
     }
 #pragma unroll
     for (int u = 0; u < SR; ++u) {
       const int i = i0 + u;
-      float2 x;
-      if (i >= a.rows_t) {
-        x = make_float2(0.f, 0.f);
-      } else if (a.no_tpc) {
-        x = cscale(v[u + 1], win[i]);
-      } else {
-        x = make_float2(v[u].x + v[u + 2].x - 2.0f * v[u + 1].x,
-                        v[u].y + v[u + 2].y - 2.0f * v[u + 1].y);
-        x = cscale(x, win[i]);
-      }
+      const float2 cf = coef[i];
+      const float2 x = make_float2(cf.x * (v[u].x + v[u + 2].x) + cf.y * v[u + 1].x,
+                                   cf.x * (v[u].y + v[u + 2].y) + cf.y * v[u + 1].y);
       lds[LY::addr(lane, i)] = x;
     }
   }
   wg_barrier();
 
   // ---- 2. Doppler FFT down the columns; the last pass leaves |X|^2 in LDS
+  // as rows 0..ND-1 of a power tile with HY zero rows above and below (the
+  // CFAR's zero padding along Doppler)
+  float *const pwt = reinterpret_cast<float *>(lds) + HY * TB;  // row 0
   if constexpr (K > 1) lds_passes<LY, SQ, ND, NT, TB, false, 0, K - 1>(lds, a.tw);
   {
     constexpr int RL = SQ::template R<K - 1>;
@@ -163,111 +189,133 @@ __global__ __launch_bounds__(DC_NT) void dopcfar_kernel(DcArgs a) {
     Pass<LY, RL, ND, NT, TB> pl;
     pl.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
     wg_barrier();  // the power map overwrites the tile
+    for (int i = threadIdx.x; i < 2 * HY * TB; i += NT) {
+      const int row = i / TB < HY ? i / TB - HY : ND + i / TB - HY;
+      pwt[row * TB + (i % TB)] = 0.f;
+    }
     pl.template compute<LSL, false>(a.tw + SQ::template TWOFF<K - 1>);
-    float *pwl = reinterpret_cast<float *>(lds);
-    pl.template store<LSL>([&](int t, int k, float2 x) { pwl[k * TB + t] = cabs2(x); });
+    pl.template store<LSL>([&](int t, int k, float2 x) { pwt[k * TB + t] = cabs2(x); });
   }
   wg_barrier();
 
-  // ---- 3. CFAR over the inner columns
+  // ---- 3. CFAR over the inner columns: 16 column quads x NB row bands
   constexpr int GR = TB / 4;                  // column quads per row
-  constexpr int NB = NT / GR;                 // row bands
-  constexpr int RB = ND >= NB ? ND / NB : 1;  // rows per band
+  constexpr int NB = ND < 32 ? ND : 32;       // row bands
+  constexpr int RB = ND / NB;                 // rows per band
   constexpr int RING = 2 * HY + 1;
   constexpr int NQ = 1 + 2 * HXA / 4;         // quads read per row
   constexpr int NV = 4 * NQ;
-  const float *pwl = reinterpret_cast<const float *>(lds);
   const int grp = threadIdx.x % GR, band = threadIdx.x / GR;
   const int tc0 = 4 * grp;  // first tile column of the quad
   const int k0 = band * RB;
-  if (k0 >= ND || tc0 < HXA || tc0 >= HXA + OUT) return;  // no barrier follows
+  if (band >= NB || tc0 < HXA || tc0 >= HXA + OUT) return;  // no barrier follows
   const int gr0 = r0 + tc0;  // its range bin (a multiple of 4, >= 0)
   if (gr0 >= a.R) return;
   const int R = a.R;
 
-  int co[4], chh[4];
+  // in-bounds cell count = ro*co - rh*chh; away from the first and last HY
+  // Doppler rows ro = 2HY+1 and rh = 2EY+1, so it depends on the column only
+  int co[4], chh[4], nrm_int[4];
+  float thr_int[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int gr = gr0 + j;
     co[j] = min(gr + HX, R - 1) - max(gr - HX, 0) + 1;
     chh[j] = min(gr + EX, R - 1) - max(gr - EX, 0) + 1;
+    nrm_int[j] = (2 * HY + 1) * co[j] - (2 * EY + 1) * chh[j];
+    thr_int[j] = thr_tab[min(max(nrm_int[j], 0), NCELL)];
   }
-  const bool quad_in = gr0 + 3 < R;
-  int *const dets_c = a.dets + (int64_t)c * a.ds_c + gr0;
+  const bool vec = a.vec_out && gr0 + 3 < R;
+  int *const dets_c = a.dets + (int64_t)c * a.ds_c;
+  float *const ba_c = EXACT ? a.ba + (int64_t)c * a.bs_c : nullptr;
+  float *const pw_c = a.pw ? a.pw + (int64_t)c * a.ps_c : nullptr;
 
-  float full[RING][4], lr[RING][4], ctr[RING][4];
+  float full[RING][4], lr[RING][4];
+  constexpr int NLY = RB + 2 * HY;  // rows walked by the band
+  // one ring period unrolled per trip (ring slots stay compile-time, live
+  // ranges stay within a period)
+#pragma unroll 1
+  for (int ly0 = 0; ly0 < NLY; ly0 += RING)
 #pragma unroll
-  for (int ly = 0; ly < RB + 2 * HY; ++ly) {
-    const int k = k0 - HY + ly;
+  for (int sl = 0; sl < RING; ++sl) {
+    const int ly = ly0 + sl;
+    if (ly >= NLY) break;
+    const int k = k0 - HY + ly;  // in [-HY, ND + HY): zero rows outside the map
     float u[NV];
-    if (k >= 0 && k < ND) {
-      const float *row = pwl + k * TB + tc0 - HXA;
+    const float *row = pwt + k * TB + tc0 - HXA;
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const float4 z = *reinterpret_cast<const float4 *>(row + 4 * q);
-        u[4 * q] = z.x;
-        u[4 * q + 1] = z.y;
-        u[4 * q + 2] = z.z;
-        u[4 * q + 3] = z.w;
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < NV; ++i) u[i] = 0.f;
+    for (int q = 0; q < NQ; ++q) {
+      const float4 z = *reinterpret_cast<const float4 *>(row + 4 * q);
+      u[4 * q] = z.x;
+      u[4 * q + 1] = z.y;
+      u[4 * q + 2] = z.z;
+      u[4 * q + 3] = z.w;
     }
     // u[HXA + j + d] is tile column tc0 + j + d
     float L[4], Mh[4], Rt[4];
     dc_win4<HX - EX>(u + HXA - HX, L);
     dc_win4<2 * EX + 1>(u + HXA - EX, Mh);
     dc_win4<HX - EX>(u + HXA + EX + 1, Rt);
-    const int s = ly % RING;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      lr[s][j] = L[j] + Rt[j];
-      full[s][j] = lr[s][j] + Mh[j];
-      ctr[s][j] = u[HXA + j];
+      lr[sl][j] = L[j] + Rt[j];
+      full[sl][j] = lr[sl][j] + Mh[j];
     }
     if (ly < 2 * HY) continue;
-    const int oy = ly - HY;  // ring coordinate of the cells under test
-    const int kc = k0 + ly - 2 * HY;
-    if (kc >= ND) break;
-    const int ro = min(kc + HY, ND - 1) - max(kc - HY, 0) + 1;
-    const int rh = min(kc + EY, ND - 1) - max(kc - EY, 0) + 1;
-    int det[4];
-    float bav[4], pv[4];
+    const int kc = k0 + ly - 2 * HY;  // Doppler row of the cells under test
+    float thr[4];
+    int nrm[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int nrm = ro * co[j] - rh * chh[j];
-      const float thr = thr_tab[min(max(nrm, 0), NCELL)];
+      thr[j] = thr_int[j];
+      nrm[j] = nrm_int[j];
+    }
+    if (kc < HY || kc + HY >= ND) {  // rows whose window leaves the map
+      const int ro = min(kc + HY, ND - 1) - max(kc - HY, 0) + 1;
+      const int rh = min(kc + EY, ND - 1) - max(kc - EY, 0) + 1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        nrm[j] = ro * co[j] - rh * chh[j];
+        thr[j] = thr_tab[min(max(nrm[j], 0), NCELL)];
+      }
+    }
+    const float4 cut = *reinterpret_cast<const float4 *>(pwt + kc * TB + tc0);
+    const float cv[4] = {cut.x, cut.y, cut.z, cut.w};
+    int det[4];
+    float bav[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
       float sm = 0.f;
 #pragma unroll
       for (int dy = -HY; dy <= HY; ++dy) {
-        const int kk = (oy + dy) % RING;
+        const int kk = (sl - HY + dy + RING) % RING;  // ring slot of row kc + dy
         const float t = (dy >= -EY && dy <= EY) ? lr[kk][j] : full[kk][j];
         sm = dy == -HY ? t : sm + t;
       }
-      const float p = ctr[oy % RING][j];
-      pv[j] = p;
       if constexpr (EXACT) {
-        bav[j] = sm / (float)nrm;
-        det[j] = p > thr * bav[j] ? 1 : 0;
+        bav[j] = sm / (float)nrm[j];
+        det[j] = cv[j] > thr[j] * bav[j] ? 1 : 0;
       } else {
-        det[j] = p > thr * sm ? 1 : 0;
+        det[j] = cv[j] > thr[j] * sm ? 1 : 0;
       }
     }
-    int *drow = dets_c + (int64_t)kc * a.ds_k;
-    float *brow = EXACT ? a.ba + (int64_t)c * a.bs_c + (int64_t)kc * a.bs_k + gr0 : nullptr;
-    float *prow = a.pw ? a.pw + (int64_t)c * a.ps_c + (int64_t)kc * a.ps_k + gr0 : nullptr;
-    if (a.vec_out && quad_in) {
-      *reinterpret_cast<int4 *>(drow) = make_int4(det[0], det[1], det[2], det[3]);
-      if constexpr (EXACT) *reinterpret_cast<float4 *>(brow) = make_float4(bav[0], bav[1], bav[2], bav[3]);
-      if (prow) *reinterpret_cast<float4 *>(prow) = make_float4(pv[0], pv[1], pv[2], pv[3]);
+    const uint32_t od = (uint32_t)(kc * a.ds_k + gr0) * 4u;
+    const uint32_t ob = EXACT ? (uint32_t)(kc * a.bs_k + gr0) * 4u : 0u;
+    const uint32_t op = pw_c ? (uint32_t)(kc * a.ps_k + gr0) * 4u : 0u;
+    if (vec) {
+      *reinterpret_cast<int4 *>(reinterpret_cast<char *>(dets_c) + od) =
+          make_int4(det[0], det[1], det[2], det[3]);
+      if constexpr (EXACT)
+        *reinterpret_cast<float4 *>(reinterpret_cast<char *>(ba_c) + ob) =
+            make_float4(bav[0], bav[1], bav[2], bav[3]);
+      if (pw_c) *reinterpret_cast<float4 *>(reinterpret_cast<char *>(pw_c) + op) = cut;
     } else {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         if (gr0 + j >= R) break;
-        drow[j] = det[j];
-        if constexpr (EXACT) brow[j] = bav[j];
-        if (prow) prow[j] = pv[j];
+        st32(dets_c, od + 4u * j, det[j]);
+        if constexpr (EXACT) st32(ba_c, ob + 4u * j, bav[j]);
+        if (pw_c) st32(pw_c, op + 4u * j, cv[j]);
       }
     }
   }
@@ -389,24 +437,29 @@ mxr_status doppler_cfar_impl(const mxr_tensor *dets, const mxr_tensor *ba_out,
               "doppler_cfar: shape too large");
 
   auto unit_rows = [](const mxr_tensor *t) { return !t || t->stride[2] == 1; };
+  // the kernel addresses one channel of each tensor with 32-bit byte offsets
+  auto span_ok = [&](const mxr_tensor *t, int64_t rows, int64_t esz) {
+    return !t || (t->stride[1] >= 0 && ((rows - 1) * t->stride[1] + R) * esz < (1ll << 32));
+  };
   const bool fusable = nd <= 128 && nd >= 8 && unit_rows(y) && unit_rows(dets) &&
-                       unit_rows(ba_out) && unit_rows(pow_out);
+                       unit_rows(ba_out) && unit_rows(pow_out) && span_ok(y, P, 8) &&
+                       span_ok(dets, nd, 4) && span_ok(ba_out, nd, 4) && span_ok(pow_out, nd, 4);
   DcArgs a{};
   a.y = (const float2 *)y->data;
   a.ys_c = y->stride[0];
-  a.ys_p = y->stride[1];
+  a.ys_p = (int)y->stride[1];
   a.dets = (int *)dets->data;
   a.ds_c = dets->stride[0];
-  a.ds_k = dets->stride[1];
+  a.ds_k = (int)dets->stride[1];
   if (ba_out) {
     a.ba = (float *)ba_out->data;
     a.bs_c = ba_out->stride[0];
-    a.bs_k = ba_out->stride[1];
+    a.bs_k = (int)ba_out->stride[1];
   }
   if (pow_out) {
     a.pw = (float *)pow_out->data;
     a.ps_c = pow_out->stride[0];
-    a.ps_k = pow_out->stride[1];
+    a.ps_k = (int)pow_out->stride[1];
   }
   a.C = (int)C;
   a.P = (int)P;
@@ -415,12 +468,18 @@ mxr_status doppler_cfar_impl(const mxr_tensor *dets, const mxr_tensor *ba_out,
   a.p_off = tpc_mode == MXR_CONV_VALID ? 1 : 0;
   a.no_tpc = tpc_mode == MXR_TPC_NONE;
   a.win_rows = (int)win_rows;
-  a.pfa = pp->pfa;
   auto al16 = [](const mxr_tensor *t) {
     return !t || ((uintptr_t)t->data % 16 == 0 && t->stride[1] % 4 == 0 && t->stride[0] % 4 == 0);
   };
   a.vec_out = al16(dets) && al16(ba_out) && al16(pow_out);
   if (fusable) {
+    const int ncell = wy * wx;
+    a.thr[0] = 0.f;
+    for (int n = 1; n <= DC_MAXCELL && n <= ncell; ++n) {
+      const float nf = (float)n;
+      const float alpha = nf * (std::pow(pp->pfa, -1.0f / nf) - 1.0f);
+      a.thr[n] = ba_out ? alpha : alpha / nf;
+    }
     a.tw = twiddles_device((int)nd, false);
     MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "doppler_cfar: twiddle table allocation failed");
     const int hy = wy / 2, hx = wx / 2, ey = gy / 2, ex = gx / 2;

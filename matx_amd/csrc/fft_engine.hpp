@@ -234,7 +234,12 @@ struct Layout {
   template <int R>
   __device__ static __forceinline__ void bfly(int g, int &t, int &j) {
     constexpr int NB = N / R;
-    if constexpr (COL) {
+    if constexpr (COL && TB % 64 == 0) {
+      // a wave spans whole rows of columns: j is the same on every lane
+      // (kept in a scalar register, so twiddle reads become scalar loads)
+      t = g % TB;
+      j = __builtin_amdgcn_readfirstlane(g / TB);
+    } else if constexpr (COL) {
       t = g % TB;
       j = g / TB;
     } else {
