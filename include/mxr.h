@@ -99,13 +99,27 @@ mxr_status mxr_fft_c2c(const mxr_tensor *out, const mxr_tensor *in, int64_t n,
 /* fft_conv: out = ifft(fft(x, n) .* conj(fft(w, n)))  (conj_w = 1, circular
  * cross-correlation, the BASELINE operator) or ifft(fft(x,n) .* fft(w,n))
  * (conj_w = 0, circular convolution), along the last axis of x.
- * x: [..., S] c64; w: [L] c64; out: [..., nout] c64 with nout <= n, holding
- * lags 0..nout-1.  There is no MatX operator of this name; it is the idiom of
+ * x: [..., S] c64; w: [L] c64 (one waveform for every row) or [..., L] with
+ * x's batch dims (one waveform per row, n in [1024, 8192]); out: [..., nout]
+ * c64 with nout <= n, holding lags 0..nout-1.  There is no MatX operator of this name; it is the idiom of
  * examples/fft_conv.cu:90-108 and the conj-multiply of
  * examples/simple_radar_pipeline.h:295-300. */
 mxr_status mxr_fft_conv(const mxr_tensor *out, const mxr_tensor *x,
                         const mxr_tensor *w, int64_t n, int conj_w,
                         hipStream_t stream);
+
+/* Adjoint of mxr_fft_conv with one waveform per row (the backward pass of
+ * a loss through fft_conv; no MatX counterpart: MatX has no autodiff).
+ * gy: [..., nout] c64, the gradient of a real loss w.r.t. y in the
+ * convention g = dL/dRe(y) + i dL/dIm(y).  x: [..., S], w: [..., L] as in the
+ * forward call.  Writes, for the rows' correlation (conj_w = 1)
+ *   gx = IFFT_n(FFT_n(gy) .* FFT_n(w))[0..S),  gw = IFFT_n(FFT_n(x) .* conj(FFT_n(gy)))[0..L)
+ * and for convolution (conj_w = 0)
+ *   gx = IFFT_n(FFT_n(gy) .* conj(FFT_n(w))),  gw = IFFT_n(FFT_n(gy) .* conj(FFT_n(x))).
+ * gx or gw may be NULL.  n in [1024, 8192]. */
+mxr_status mxr_fft_conv_adjoint(const mxr_tensor *gx, const mxr_tensor *gw, const mxr_tensor *gy,
+                                const mxr_tensor *x, const mxr_tensor *w, int64_t n, int conj_w,
+                                hipStream_t stream);
 
 /* Pulse compression stage, examples/simple_radar_pipeline.h:273-301:
  *   w1 = w .* hamming(L) (if MXR_PC_WINDOW); w2 = w1/||w1|| (if
@@ -164,6 +178,23 @@ mxr_status mxr_pc_plan_destroy(mxr_pc_plan plan);
 mxr_status mxr_doppler(const mxr_tensor *pow_out, const mxr_tensor *rd_out,
                        const mxr_tensor *y, int tpc_mode, int64_t win_rows,
                        int64_t nd, hipStream_t stream);
+
+/* Doppler stage with an arbitrary real window (the differentiable form):
+ * rd = FFT_nd(taps .* x) along pulses, pow = |rd|^2.  x: [C, P, R] c64 (no
+ * canceller), taps: [P] f32; outputs as mxr_doppler. */
+mxr_status mxr_doppler_taps(const mxr_tensor *pow_out, const mxr_tensor *rd_out,
+                            const mxr_tensor *x, const mxr_tensor *taps, int64_t nd,
+                            hipStream_t stream);
+
+/* Adjoint of mxr_doppler_taps' power output (no MatX counterpart: MatX has
+ * no autodiff).  gpow: [C, nd, R] f32 = dL/dpow.  With D = FFT_nd(taps.*x),
+ * gu = F^H (2 gpow .* D) (unnormalised inverse DFT, rows p < P):
+ *   gx = taps .* gu  ([C, P, R] c64, g = dL/dRe + i dL/dIm)
+ *   gtaps[p] = sum_{c,r} Re(conj(gu[c,p,r]) x[c,p,r])  ([P] f32)
+ * D is recomputed from x (not stored).  Either output may be NULL. */
+mxr_status mxr_doppler_taps_adjoint(const mxr_tensor *gtaps, const mxr_tensor *gx,
+                                    const mxr_tensor *gpow, const mxr_tensor *x,
+                                    const mxr_tensor *taps, int64_t nd, hipStream_t stream);
 
 /* ------------------------------------------------------------------ CFAR */
 /* Cell-averaging CFAR with a rectangular training window of win_y x win_x

@@ -25,7 +25,8 @@ EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_ena
            "mxr_profile_reset", "mxr_profile_query", "mxr_fft_c2c",
            "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar", "mxr_doppler_cfar",
            "mxr_conv1d", "mxr_conv2d", "mxr_ew_eval", "mxr_sum", "mxr_pc_plan_create",
-           "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy")
+           "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy",
+           "mxr_fft_conv_adjoint", "mxr_doppler_taps", "mxr_doppler_taps_adjoint")
 
 STATUS_NAMES = {0: "SUCCESS", 1: "NOT_SUPPORTED", 2: "INVALID_PARAMETER",
                 3: "INVALID_DIM", 4: "INVALID_SIZE", 5: "OUT_OF_MEMORY",
@@ -96,6 +97,9 @@ def lib():
         "mxr_pc_plan_set_waveform": [V, T, V],
         "mxr_pc_plan_execute": [V, T, T, V],
         "mxr_pc_plan_destroy": [V],
+        "mxr_fft_conv_adjoint": [T, T, T, T, T, I64, I, V],
+        "mxr_doppler_taps": [T, T, T, T, I64, V],
+        "mxr_doppler_taps_adjoint": [T, T, T, T, T, I64, V],
     }
     for name, argt in sigs.items():
         fn = getattr(L, name)

@@ -1,0 +1,111 @@
+"""torch.autograd.Functions over the HIP kernels, for gradient-based
+waveform and Doppler-window design.
+
+FFTConv:      y = fft_conv(x, w, n) with one waveform per row; backward runs
+              mxr_fft_conv_adjoint (gx = IFFT(GY .* W), gw = IFFT(X .* conj(GY))
+              for a correlation).  With x and w the same tensor (a waveform's
+              matched filter to itself) torch adds both gradients.
+DopplerPower: pow = |FFT_nd(taps .* x)|^2 along pulses; backward runs
+              mxr_doppler_taps_adjoint, which recomputes the spectrum from x
+              instead of keeping it (the Doppler-stage recomputation that keeps
+              the saved state at the size of the input).
+
+`design_loss` is the waveform/window design objective: integrated sidelobe
+level of every waveform's matched-filter output plus a peak-sidelobe penalty
+on the Doppler spectra of a target cube.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import ops
+
+
+class FFTConv(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, w, n: int, conj: bool = True):
+        x = x.contiguous()
+        w = w.contiguous()
+        ctx.save_for_backward(x, w)
+        ctx.n, ctx.conj = n, conj
+        return ops.fft_conv(x, w, n, conj=conj)
+
+    @staticmethod
+    def backward(ctx, gy):
+        x, w = ctx.saved_tensors
+        gx, gw = ops.fft_conv_adjoint(gy.contiguous(), x, w, ctx.n, ctx.conj,
+                                      want_gx=ctx.needs_input_grad[0],
+                                      want_gw=ctx.needs_input_grad[1])
+        return gx, gw, None, None
+
+
+class DopplerPower(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, taps, nd: int):
+        x = x.contiguous()
+        taps = taps.contiguous()
+        ctx.save_for_backward(x, taps)
+        ctx.nd = nd
+        pw, _ = ops.doppler_taps(x, taps, nd)
+        return pw
+
+    @staticmethod
+    def backward(ctx, gpow):
+        x, taps = ctx.saved_tensors
+        gx, gt = ops.doppler_taps_adjoint(gpow.contiguous(), x, taps, ctx.nd,
+                                          want_gx=ctx.needs_input_grad[0],
+                                          want_gtaps=ctx.needs_input_grad[1])
+        return gx, gt, None
+
+
+def fft_conv(x, w, n: int, conj: bool = True):
+    return FFTConv.apply(x, w, n, conj)
+
+
+def doppler_power(x, taps, nd: int):
+    return DopplerPower.apply(x, taps, nd)
+
+
+def sidelobe_mask(nd: int, kd: torch.Tensor, ml: int) -> torch.Tensor:
+    """[T, nd] bool: Doppler bins farther than ml (circularly) from kd."""
+    k = torch.arange(nd, device=kd.device)[None, :]
+    d = torch.minimum((k - kd[:, None]) % nd, (kd[:, None] - k) % nd)
+    return d > ml
+
+
+def design_loss(phi, taps, cube, targets, n: int, nd: int, lam: float = 1.0, ml: int = 2,
+                q: int = 8):
+    """phi: [B, Lw] phases; taps: [P] Doppler window; cube: [C, P, R] c64;
+    targets: [T, 3] int64 (channel, range bin, Doppler bin).  Returns
+    mean ISL + lam * mean PSL (the definitions of oracle/grad.py)."""
+    s = torch.polar(torch.ones_like(phi), phi)
+    r = fft_conv(s, s, n)
+    pr = r.real ** 2 + r.imag ** 2
+    isl = (pr.sum(-1) - pr[:, 0]) / pr[:, 0]
+    pw = doppler_power(cube, taps, nd)
+    c, rb, kd = targets[:, 0], targets[:, 1], targets[:, 2]
+    cols = pw[c, :, rb]                                    # [T, nd]
+    side = torch.where(sidelobe_mask(nd, kd, ml), cols, torch.zeros_like(cols))
+    A = (side ** q).sum(-1) ** (1.0 / q)
+    M = cols.gather(1, kd[:, None])[:, 0]
+    return isl.mean() + lam * (A / M).mean()
+
+
+def make_design_problem(seed: int, B: int = 64, Lw: int = 1024, C: int = 16, P: int = 128,
+                        R: int = 4096, T: int = 24):
+    """Synthetic design problem (numpy): B phase-only waveforms of Lw samples
+    (phases uniform in [0, 2 pi)), a Hamming-initialised P-tap window, and a
+    [C, P, R] cube of unit complex noise with T Doppler tones (amplitude
+    uniform in [5, 50]) at seeded (channel, range bin, Doppler bin)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    phi = rng.uniform(0, 2 * np.pi, (B, Lw)).astype(np.float32)
+    h = np.hamming(P).astype(np.float32)
+    X = ((rng.standard_normal((C, P, R)) + 1j * rng.standard_normal((C, P, R))) /
+         np.sqrt(2)).astype(np.complex64)
+    targets = []
+    for _ in range(T):
+        c, r, kd = int(rng.integers(C)), int(rng.integers(R)), int(rng.integers(P))
+        X[c, :, r] += np.float32(rng.uniform(5, 50)) * np.exp(2j * np.pi * kd * np.arange(P) / P)
+        targets.append((c, r, kd))
+    return phi, h, X.astype(np.complex64), targets

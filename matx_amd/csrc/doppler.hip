@@ -27,6 +27,8 @@ struct DopArgs {
   int p_off;     // canceller row i is centred on pulse i + p_off
   int win_rows;  // rows < win_rows get hamming(win_rows)
   int no_tpc;    // input is already the canceller output
+  const float *taps;  // optional real window replacing hamming(win_rows)
+  int64_t taps_s;
   const float2 *tw;
 };
 
@@ -52,7 +54,8 @@ __global__ __launch_bounds__(256) void doppler_kernel(DopArgs a, int rows_buf) {
   const int r0 = blockIdx.x * TR;
   for (int i = threadIdx.x; i < ND; i += NT) {
     float h = 0.f;
-    if (i < a.rows_t) h = i < a.win_rows ? hamming_f32(i, a.win_rows) : 1.0f;
+    if (i < a.rows_t)
+      h = a.taps ? a.taps[(int64_t)i * a.taps_s] : (i < a.win_rows ? hamming_f32(i, a.win_rows) : 1.0f);
     win[i] = h;
   }
   // coalesced tile load: rows of TR range bins, 16 loads in flight per lane
@@ -137,8 +140,9 @@ static mxr_status launch_doppler(const DopArgs &a, hipStream_t s) {
   return MXR_SUCCESS;
 }
 
-mxr_status doppler_impl(const mxr_tensor *pow_out, const mxr_tensor *rd_out, const mxr_tensor *y,
-                        int tpc_mode, int64_t win_rows, int64_t nd, hipStream_t stream) {
+static mxr_status doppler_run(const mxr_tensor *pow_out, const mxr_tensor *rd_out,
+                              const mxr_tensor *y, int tpc_mode, int64_t win_rows,
+                              const mxr_tensor *taps, int64_t nd, hipStream_t stream) {
   mxr_status st;
   if ((st = check_tensor(y, "doppler input", MXR_C64, 3, 3))) return st;
   MXR_REQUIRE(pow_out || rd_out, MXR_INVALID_PARAMETER, "doppler: no output requested");
@@ -194,6 +198,13 @@ mxr_status doppler_impl(const mxr_tensor *pow_out, const mxr_tensor *rd_out, con
   a.p_off = tpc_mode == MXR_CONV_VALID ? 1 : 0;
   a.no_tpc = tpc_mode == MXR_TPC_NONE;
   a.win_rows = (int)win_rows;
+  if (taps) {
+    if ((st = check_tensor(taps, "doppler taps", MXR_F32, 1, 1))) return st;
+    MXR_REQUIRE(taps->shape[0] == rows_t, MXR_INVALID_SIZE,
+                "doppler: %lld taps for %lld rows", (long long)taps->shape[0], (long long)rows_t);
+    a.taps = (const float *)taps->data;
+    a.taps_s = taps->stride[0];
+  }
   MXR_REQUIRE(nd >= 2 && nd <= 1024 && (nd & (nd - 1)) == 0, MXR_NOT_SUPPORTED,
               "doppler: FFT size %lld must be a power of two in [2, 1024]", (long long)nd);
   a.tw = twiddles_device((int)nd, false);
@@ -213,6 +224,205 @@ mxr_status doppler_impl(const mxr_tensor *pow_out, const mxr_tensor *rd_out, con
   return MXR_NOT_SUPPORTED;
 }
 
+mxr_status doppler_impl(const mxr_tensor *pow_out, const mxr_tensor *rd_out, const mxr_tensor *y,
+                        int tpc_mode, int64_t win_rows, int64_t nd, hipStream_t stream) {
+  return doppler_run(pow_out, rd_out, y, tpc_mode, win_rows, nullptr, nd, stream);
+}
+
+// ------------------------------------------------------- window adjoint
+// Backward of pow = |FFT_nd(h .* x)|^2 along pulses, for a real window h:
+//   gD = 2 gpow D                      (torch convention g = dL/dRe + i dL/dIm)
+//   gu = F^H gD, rows p < P            (unnormalised inverse DFT)
+//   gx[c,p,r] = h[p] gu[c,p,r]
+//   gh[p] = sum_{c,r} Re(conj(gu[c,p,r]) x[c,p,r])
+// D is recomputed from x and h in the tile (no stored spectrum: the backward
+// costs one more read of x instead of a C x nd x R complex buffer).  A
+// workgroup reduces its tile's gh contributions over the range bins; a
+// second kernel sums the per-tile partials in a fixed order.
+struct DopAdjArgs {
+  const float2 *x;
+  int64_t xs_c, xs_p, xs_r;
+  const float *taps;
+  int64_t taps_s;
+  const float *gp;
+  int64_t gs_c, gs_k, gs_r;
+  float2 *gx;
+  int64_t gxs_c, gxs_p, gxs_r;
+  float *partial;  // [C * tiles][P]
+  int C, P, R, tiles;
+  const float2 *tw;
+};
+
+template <int ND>
+__global__ __launch_bounds__(256) void doppler_taps_adj_kernel(DopAdjArgs a) {
+  constexpr int NT = 256;
+  constexpr int TR = DopCfg<ND>::TR;
+  static_assert(TR <= 64 && 64 % TR == 0, "range tile must divide a wave");
+  using LY = Layout<ND, TR, true>;
+  using SQ = Seq<ND, false>;
+  constexpr int K = SQ::n;
+  extern __shared__ float2 lds[];  // ND * TR
+  __shared__ float hw[ND];
+  const int c = blockIdx.y;
+  const int r0 = blockIdx.x * TR;
+  for (int i = threadIdx.x; i < ND; i += NT) hw[i] = i < a.P ? a.taps[(int64_t)i * a.taps_s] : 0.f;
+  __syncthreads();
+  const float2 *xc = a.x + (int64_t)c * a.xs_c;
+  for (int idx = threadIdx.x; idx < ND * TR; idx += NT) {
+    const int i = idx / TR, t = idx % TR, r = r0 + t;
+    float2 v = make_float2(0.f, 0.f);
+    if (i < a.P && r < a.R) v = cscale(xc[(int64_t)i * a.xs_p + (int64_t)r * a.xs_r], hw[i]);
+    lds[LY::addr(t, i)] = v;
+  }
+  wg_barrier();
+  lds_passes<LY, SQ, ND, NT, TR, false, 0, K>(lds, a.tw);  // D
+  const float *gpc = a.gp + (int64_t)c * a.gs_c;
+  for (int idx = threadIdx.x; idx < ND * TR; idx += NT) {
+    const int k = idx / TR, t = idx % TR, r = r0 + t;
+    const float g = r < a.R ? gpc[(int64_t)k * a.gs_k + (int64_t)r * a.gs_r] : 0.f;
+    lds[LY::addr(t, k)] = cscale(lds[LY::addr(t, k)], 2.0f * g);
+  }
+  wg_barrier();
+  lds_passes<LY, SQ, ND, NT, TR, true, 0, K>(lds, a.tw);  // gu (unnormalised inverse)
+  float *part = a.partial + ((int64_t)c * a.tiles + blockIdx.x) * a.P;
+  for (int base = 0; base < a.P * TR; base += NT) {
+    const int idx = base + threadIdx.x;
+    const int p = idx / TR, t = idx % TR, r = r0 + t;
+    float contrib = 0.f;
+    if (p < a.P && r < a.R) {
+      const float2 gu = lds[LY::addr(t, p)];
+      const float2 xv = xc[(int64_t)p * a.xs_p + (int64_t)r * a.xs_r];
+      contrib = gu.x * xv.x + gu.y * xv.y;
+      if (a.gx)
+        a.gx[(int64_t)c * a.gxs_c + (int64_t)p * a.gxs_p + (int64_t)r * a.gxs_r] = cscale(gu, hw[p]);
+    }
+#pragma unroll
+    for (int o = TR / 2; o > 0; o >>= 1) contrib += __shfl_xor(contrib, o);
+    if (t == 0 && p < a.P) part[p] = contrib;
+  }
+}
+
+// gh[p] = sum over parts of partial[part][p], in part order (double).
+__global__ __launch_bounds__(256) void taps_reduce_kernel(const float *partial, int nparts, int P,
+                                                          float *gh, int64_t ghs) {
+  __shared__ double acc[4][64];
+  const int p = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int q = threadIdx.x >> 6;  // four interleaved part streams
+  double s = 0.0;
+  if (p < P)
+    for (int part = q; part < nparts; part += 4) s += (double)partial[(int64_t)part * P + p];
+  acc[q][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (q == 0 && p < P) gh[(int64_t)p * ghs] = (float)(((acc[0][threadIdx.x] + acc[1][threadIdx.x]) +
+                                                      acc[2][threadIdx.x]) + acc[3][threadIdx.x]);
+}
+
+template <int ND>
+static mxr_status launch_taps_adj(const DopAdjArgs &a, const mxr_tensor *gtaps, hipStream_t s) {
+  constexpr int TR = DopCfg<ND>::TR;
+  const size_t lds = (size_t)ND * TR * sizeof(float2);
+  MXR_REQUIRE(lds + ND * sizeof(float) <= (size_t)max_lds_per_block(), MXR_NOT_SUPPORTED,
+              "doppler_taps_adjoint: tile does not fit in LDS");
+  if (lds > 65536)
+    MXR_HIP_TRY(hipFuncSetAttribute((const void *)doppler_taps_adj_kernel<ND>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  {
+    ProfScope ps("doppler_taps_adj_kernel", s);
+    doppler_taps_adj_kernel<ND><<<dim3(a.tiles, a.C), dim3(256), lds, s>>>(a);
+  }
+  MXR_HIP_TRY(hipGetLastError());
+  if (gtaps) {
+    ProfScope ps("taps_reduce_kernel", s);
+    taps_reduce_kernel<<<dim3((a.P + 63) / 64), dim3(256), 0, s>>>(
+        a.partial, a.C * a.tiles, a.P, (float *)gtaps->data, gtaps->stride[0]);
+    MXR_HIP_TRY(hipGetLastError());
+  }
+  return MXR_SUCCESS;
+}
+
+static mxr_status doppler_taps_adjoint_impl(const mxr_tensor *gtaps, const mxr_tensor *gx,
+                                            const mxr_tensor *gpow, const mxr_tensor *x,
+                                            const mxr_tensor *taps, int64_t nd, hipStream_t s) {
+  const char *fn = "doppler_taps_adjoint";
+  mxr_status st;
+  if ((st = check_tensor(x, "x", MXR_C64, 3, 3))) return st;
+  if ((st = check_tensor(taps, "taps", MXR_F32, 1, 1))) return st;
+  if ((st = check_tensor(gpow, "gpow", MXR_F32, 3, 3))) return st;
+  MXR_REQUIRE(gtaps || gx, MXR_INVALID_PARAMETER, "%s: no gradient requested", fn);
+  const int64_t C = x->shape[0], P = x->shape[1], R = x->shape[2];
+  MXR_REQUIRE(taps->shape[0] == P, MXR_INVALID_SIZE, "%s: %lld taps for %lld pulses", fn,
+              (long long)taps->shape[0], (long long)P);
+  MXR_REQUIRE(nd >= 2 && nd <= 1024 && (nd & (nd - 1)) == 0 && nd >= P, MXR_NOT_SUPPORTED,
+              "%s: FFT size %lld must be a power of two in [max(2, P), 1024]", fn, (long long)nd);
+  MXR_REQUIRE(gpow->shape[0] == C && gpow->shape[1] == nd && gpow->shape[2] == R,
+              MXR_INVALID_SIZE, "%s: gpow must be [%lld, %lld, %lld]", fn, (long long)C,
+              (long long)nd, (long long)R);
+  if (gtaps) {
+    if ((st = check_tensor(gtaps, "gtaps", MXR_F32, 1, 1))) return st;
+    MXR_REQUIRE(gtaps->shape[0] == P, MXR_INVALID_SIZE, "%s: gtaps must have %lld entries", fn,
+                (long long)P);
+  }
+  if (gx) {
+    if ((st = check_tensor(gx, "gx", MXR_C64, 3, 3))) return st;
+    MXR_REQUIRE(gx->shape[0] == C && gx->shape[1] == P && gx->shape[2] == R, MXR_INVALID_SIZE,
+                "%s: gx must have x's shape", fn);
+  }
+  for (const mxr_tensor *o : {gtaps, gx}) {
+    if (!o) continue;
+    for (const mxr_tensor *i : {x, taps, gpow})
+      MXR_REQUIRE(!views_overlap(*o, *i), MXR_ALIAS_ERROR, "%s: an output overlaps an input", fn);
+  }
+  if (gtaps && gx) MXR_REQUIRE(!views_overlap(*gtaps, *gx), MXR_ALIAS_ERROR, "%s: outputs overlap", fn);
+  if (C == 0 || R == 0 || P == 0) {
+    return MXR_SUCCESS;
+  }
+  MXR_REQUIRE(C <= 65535 && R < (1ll << 30), MXR_INVALID_SIZE, "%s: shape too large", fn);
+  DopAdjArgs a{};
+  a.x = (const float2 *)x->data;
+  a.xs_c = x->stride[0];
+  a.xs_p = x->stride[1];
+  a.xs_r = x->stride[2];
+  a.taps = (const float *)taps->data;
+  a.taps_s = taps->stride[0];
+  a.gp = (const float *)gpow->data;
+  a.gs_c = gpow->stride[0];
+  a.gs_k = gpow->stride[1];
+  a.gs_r = gpow->stride[2];
+  if (gx) {
+    a.gx = (float2 *)gx->data;
+    a.gxs_c = gx->stride[0];
+    a.gxs_p = gx->stride[1];
+    a.gxs_r = gx->stride[2];
+  }
+  a.C = (int)C;
+  a.P = (int)P;
+  a.R = (int)R;
+  a.tw = twiddles_device((int)nd, false);
+  MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
+  switch (nd) {
+#define MXR_TA(N)                                                                              \
+  case N: {                                                                                    \
+    a.tiles = (int)((R + DopCfg<N>::TR - 1) / DopCfg<N>::TR);                                  \
+    a.partial = (float *)stream_workspace("doppler_taps_partial",                              \
+                                          sizeof(float) * (size_t)C * a.tiles * P, s);         \
+    MXR_REQUIRE(a.partial != nullptr, MXR_OUT_OF_MEMORY, "%s: workspace allocation failed", fn); \
+    return launch_taps_adj<N>(a, gtaps, s);                                                    \
+  }
+    MXR_TA(2)
+    MXR_TA(4)
+    MXR_TA(8)
+    MXR_TA(16)
+    MXR_TA(32)
+    MXR_TA(64)
+    MXR_TA(128)
+    MXR_TA(256)
+    MXR_TA(512)
+    MXR_TA(1024)
+#undef MXR_TA
+  }
+  return MXR_NOT_SUPPORTED;
+}
+
 }  // namespace mxr
 
 extern "C" mxr_status mxr_doppler(const mxr_tensor *pow_out, const mxr_tensor *rd_out,
@@ -220,5 +430,23 @@ extern "C" mxr_status mxr_doppler(const mxr_tensor *pow_out, const mxr_tensor *r
                                   hipStream_t stream) {
   return mxr::guarded("mxr_doppler", [&] {
     return mxr::doppler_impl(pow_out, rd_out, y, tpc_mode, win_rows, nd, stream);
+  });
+}
+
+extern "C" mxr_status mxr_doppler_taps(const mxr_tensor *pow_out, const mxr_tensor *rd_out,
+                                       const mxr_tensor *x, const mxr_tensor *taps, int64_t nd,
+                                       hipStream_t stream) {
+  return mxr::guarded("mxr_doppler_taps", [&] {
+    MXR_REQUIRE(taps != nullptr, MXR_INVALID_PARAMETER, "doppler_taps: null taps");
+    return mxr::doppler_run(pow_out, rd_out, x, MXR_TPC_NONE, 0, taps, nd, stream);
+  });
+}
+
+extern "C" mxr_status mxr_doppler_taps_adjoint(const mxr_tensor *gtaps, const mxr_tensor *gx,
+                                               const mxr_tensor *gpow, const mxr_tensor *x,
+                                               const mxr_tensor *taps, int64_t nd,
+                                               hipStream_t stream) {
+  return mxr::guarded("mxr_doppler_taps_adjoint", [&] {
+    return mxr::doppler_taps_adjoint_impl(gtaps, gx, gpow, x, taps, nd, stream);
   });
 }

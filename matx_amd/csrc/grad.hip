@@ -1,0 +1,241 @@
+// Row-wise correlation with one waveform per row, and the adjoints that a
+// gradient through fft_conv needs (the BASELINE's differentiable form of
+// y = ifft(fft(x, n) .* conj(fft(w, n))), examples/fft_conv.cu:90-108).
+//
+// Forward (conj_w = 1: correlation, 0: convolution), per row b:
+//   y_b = IFFT_n(X_b .* W_b')          W' = conj(W) or W,  X = FFT_n(x_b), W = FFT_n(w_b)
+// y is complex-linear in x and in conj(w) (correlation) or w (convolution).
+// For a real loss with torch's gradient convention g = dL/dRe + i dL/dIm,
+//   correlation:  gx = IFFT(GY .* W),        gw = IFFT(X .* conj(GY))
+//   convolution:  gx = IFFT(GY .* conj(W)),  gw = IFFT(GY .* conj(X))
+// with GY = FFT_n(gy), restricted to the supports of x and w.  Every one of
+// these is one more row-wise correlation or convolution, so one kernel body
+// serves the forward and both adjoints: a workgroup per row keeps two
+// n-point spectra in LDS (n <= 8192: 2 x 68 KiB), runs the Stockham passes of
+// the shared FFT engine, multiplies, inverts and stores the kept lags.
+#include <cmath>
+
+#include "fft_engine.hpp"
+#include "mxr_cache.hpp"
+
+namespace mxr {
+
+// One row operand: element i of row b at p[off(b) + i*s], i < len.
+struct RowOp {
+  const float2 *p;
+  int64_t len, s;
+  BatchIdx bi;  // sa: offsets of this operand per batch dim
+};
+
+struct XcArgs {
+  RowOp a, b;        // out_b = IFFT(FFT(a_b) .* f(FFT(b_b)))
+  int mode;          // 0: A.*conj(B), 1: A.*B, 3: |A|^2 (b unused)
+  float2 *out;
+  int64_t nout, os;  // kept lags, out element stride
+  BatchIdx obi;      // sa: out offsets per batch dim
+  const float2 *tw;
+  float scale;
+};
+
+template <int M>
+struct XcCfg {
+  static constexpr int NT = M / 16;
+};
+
+template <int M, bool INV>
+__device__ __forceinline__ void row_fft(float2 *buf, const float2 *tw) {
+  using LY = Layout<M, 1, false>;
+  using SQ = Seq<M, false>;
+  lds_passes<LY, SQ, M, XcCfg<M>::NT, 1, INV, 0, SQ::n>(buf, tw);
+}
+
+template <int M>
+__device__ __forceinline__ void load_row(float2 *buf, const RowOp &op, int64_t row) {
+  using LY = Layout<M, 1, false>;
+  int64_t off, unused;
+  batch_offsets(op.bi, row, &off, &unused);
+  const float2 *src = op.p + off;
+  for (int i = threadIdx.x; i < M; i += XcCfg<M>::NT)
+    buf[LY::addr(0, i)] = i < op.len ? src[(int64_t)i * op.s] : make_float2(0.f, 0.f);
+}
+
+template <int M>
+__global__ __launch_bounds__(XcCfg<M>::NT) void xcorr_rows_kernel(XcArgs a) {
+  using LY = Layout<M, 1, false>;
+  constexpr int NT = XcCfg<M>::NT;
+  __shared__ float2 A[LY::LDS_ELEMS];
+  __shared__ float2 B[LY::LDS_ELEMS];
+  const int64_t row = blockIdx.x;
+  load_row<M>(A, a.a, row);
+  if (a.mode != 3) load_row<M>(B, a.b, row);
+  wg_barrier();
+  row_fft<M, false>(A, a.tw);
+  if (a.mode != 3) row_fft<M, false>(B, a.tw);
+  for (int i = threadIdx.x; i < M; i += NT) {
+    const float2 x = A[LY::addr(0, i)];
+    float2 r;
+    if (a.mode == 3) {
+      r = make_float2(cabs2(x), 0.f);
+    } else {
+      const float2 y = B[LY::addr(0, i)];
+      r = a.mode == 0 ? cmulc(x, y) : cmul(x, y);
+    }
+    A[LY::addr(0, i)] = r;
+  }
+  wg_barrier();
+  row_fft<M, true>(A, a.tw);
+  int64_t off, unused;
+  batch_offsets(a.obi, row, &off, &unused);
+  float2 *dst = a.out + off;
+  for (int i = threadIdx.x; i < a.nout; i += NT) dst[(int64_t)i * a.os] = cscale(A[LY::addr(0, i)], a.scale);
+}
+
+static mxr_status xc_dispatch(int M, const XcArgs &a, int64_t rows, hipStream_t s) {
+  if (rows == 0 || a.nout == 0) return MXR_SUCCESS;
+  MXR_REQUIRE(rows <= 0x7fffffff, MXR_INVALID_SIZE, "row-wise correlation: too many rows");
+  ProfScope ps("xcorr_rows_kernel", s);
+  switch (M) {
+#define MXR_XC(MM)                                                                            \
+  case MM:                                                                                    \
+    xcorr_rows_kernel<MM><<<dim3((unsigned)rows), dim3(XcCfg<MM>::NT), 0, s>>>(a);             \
+    break;
+    MXR_XC(1024)
+    MXR_XC(2048)
+    MXR_XC(4096)
+    MXR_XC(8192)
+#undef MXR_XC
+    default:
+      set_error(MXR_NOT_SUPPORTED, "row-wise correlation: size %d not in 1024..8192", M);
+      return MXR_NOT_SUPPORTED;
+  }
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
+// Fills a RowOp for a [..., len] operand whose batch dims are `batch`.
+static mxr_status row_op(const mxr_tensor *t, const mxr_tensor *batch, const char *fn,
+                         const char *name, RowOp *op) {
+  mxr_status st;
+  if ((st = check_tensor(t, name, MXR_C64, 1, MXR_MAX_RANK))) return st;
+  const int nb = batch->rank - 1;
+  MXR_REQUIRE(t->rank == batch->rank, MXR_INVALID_DIM, "%s: %s has rank %d, expected %d", fn, name,
+              t->rank, batch->rank);
+  for (int d = 0; d < nb; ++d)
+    MXR_REQUIRE(t->shape[d] == batch->shape[d], MXR_INVALID_SIZE, "%s: %s batch dim %d differs", fn,
+                name, d);
+  op->p = (const float2 *)t->data;
+  op->len = t->shape[t->rank - 1];
+  op->s = t->stride[t->rank - 1];
+  op->bi.nd = nb;
+  for (int d = 0; d < nb; ++d) {
+    op->bi.shape[d] = t->shape[d];
+    op->bi.sa[d] = t->stride[d];
+    op->bi.sb[d] = 0;
+  }
+  return MXR_SUCCESS;
+}
+
+static int64_t batch_rows(const mxr_tensor *t) {
+  int64_t r = 1;
+  for (int d = 0; d < t->rank - 1; ++d) r *= t->shape[d];
+  return r;
+}
+
+static mxr_status xc_run(const char *fn, const mxr_tensor *out, const RowOp &A, const RowOp &B,
+                         int mode, int64_t n, hipStream_t s) {
+  MXR_REQUIRE(n >= 1024 && n <= 8192 && (n & (n - 1)) == 0, MXR_NOT_SUPPORTED,
+              "%s: per-row transform size %lld must be a power of two in [1024, 8192]", fn,
+              (long long)n);
+  XcArgs a{};
+  a.a = A;
+  a.b = B;
+  a.mode = mode;
+  a.out = (float2 *)out->data;
+  a.nout = out->shape[out->rank - 1];
+  a.os = out->stride[out->rank - 1];
+  a.obi.nd = out->rank - 1;
+  for (int d = 0; d < out->rank - 1; ++d) {
+    a.obi.shape[d] = out->shape[d];
+    a.obi.sa[d] = out->stride[d];
+  }
+  a.tw = twiddles_device((int)n, false);
+  MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
+  a.scale = (float)(1.0 / (double)n);
+  return xc_dispatch((int)n, a, batch_rows(out), s);
+}
+
+static bool same_view(const mxr_tensor *a, const mxr_tensor *b) {
+  if (a->data != b->data || a->rank != b->rank) return false;
+  for (int d = 0; d < a->rank; ++d)
+    if (a->shape[d] != b->shape[d] || a->stride[d] != b->stride[d]) return false;
+  return true;
+}
+
+// Forward with one waveform per row (called by mxr_fft_conv when w has
+// batch dims).
+mxr_status fft_conv_rows(const mxr_tensor *out, const mxr_tensor *x, const mxr_tensor *w,
+                         int64_t n, int conj_w, hipStream_t s) {
+  const char *fn = "fft_conv";
+  mxr_status st;
+  if ((st = check_tensor(out, "output", MXR_C64, 1, MXR_MAX_RANK))) return st;
+  RowOp A{}, B{};
+  if ((st = row_op(x, out, fn, "input", &A))) return st;
+  if ((st = row_op(w, out, fn, "waveform", &B))) return st;
+  MXR_REQUIRE(A.len <= n && B.len <= n && out->shape[out->rank - 1] <= n, MXR_INVALID_SIZE,
+              "%s: rows longer than the transform size %lld", fn, (long long)n);
+  MXR_REQUIRE(!views_overlap(*out, *x) && !views_overlap(*out, *w), MXR_ALIAS_ERROR,
+              "%s: output overlaps an input", fn);
+  const int mode = conj_w ? (same_view(x, w) ? 3 : 0) : 1;
+  return xc_run(fn, out, A, B, mode, n, s);
+}
+
+static mxr_status fft_conv_adjoint_impl(const mxr_tensor *gx, const mxr_tensor *gw,
+                                        const mxr_tensor *gy, const mxr_tensor *x,
+                                        const mxr_tensor *w, int64_t n, int conj_w,
+                                        hipStream_t s) {
+  const char *fn = "fft_conv_adjoint";
+  mxr_status st;
+  if ((st = check_tensor(gy, "gy", MXR_C64, 1, MXR_MAX_RANK))) return st;
+  MXR_REQUIRE(gx || gw, MXR_INVALID_PARAMETER, "%s: no gradient requested", fn);
+  RowOp X{}, W{}, GY{};
+  if ((st = row_op(x, gy, fn, "x", &X))) return st;
+  if ((st = row_op(gy, gy, fn, "gy", &GY))) return st;
+  MXR_REQUIRE(w->rank == gy->rank, MXR_NOT_SUPPORTED,
+              "%s: the waveform needs one row per signal row (rank %d)", fn, gy->rank);
+  if ((st = row_op(w, gy, fn, "w", &W))) return st;
+  MXR_REQUIRE(X.len <= n && W.len <= n && GY.len <= n, MXR_INVALID_SIZE,
+              "%s: rows longer than the transform size %lld", fn, (long long)n);
+  if (gx) {
+    if ((st = check_tensor(gx, "gx", MXR_C64, 1, MXR_MAX_RANK))) return st;
+    for (int d = 0; d < gx->rank; ++d)
+      MXR_REQUIRE(gx->rank == x->rank && gx->shape[d] == x->shape[d], MXR_INVALID_SIZE,
+                  "%s: gx must have x's shape", fn);
+    MXR_REQUIRE(!views_overlap(*gx, *gy) && !views_overlap(*gx, *x) && !views_overlap(*gx, *w),
+                MXR_ALIAS_ERROR, "%s: gx overlaps an input", fn);
+  }
+  if (gw) {
+    if ((st = check_tensor(gw, "gw", MXR_C64, 1, MXR_MAX_RANK))) return st;
+    for (int d = 0; d < gw->rank; ++d)
+      MXR_REQUIRE(gw->rank == w->rank && gw->shape[d] == w->shape[d], MXR_INVALID_SIZE,
+                  "%s: gw must have w's shape", fn);
+    MXR_REQUIRE(!views_overlap(*gw, *gy) && !views_overlap(*gw, *x) && !views_overlap(*gw, *w),
+                MXR_ALIAS_ERROR, "%s: gw overlaps an input", fn);
+    if (gx) MXR_REQUIRE(!views_overlap(*gx, *gw), MXR_ALIAS_ERROR, "%s: gx and gw overlap", fn);
+  }
+  // correlation: gx = IFFT(GY.*W), gw = IFFT(X.*conj(GY))
+  // convolution: gx = IFFT(GY.*conj(W)), gw = IFFT(GY.*conj(X))
+  if (gx && (st = xc_run(fn, gx, GY, W, conj_w ? 1 : 0, n, s))) return st;
+  if (gw && (st = xc_run(fn, gw, conj_w ? X : GY, conj_w ? GY : X, 0, n, s))) return st;
+  return MXR_SUCCESS;
+}
+
+}  // namespace mxr
+
+extern "C" mxr_status mxr_fft_conv_adjoint(const mxr_tensor *gx, const mxr_tensor *gw,
+                                           const mxr_tensor *gy, const mxr_tensor *x,
+                                           const mxr_tensor *w, int64_t n, int conj_w,
+                                           hipStream_t stream) {
+  return mxr::guarded("mxr_fft_conv_adjoint", [&] {
+    return mxr::fft_conv_adjoint_impl(gx, gw, gy, x, w, n, conj_w, stream);
+  });
+}

@@ -12,6 +12,10 @@ struct CacheClearRegistrar {
   explicit CacheClearRegistrar(void (*fn)()) { register_cache_clear(fn); }
 };
 
+// Device scratch of at least `bytes` for (current device, stream, tag);
+// owned by the library, released by mxr_clear_caches.  nullptr on failure.
+void *stream_workspace(const char *tag, size_t bytes, hipStream_t s);
+
 // Brackets one kernel launch with profiling events when profiling is on.
 class ProfScope {
  public:

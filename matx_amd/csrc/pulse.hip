@@ -523,6 +523,9 @@ static mxr_status pc_blocks(const char *fn, const PcPlan &plan, bool in_place, c
   return MXR_SUCCESS;
 }
 
+mxr_status fft_conv_rows(const mxr_tensor *out, const mxr_tensor *x, const mxr_tensor *w,
+                         int64_t n, int conj_w, hipStream_t s);
+
 // Shared body of mxr_pulse_compress and mxr_fft_conv.
 mxr_status pc_common(const char *fn, const mxr_tensor *y, const mxr_tensor *x, const mxr_tensor *w,
                      int64_t n, int window, int normalize, int writeback, int conj_w,
@@ -676,6 +679,7 @@ mxr_status mxr_pulse_compress(const mxr_tensor *y, const mxr_tensor *x, const mx
 mxr_status mxr_fft_conv(const mxr_tensor *out, const mxr_tensor *x, const mxr_tensor *w, int64_t n,
                         int conj_w, hipStream_t stream) {
   return mxr::guarded("mxr_fft_conv", [&] {
+    if (w && w->rank > 1) return mxr::fft_conv_rows(out, x, w, n, conj_w ? 1 : 0, stream);
     return mxr::pc_common("fft_conv", out, x, w, n, 0, 0, 0, conj_w ? 1 : 0, stream);
   });
 }

@@ -4,6 +4,8 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <string>
+#include <tuple>
 #include <mutex>
 #include <vector>
 
@@ -217,6 +219,48 @@ const float4 *pow_base_device(int N, int Ls) {
   g_pb[key] = d;
   return d;
 }
+
+// Per (device, stream, tag) scratch buffers that only grow: partial sums of
+// reductions that a launch on that stream writes and a later launch on the
+// same stream consumes.  Growing frees the old buffer (hipFree waits for the
+// device), so work already queued never sees a freed buffer.
+namespace {
+std::mutex g_ws_mu;
+struct WsBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+std::map<std::tuple<int, uintptr_t, std::string>, WsBuf> g_ws;
+}  // namespace
+
+void *stream_workspace(const char *tag, size_t bytes, hipStream_t s) {
+  const auto key = std::make_tuple(device_id(), (uintptr_t)s, std::string(tag));
+  std::lock_guard<std::mutex> g(g_ws_mu);
+  WsBuf &b = g_ws[key];
+  if (b.bytes >= bytes && b.p) return b.p;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    b.p = nullptr;
+    return nullptr;
+  }
+  b.bytes = bytes;
+  return b.p;
+}
+
+static void clear_workspaces() {
+  std::lock_guard<std::mutex> g(g_ws_mu);
+  for (auto &kv : g_ws) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(std::get<0>(kv.first));
+    (void)hipFree(kv.second.p);
+    (void)hipSetDevice(cur);
+  }
+  g_ws.clear();
+}
+CacheClearRegistrar g_reg_ws(clear_workspaces);
 
 static void clear_twiddles() {
   std::lock_guard<std::mutex> g(g_tw_mu);

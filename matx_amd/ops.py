@@ -171,6 +171,42 @@ def doppler_cfar(y, nd: int, tpc: str = "same", win_rows: int | None = None, win
     return dets, ba, pw
 
 
+def fft_conv_adjoint(gy, x, w, n: int, conj: bool = True, want_gx: bool = True,
+                     want_gw: bool = True, stream=None):
+    """Gradients of a real loss through y = fft_conv(x, w, n) with one
+    waveform per row: returns (gx, gw) for gy = dL/dRe(y) + i dL/dIm(y)."""
+    gx = torch.empty_like(x) if want_gx else None
+    gw = torch.empty_like(w) if want_gw else None
+    L.check(L.lib().mxr_fft_conv_adjoint(L.ref(L.desc(gx)), L.ref(L.desc(gw)), L.ref(L.desc(gy)),
+                                         L.ref(L.desc(x)), L.ref(L.desc(w)), int(n),
+                                         1 if conj else 0, L.stream_handle(stream)))
+    return gx, gw
+
+
+def doppler_taps(x, taps, nd: int, want_rd: bool = False, want_pow: bool = True, stream=None):
+    """Doppler stage with an arbitrary real window: pow = |FFT_nd(taps .* x)|^2
+    along pulses (x: [C, P, R]).  Returns (pow, rd)."""
+    C, P, R = x.shape
+    pw = torch.empty((C, nd, R), dtype=torch.float32, device=x.device) if want_pow else None
+    rd = torch.empty((C, nd, R), dtype=torch.complex64, device=x.device) if want_rd else None
+    L.check(L.lib().mxr_doppler_taps(L.ref(L.desc(pw)), L.ref(L.desc(rd)), L.ref(L.desc(x)),
+                                     L.ref(L.desc(taps)), int(nd), L.stream_handle(stream)))
+    return pw, rd
+
+
+def doppler_taps_adjoint(gpow, x, taps, nd: int, want_gx: bool = False, want_gtaps: bool = True,
+                         stream=None):
+    """Gradients of a real loss through pow = doppler_taps(x, taps, nd):
+    returns (gx, gtaps) for gpow = dL/dpow."""
+    gx = torch.empty_like(x) if want_gx else None
+    gt = torch.empty_like(taps) if want_gtaps else None
+    L.check(L.lib().mxr_doppler_taps_adjoint(L.ref(L.desc(gt)), L.ref(L.desc(gx)),
+                                             L.ref(L.desc(gpow)), L.ref(L.desc(x)),
+                                             L.ref(L.desc(taps)), int(nd),
+                                             L.stream_handle(stream)))
+    return gx, gt
+
+
 def conv1d(x, f, mode="full", out=None, stream=None):
     """Direct conv1d along the last axis (include/matx/operators/conv.h:244-266)."""
     m = _mode(m=mode)

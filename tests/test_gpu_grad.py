@@ -1,0 +1,129 @@
+"""GPU parity of the differentiable path: per-row fft_conv and its adjoint,
+the tapped Doppler stage and its adjoint (FP64 numpy formulas, rel L2 <=
+1e-5), and the waveform/window design-loss gradients from torch.autograd
+over the HIP kernels against the FP64 oracle gradient (rel <= 1e-4; the
+oracle gradient itself is pinned to FP64 central differences in
+tests/test_grad_oracle.py)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+RTOL = 1e-5
+GTOL = 1e-4
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def dev(a, dtype=torch.complex64):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def crandn(rng, *shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+@pytest.fixture(scope="module")
+def mx():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import matx_amd
+    matx_amd.lib()
+    return matx_amd
+
+
+def np_corr(x, w, n, conj=True):
+    X = np.fft.fft(x.astype(np.complex128), n, axis=-1)
+    W = np.fft.fft(w.astype(np.complex128), n, axis=-1)
+    return np.fft.ifft(X * (np.conj(W) if conj else W), axis=-1)
+
+
+@pytest.mark.parametrize("B,S,L,n,nout,conj", [
+    (5, 1000, 1000, 2048, 2048, True), (3, 700, 300, 1024, 1000, True),
+    (4, 2048, 256, 4096, 4096, False), (2, 1024, 1024, 8192, 8192, True)])
+def test_fft_conv_rows(mx, B, S, L, n, nout, conj):
+    rng = np.random.default_rng(n + S)
+    x = crandn(rng, B, S)
+    w = crandn(rng, B, L)
+    out = torch.empty((B, nout), dtype=torch.complex64, device="cuda")
+    mx.fft_conv(dev(x), dev(w), n, conj=conj, out=out)
+    assert rel(host(out), np_corr(x, w, n, conj)[:, :nout]) < RTOL
+
+
+def test_fft_conv_rows_autocorrelation(mx):
+    rng = np.random.default_rng(1)
+    s = crandn(rng, 6, 1024)
+    sd = dev(s)
+    got = host(mx.fft_conv(sd, sd, 8192))
+    assert rel(got, np_corr(s, s, 8192)) < RTOL
+
+
+@pytest.mark.parametrize("conj", [True, False])
+def test_fft_conv_adjoint(mx, conj):
+    rng = np.random.default_rng(2 + conj)
+    B, S, L, n, nout = 4, 900, 333, 2048, 2048
+    x, w, gy = crandn(rng, B, S), crandn(rng, B, L), crandn(rng, B, nout)
+    gx, gw = mx.ops.fft_conv_adjoint(dev(gy), dev(x), dev(w), n, conj)
+    X = np.fft.fft(x.astype(complex), n, axis=-1)
+    W = np.fft.fft(w.astype(complex), n, axis=-1)
+    GY = np.fft.fft(gy.astype(complex), n, axis=-1)
+    if conj:
+        rx, rw = np.fft.ifft(GY * W)[:, :S], np.fft.ifft(X * np.conj(GY))[:, :L]
+    else:
+        rx, rw = np.fft.ifft(GY * np.conj(W))[:, :S], np.fft.ifft(GY * np.conj(X))[:, :L]
+    assert rel(host(gx), rx) < RTOL
+    assert rel(host(gw), rw) < RTOL
+    # adjoint identity <A x, gy> = <x, A^H gy> on the forward operator
+    y = np_corr(x, w, n, conj)[:, :nout]
+    lhs = np.vdot(gy.astype(complex), y)
+    rhs = np.vdot(host(gx).astype(complex), x)
+    assert abs(lhs - rhs) / abs(lhs) < 1e-4
+
+
+@pytest.mark.parametrize("C,P,R,nd", [(2, 128, 300, 128), (3, 20, 77, 32), (1, 200, 50, 256)])
+def test_doppler_taps_and_adjoint(mx, C, P, R, nd):
+    rng = np.random.default_rng(C * P + R)
+    x = crandn(rng, C, P, R)
+    h = (np.hamming(P) + 0.1 * rng.standard_normal(P)).astype(np.float32)
+    gp = rng.standard_normal((C, nd, R)).astype(np.float32)
+    pw, rd = mx.ops.doppler_taps(dev(x), dev(h, torch.float32), nd, want_rd=True)
+    D = np.fft.fft(h[None, :, None].astype(float) * x.astype(complex), nd, axis=1)
+    assert rel(host(rd), D) < RTOL
+    assert rel(host(pw), np.abs(D) ** 2) < RTOL
+    gx, gt = mx.ops.doppler_taps_adjoint(dev(gp, torch.float32), dev(x), dev(h, torch.float32),
+                                         nd, want_gx=True)
+    gu = nd * np.fft.ifft(2 * gp * D, axis=1)[:, :P, :]
+    assert rel(host(gx), h[None, :, None] * gu) < RTOL
+    assert rel(host(gt), np.sum(np.real(np.conj(gu) * x), axis=(0, 2))) < RTOL
+
+
+@pytest.mark.parametrize("size", ["small", "design"])
+def test_design_loss_gradient(mx, size):
+    from matx_amd import autograd as ag
+    from oracle import grad as og
+    if size == "small":
+        phi, h, X, targets = ag.make_design_problem(5, B=4, Lw=256, C=2, P=32, R=100, T=4)
+        n, nd = 1024, 32
+    else:
+        phi, h, X, targets = ag.make_design_problem(6)
+        n, nd = 8192, 128
+    L64, gphi64, gh64 = og.loss_and_grad(phi, h, X, targets, n, nd)
+    p = dev(phi, torch.float32).requires_grad_()
+    t = dev(h, torch.float32).requires_grad_()
+    tg = torch.as_tensor(np.array(targets), dtype=torch.int64, device="cuda")
+    L = ag.design_loss(p, t, dev(X), tg, n, nd)
+    L.backward()
+    assert abs(float(L) - L64) / abs(L64) < 1e-5
+    assert rel(host(p.grad), gphi64) < GTOL
+    assert rel(host(t.grad), gh64) < GTOL
