@@ -196,6 +196,47 @@ mxr_status mxr_doppler_taps_adjoint(const mxr_tensor *gtaps, const mxr_tensor *g
                                     const mxr_tensor *gpow, const mxr_tensor *x,
                                     const mxr_tensor *taps, int64_t nd, hipStream_t stream);
 
+/* --------------------------------------------------------- ragged batch */
+/* Many independent pulse-compression + Doppler jobs of uneven shapes in a
+ * few persistent launches (one per block-size class and stage), for capture
+ * into one hipGraph.  No MatX counterpart: MatX runs such jobs as separate
+ * transform calls (fft_impl, include/matx/transforms/fft/fft_cuda.h:680-774).
+ * Per job:
+ *   y  = pulse compression of x ([C, P, S] c64, unit sample stride) with
+ *        the window-normalised waveform w ([L] c64, unit stride) at
+ *        n = next_pow2(S), lags r < R = S - L + 1 (as mxr_pulse_compress
+ *        with MXR_PC_WINDOW | MXR_PC_NORMALIZE), kept in plan scratch;
+ *   pw = |FFT_nd(hamming_P .* y)|^2 along pulses, nd = next_pow2(P)
+ *        ([C, nd, R] f32, unit range stride), as mxr_doppler(pw, NULL, y,
+ *        MXR_TPC_NONE, P, nd).
+ * Strides are in elements.  S <= 16384, P <= 1024. */
+typedef struct mxr_ragged_job {
+  const void *x;
+  int64_t C, P, S;
+  int64_t x_cs, x_ps; /* channel and pulse strides of x */
+  const void *w;
+  int64_t L;
+  void *pw;
+  int64_t pw_cs, pw_ks; /* channel and Doppler-bin strides of pw */
+} mxr_ragged_job;
+typedef struct mxr_ragged_plan_s *mxr_ragged_plan;
+/* Pick each size class's overlap-save block size by timing the candidates
+ * on the device (cached per process); otherwise a flop-count cost model. */
+#define MXR_RAGGED_AUTOTUNE 1
+/* Run each class as a fixed grid of resident workgroups looping over the
+ * class's items (a persistent launch) instead of one workgroup per item. */
+#define MXR_RAGGED_PERSISTENT 2
+/* Copies the job table, allocates the scratch, runs the autotuner on
+ * `stream` (synchronously). */
+mxr_status mxr_ragged_plan_create(mxr_ragged_plan *plan, const mxr_ragged_job *jobs, int njobs,
+                                  int flags, hipStream_t stream);
+/* Enqueues every job: waveform spectra, pulse compression, Doppler.  No
+ * allocation and no host synchronisation: capturable in a hipGraph. */
+mxr_status mxr_ragged_execute(mxr_ragged_plan plan, hipStream_t stream);
+/* The block size chosen for job `job`. */
+mxr_status mxr_ragged_plan_block_size(mxr_ragged_plan plan, int job, int *M);
+mxr_status mxr_ragged_plan_destroy(mxr_ragged_plan plan);
+
 /* ------------------------------------------------------------------ CFAR */
 /* Cell-averaging CFAR with a rectangular training window of win_y x win_x
  * cells minus a centred hole of guard_y x guard_x cells (CUT + guards), zero

@@ -4,7 +4,7 @@ three-pulse canceller + Doppler, CA-CFAR) executed by hand-written gfx950 HIP
 kernels in libmxr.so (C ABI: include/mxr.h; C++ front end:
 include/matx_amd.h)."""
 from ._lib import MatxError, lib  # noqa: F401
-from . import expr, ops  # noqa: F401
+from . import expr, ops, ragged  # noqa: F401
 from .ops import (cfar, conv1d, conv2d, doppler, doppler_cfar, fft, fft_conv, ifft,  # noqa: F401
                   pulse_compress)
 from .radar import FusedRadarPipeline, RadarPipeline  # noqa: F401

@@ -26,7 +26,9 @@ EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_ena
            "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar", "mxr_doppler_cfar",
            "mxr_conv1d", "mxr_conv2d", "mxr_ew_eval", "mxr_sum", "mxr_pc_plan_create",
            "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy",
-           "mxr_fft_conv_adjoint", "mxr_doppler_taps", "mxr_doppler_taps_adjoint")
+           "mxr_fft_conv_adjoint", "mxr_doppler_taps", "mxr_doppler_taps_adjoint",
+           "mxr_ragged_plan_create", "mxr_ragged_execute", "mxr_ragged_plan_block_size",
+           "mxr_ragged_plan_destroy")
 
 STATUS_NAMES = {0: "SUCCESS", 1: "NOT_SUPPORTED", 2: "INVALID_PARAMETER",
                 3: "INVALID_DIM", 4: "INVALID_SIZE", 5: "OUT_OF_MEMORY",
@@ -55,6 +57,17 @@ class mxr_cfar_params(ctypes.Structure):
 
 
 _lib = None
+
+
+class mxr_ragged_job(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_void_p), ("C", ctypes.c_int64), ("P", ctypes.c_int64),
+                ("S", ctypes.c_int64), ("x_cs", ctypes.c_int64), ("x_ps", ctypes.c_int64),
+                ("w", ctypes.c_void_p), ("L", ctypes.c_int64), ("pw", ctypes.c_void_p),
+                ("pw_cs", ctypes.c_int64), ("pw_ks", ctypes.c_int64)]
+
+
+RAGGED_AUTOTUNE = 1
+RAGGED_PERSISTENT = 2
 
 
 def lib():
@@ -100,6 +113,10 @@ def lib():
         "mxr_fft_conv_adjoint": [T, T, T, T, T, I64, I, V],
         "mxr_doppler_taps": [T, T, T, T, I64, V],
         "mxr_doppler_taps_adjoint": [T, T, T, T, T, I64, V],
+        "mxr_ragged_plan_create": [ctypes.POINTER(V), ctypes.POINTER(mxr_ragged_job), I, I, V],
+        "mxr_ragged_execute": [V, V],
+        "mxr_ragged_plan_block_size": [V, I, ctypes.POINTER(I)],
+        "mxr_ragged_plan_destroy": [V],
     }
     for name, argt in sigs.items():
         fn = getattr(L, name)

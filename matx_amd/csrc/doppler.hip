@@ -12,25 +12,9 @@
 // HBM traffic per tile: P*TR*8 bytes in, nd*TR*(4 and/or 8) bytes out.
 #include "fft_engine.hpp"
 #include "mxr_cache.hpp"
+#include "ragged.hpp"
 
 namespace mxr {
-
-struct DopArgs {
-  const float2 *y;
-  int64_t ys_c, ys_p, ys_r;
-  float2 *rd;
-  int64_t rs_c, rs_k, rs_r;
-  float *pw;
-  int64_t ps_c, ps_k, ps_r;
-  int C, P, R;
-  int rows_t;    // canceller output rows fed to the FFT (P or P-2)
-  int p_off;     // canceller row i is centred on pulse i + p_off
-  int win_rows;  // rows < win_rows get hamming(win_rows)
-  int no_tpc;    // input is already the canceller output
-  const float *taps;  // optional real window replacing hamming(win_rows)
-  int64_t taps_s;
-  const float2 *tw;
-};
 
 template <int ND>
 struct DopCfg {
@@ -40,18 +24,17 @@ struct DopCfg {
   static constexpr int TR = ND <= 128 ? 32 : 4096 / ND;
 };
 
+// One tile: TR range bins from r0 of channel c.  lds holds rows_buf * TR
+// elements, win ND floats.  Pre: the workgroup's previous use of both is
+// finished.
 template <int ND>
-__global__ __launch_bounds__(256) void doppler_kernel(DopArgs a, int rows_buf) {
+__device__ __forceinline__ void doppler_tile(const DopArgs &a, int c, int r0, int rows_buf,
+                                             float2 *lds, float *win) {
   constexpr int NT = DopCfg<ND>::NT;
   constexpr int TR = DopCfg<ND>::TR;
   using LY = Layout<ND, TR, true>;
   using SQ = Seq<ND, false>;
   constexpr int K = SQ::n;
-  extern __shared__ float2 lds[];  // rows_buf * TR
-  __shared__ float win[ND];
-
-  const int c = blockIdx.y;
-  const int r0 = blockIdx.x * TR;
   for (int i = threadIdx.x; i < ND; i += NT) {
     float h = 0.f;
     if (i < a.rows_t)
@@ -119,6 +102,90 @@ __global__ __launch_bounds__(256) void doppler_kernel(DopArgs a, int rows_buf) {
     pl.template compute<LSL, false>(a.tw + SQ::template TWOFF<K - 1>);
     pl.template store<LSL>(store_out);
   }
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void doppler_kernel(DopArgs a, int rows_buf) {
+  extern __shared__ float2 lds[];  // rows_buf * TR
+  __shared__ float win[ND];
+  doppler_tile<ND>(a, blockIdx.y, blockIdx.x * DopCfg<ND>::TR, rows_buf, lds, win);
+}
+
+// Ragged batch: one persistent grid walks the (job, channel, tile) items of
+// every job of one Doppler size (jobs[j].item0 are prefix sums of C*tiles).
+template <int ND, bool PERSIST>
+__global__ __launch_bounds__(256) void doppler_batch_kernel(const DopJob *jobs, int njobs,
+                                                            int64_t items) {
+  constexpr int TR = DopCfg<ND>::TR;
+  __shared__ float2 lds[ND * TR];
+  __shared__ float win[ND];
+  auto item = [&](int64_t it) {
+    int lo = 0, hi = njobs - 1;  // last job with item0 <= it
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].item0 <= it) lo = mid;
+      else hi = mid - 1;
+    }
+    const DopJob jb = uniform_copy(&jobs[lo]);
+    const int64_t local = it - jb.item0;
+    doppler_tile<ND>(jb.a, (int)(local / jb.tiles), (int)(local % jb.tiles) * TR, ND, lds, win);
+  };
+  if constexpr (!PERSIST) {
+    item(blockIdx.x);
+    return;
+  }
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    if (it != blockIdx.x) __syncthreads();  // previous item's LDS use is done
+    item(it);
+  }
+}
+
+mxr_status launch_doppler_batch(int nd, const DopJob *jobs, int njobs, int64_t items, int grid,
+                                hipStream_t s) {
+  if (items == 0) return MXR_SUCCESS;
+  MXR_REQUIRE(grid >= 1 && grid <= items && items <= 0x7fffffff, MXR_INVALID_SIZE,
+              "ragged batch: bad grid %d for %lld items", grid, (long long)items);
+  const bool persist = grid < items;
+  ProfScope ps("doppler_batch_kernel", s);
+  switch (nd) {
+#define MXR_DB(N)                                                                          \
+  case N:                                                                                  \
+    if (persist) doppler_batch_kernel<N, true><<<dim3(grid), dim3(256), 0, s>>>(jobs, njobs, items); \
+    else doppler_batch_kernel<N, false><<<dim3(grid), dim3(256), 0, s>>>(jobs, njobs, items); \
+    break;
+    MXR_DB(2)
+    MXR_DB(4)
+    MXR_DB(8)
+    MXR_DB(16)
+    MXR_DB(32)
+    MXR_DB(64)
+    MXR_DB(128)
+    MXR_DB(256)
+    MXR_DB(512)
+    MXR_DB(1024)
+#undef MXR_DB
+    default:
+      set_error(MXR_NOT_SUPPORTED, "ragged batch: Doppler size %d unsupported", nd);
+      return MXR_NOT_SUPPORTED;
+  }
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
+int doppler_batch_tile(int nd) {
+  switch (nd) {
+    case 2: return DopCfg<2>::TR;
+    case 4: return DopCfg<4>::TR;
+    case 8: return DopCfg<8>::TR;
+    case 16: return DopCfg<16>::TR;
+    case 32: return DopCfg<32>::TR;
+    case 64: return DopCfg<64>::TR;
+    case 128: return DopCfg<128>::TR;
+    case 256: return DopCfg<256>::TR;
+    case 512: return DopCfg<512>::TR;
+    case 1024: return DopCfg<1024>::TR;
+  }
+  return 0;
 }
 
 template <int ND>

@@ -22,6 +22,7 @@
 
 #include "fft_engine.hpp"
 #include "mxr_cache.hpp"
+#include "ragged.hpp"
 
 namespace mxr {
 
@@ -47,12 +48,22 @@ struct PcCfg {
   static constexpr int NT = M / 16;
 };
 
-// ROWLOOP: one workgroup walks blocks [b0, b1) of its row in order (needed
-// when y aliases x: a block only overwrites lags below the next block's
-// first input).  Otherwise each workgroup does exactly one block.
-template <int M, bool ROWLOOP>
-__global__ __launch_bounds__(PcCfg<M>::NT) __attribute__((amdgpu_waves_per_eu(4)))
-void pc_ols_kernel(PcArgs a) {
+// One overlap-save block of one row: x_r[(lo + i) mod n], i < M (zero past
+// seff) correlated with the M-point spectrum wc; lags lo .. lo+min(V, nout-lo)
+// stored to y_r.
+struct OlsBlock {
+  const float2 *xr;
+  float2 *yr;
+  int lo, n, seff, nout, V;
+  const float2 *wc;
+  float scale;
+};
+
+// Body shared by the per-call kernel and the ragged-batch kernels.  Pre: the
+// workgroup's previous use of lds is finished (barrier passed).
+template <int M>
+__device__ __forceinline__ void ols_block(const OlsBlock &ob, const float2 *twf,
+                                          const float2 *twi, float2 *lds) {
   constexpr int NT = PcCfg<M>::NT;
   using LY = Layout<M, 1, false>;
   using SF = Seq<M, false>;
@@ -62,8 +73,66 @@ void pc_ols_kernel(PcArgs a) {
   constexpr int RL = SF::template R<K - 1>;
   constexpr int LSL = SF::template LS<K - 1>;
   static_assert(K >= 2, "pc kernel needs at least two passes");
-  __shared__ float2 lds[LY::LDS_ELEMS];
+  const float2 *__restrict__ xr = ob.xr;
+  float2 *__restrict__ yr = ob.yr;
+  const int lo = ob.lo, n32 = ob.n, seff32 = ob.seff;
 
+  // forward pass 0 straight from HBM (zero padding and wrap implicit)
+  {
+    Pass<LY, R0, M, NT, 1> ps;
+    ps.load([&](int, int i) {
+      int gi = lo + i;
+      if (gi >= n32) gi -= n32;
+      return gi < seff32 ? xr[gi] : make_float2(0.f, 0.f);
+    });
+    ps.template compute<1, false>(nullptr);
+    ps.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
+  }
+  wg_barrier();
+  lds_passes<LY, SF, M, NT, 1, false, 1, K - 1>(lds, twf);
+
+  // last forward pass, conj-multiply, first inverse pass: one register pass
+  {
+    Pass<LY, RL, M, NT, 1> ps;
+    ps.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
+    wg_barrier();
+    ps.template compute<LSL, false>(twf + SF::template TWOFF<K - 1>);
+#pragma unroll
+    for (int u = 0; u < ps.BPT; ++u) {
+      if (!ps.active(u)) continue;
+      const int j = threadIdx.x + u * NT;  // k == j in the last pass
+#pragma unroll
+      for (int q = 0; q < RL; ++q) ps.v[u][q] = cmul(ps.v[u][q], ob.wc[j + q * LSL]);
+    }
+    ps.template compute<1, true>(nullptr);
+    ps.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
+  }
+  wg_barrier();
+  lds_passes<LY, SI, M, NT, 1, true, 1, K - 1>(lds, twi);
+
+  // last inverse pass straight to HBM with the 1/M scale
+  {
+    constexpr int RI = SI::template R<K - 1>;
+    constexpr int LSI = SI::template LS<K - 1>;
+    Pass<LY, RI, M, NT, 1> ps;
+    ps.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
+    ps.template compute<LSI, true>(twi + SI::template TWOFF<K - 1>);
+    const float s = ob.scale;
+    const int lim = (ob.nout - lo) < ob.V ? (ob.nout - lo) : ob.V;
+    ps.template store<LSI>([&](int, int i, float2 v) {
+      if (i < lim) yr[lo + i] = cscale(v, s);
+    });
+  }
+}
+
+// ROWLOOP: one workgroup walks blocks [b0, b1) of its row in order (needed
+// when y aliases x: a block only overwrites lags below the next block's
+// first input).  Otherwise each workgroup does exactly one block.
+template <int M, bool ROWLOOP>
+__global__ __launch_bounds__(PcCfg<M>::NT) __attribute__((amdgpu_waves_per_eu(4)))
+void pc_ols_kernel(PcArgs a) {
+  using LY = Layout<M, 1, false>;
+  __shared__ float2 lds[LY::LDS_ELEMS];
   const int64_t g = xcd_remap(blockIdx.x, a.groups);
   const int64_t gpr = (a.nblk + a.bpw - 1) / a.bpw;
   const int64_t row = g / gpr;
@@ -71,60 +140,11 @@ void pc_ols_kernel(PcArgs a) {
   const int64_t b1 = ROWLOOP ? (b0 + a.bpw < a.nblk ? b0 + a.bpw : a.nblk) : b0 + 1;
   int64_t ox, oy;
   batch_offsets(a.bi, row, &ox, &oy);
-  const float2 *__restrict__ xr = a.x + ox;
-  float2 *__restrict__ yr = a.y + oy;
-
-  const int n32 = (int)a.n, seff32 = (int)a.seff;
+  OlsBlock ob{a.x + ox, a.y + oy, 0, (int)a.n, (int)a.seff, (int)a.nout, (int)a.V, a.wc, a.scale};
   for (int64_t b = b0; b < b1; ++b) {
-    const int lo = (int)(b * a.V);
     if (b != b0) wg_barrier();  // previous block's LDS reads are done
-
-    // forward pass 0 straight from HBM (zero padding and wrap implicit)
-    {
-      Pass<LY, R0, M, NT, 1> ps;
-      ps.load([&](int, int i) {
-        int gi = lo + i;
-        if (gi >= n32) gi -= n32;
-        return gi < seff32 ? xr[gi] : make_float2(0.f, 0.f);
-      });
-      ps.template compute<1, false>(nullptr);
-      ps.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
-    }
-    wg_barrier();
-    lds_passes<LY, SF, M, NT, 1, false, 1, K - 1>(lds, a.twf);
-
-    // last forward pass, conj-multiply, first inverse pass: one register pass
-    {
-      Pass<LY, RL, M, NT, 1> ps;
-      ps.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
-      wg_barrier();
-      ps.template compute<LSL, false>(a.twf + SF::template TWOFF<K - 1>);
-#pragma unroll
-      for (int u = 0; u < ps.BPT; ++u) {
-        if (!ps.active(u)) continue;
-        const int j = threadIdx.x + u * NT;  // k == j in the last pass
-#pragma unroll
-        for (int q = 0; q < RL; ++q) ps.v[u][q] = cmul(ps.v[u][q], a.wc[j + q * LSL]);
-      }
-      ps.template compute<1, true>(nullptr);
-      ps.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
-    }
-    wg_barrier();
-    lds_passes<LY, SI, M, NT, 1, true, 1, K - 1>(lds, a.twi);
-
-    // last inverse pass straight to HBM with the 1/M scale
-    {
-      constexpr int RI = SI::template R<K - 1>;
-      constexpr int LSI = SI::template LS<K - 1>;
-      Pass<LY, RI, M, NT, 1> ps;
-      ps.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
-      ps.template compute<LSI, true>(a.twi + SI::template TWOFF<K - 1>);
-      const float s = a.scale;
-      const int lim = (int)((a.nout - lo) < a.V ? (a.nout - lo) : a.V);
-      ps.template store<LSI>([&](int, int i, float2 v) {
-        if (i < lim) yr[lo + i] = cscale(v, s);
-      });
-    }
+    ob.lo = (int)(b * a.V);
+    ols_block<M>(ob, a.twf, a.twi, lds);
   }
 }
 
@@ -171,21 +191,18 @@ __device__ __forceinline__ void tw_from_base(float2 (&v)[16], float4 b0, float4 
   for (int m = 1; m < 16; ++m) v[m] = cmul(v[m], w[m]);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-void pc4096_kernel(Pc4096Args pa) {
-  constexpr int M = 4096, NT = 256;
-  using LY = Layout<M, 1, false>;
-  __shared__ float2 lds[LY::LDS_ELEMS];
-  const PcArgs &a = pa.a;
+// One M = 4096 overlap-save block (256 threads); lds holds 4096 padded
+// elements.  Pre: the workgroup's previous use of lds is finished.
+// (The base rows are deliberately not __restrict__: re-fetching them for the
+// inverse passes is cheaper than keeping them live across the passes.)
+__device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *base1,
+                                              const float4 *base2, float2 *lds) {
+  constexpr int NT = 256;
+  using LY = Layout<4096, 1, false>;
   const int j = threadIdx.x;
-  const int64_t g = xcd_remap(blockIdx.x, a.groups);
-  const int64_t row = g / a.nblk;
-  const int lo = (int)((g % a.nblk) * a.V);
-  int64_t ox, oy;
-  batch_offsets(a.bi, row, &ox, &oy);
-  const float2 *__restrict__ xr = a.x + ox;
-  float2 *__restrict__ yr = a.y + oy;
-  const int n32 = (int)a.n, seff32 = (int)a.seff;
+  const float2 *__restrict__ xr = ob.xr;
+  float2 *__restrict__ yr = ob.yr;
+  const int lo = ob.lo, n32 = ob.n, seff32 = ob.seff;
 
   float2 v[16];
   // Stockham exchange: write this pass's outputs, read the next pass's inputs
@@ -198,8 +215,8 @@ void pc4096_kernel(Pc4096Args pa) {
     for (int m = 0; m < 16; ++m) v[m] = lds[LY::addr(0, j + m * NT)];
     wg_barrier();
   };
-  const float4 *b1p = pa.base1 + (j & 15) * 2;
-  const float4 *b2p = pa.base2 + j * 2;
+  const float4 *b1p = base1 + (j & 15) * 2;
+  const float4 *b2p = base2 + j * 2;
 
   // forward pass 0 (Ls = 1) straight from HBM (zero padding and wrap implicit)
 #pragma unroll
@@ -219,7 +236,7 @@ void pc4096_kernel(Pc4096Args pa) {
   tw_from_base(v, t0, t1, false);  // forward pass 2 (Ls = 256)
   dft16<false>(v);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], a.wc[j + q * 256]);
+  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], ob.wc[j + q * 256]);
   dft16<true>(v);                   // inverse pass 0 (Ls = 1)
   t0 = b1p[0];
   t1 = b1p[1];
@@ -231,24 +248,28 @@ void pc4096_kernel(Pc4096Args pa) {
   xchg(16);
   tw_from_base(v, t0, t1, true);   // inverse pass 2 (Ls = 256)
   dft16<true>(v);
-  const int lim = (int)((a.nout - lo) < a.V ? (a.nout - lo) : a.V);
+  const int lim = (ob.nout - lo) < ob.V ? (ob.nout - lo) : ob.V;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int i = j + q * 256;
-    if (i < lim) yr[lo + i] = cscale(v[q], a.scale);
+    if (i < lim) yr[lo + i] = cscale(v[q], ob.scale);
   }
 }
 
-// Waveform spectrum: Wc = conj?(fft_M(pad_M(w2))), w2 = w1/||w1|| (optional),
-// w1 = w .* hamming(L) (optional); optionally writes w2 back to w.
-struct WfArgs {
-  float2 *w;
-  int64_t wstride;
-  int64_t L;
-  int window, normalize, writeback, conj;
-  float2 *wc;
-  const float2 *tw;
-};
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void pc4096_kernel(Pc4096Args pa) {
+  using LY = Layout<4096, 1, false>;
+  __shared__ float2 lds[LY::LDS_ELEMS];
+  const PcArgs &a = pa.a;
+  const int64_t g = xcd_remap(blockIdx.x, a.groups);
+  const int64_t row = g / a.nblk;
+  int64_t ox, oy;
+  batch_offsets(a.bi, row, &ox, &oy);
+  const OlsBlock ob{a.x + ox, a.y + oy, (int)((g % a.nblk) * a.V), (int)a.n, (int)a.seff,
+                    (int)a.nout, (int)a.V, a.wc, a.scale};
+  ols_block4096(ob, pa.base1, pa.base2, lds);
+}
+
 
 // One workgroup of M/16 threads.  Thread j loads samples j + m*M/16,
 // m < 16 (all loads in flight at once), which are exactly the inputs of
@@ -258,15 +279,13 @@ struct WfArgs {
 // one LDS slot per wave); the last pass stores the (conjugated) spectrum
 // straight to memory.
 template <int M>
-__global__ __launch_bounds__(PcCfg<M>::NT) void wfspec_kernel(WfArgs a) {
+__device__ __forceinline__ void wfspec_body(const WfArgs &a, float2 *lds, double *red) {
   constexpr int NT = PcCfg<M>::NT;
   constexpr int NWV = NT / 64;
   using LY = Layout<M, 1, false>;
   using SF = Seq<M, false>;
   constexpr int K = SF::n;
   static_assert(SF::template R<0> == 16 && M / 16 == NT, "first pass must be radix 16 over NT butterflies");
-  __shared__ float2 lds[LY::LDS_ELEMS];
-  __shared__ double red[NWV];
   const int tid = threadIdx.x;
 
   Pass<LY, 16, M, NT, 1> p0;
@@ -318,6 +337,156 @@ __global__ __launch_bounds__(PcCfg<M>::NT) void wfspec_kernel(WfArgs a) {
   });
 }
 
+template <int M>
+__global__ __launch_bounds__(PcCfg<M>::NT) void wfspec_kernel(WfArgs a) {
+  __shared__ float2 lds[Layout<M, 1, false>::LDS_ELEMS];
+  __shared__ double red[PcCfg<M>::NT / 64];
+  wfspec_body<M>(a, lds, red);
+}
+
+// Ragged batch: one workgroup per job's waveform.
+template <int M>
+__global__ __launch_bounds__(PcCfg<M>::NT) void wfspec_batch_kernel(const WfArgs *args) {
+  __shared__ float2 lds[Layout<M, 1, false>::LDS_ELEMS];
+  __shared__ double red[PcCfg<M>::NT / 64];
+  wfspec_body<M>(args[blockIdx.x], lds, red);
+}
+
+// Ragged batch: one persistent grid walks the (job, row, block) items of
+// every job of one block size (jobs[j].item0 are prefix sums of C*P*nblk).
+__device__ __forceinline__ int find_job(const PcJob *jobs, int njobs, int64_t it) {
+  int lo = 0, hi = njobs - 1;  // last job with item0 <= it
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].item0 <= it) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ OlsBlock job_block(const PcJob &jb, int64_t it) {
+  const int64_t local = it - jb.item0;
+  const int64_t row = local / jb.nblk;
+  const int b = (int)(local % jb.nblk);
+  const int c = (int)(row / jb.P), p = (int)(row % jb.P);
+  return OlsBlock{jb.x + c * jb.x_cs + p * jb.x_ps, jb.y + c * jb.y_cs + p * jb.y_ps, b * jb.V,
+                  jb.n, jb.seff, jb.nout, jb.V, jb.wc, jb.scale};
+}
+
+// PERSIST: a fixed grid of resident workgroups loops over the items;
+// otherwise one workgroup per item (the dispatcher does the balancing, and
+// no per-pass state stays live across items).
+template <int M, bool PERSIST>
+__global__ __launch_bounds__(PcCfg<M>::NT) __attribute__((amdgpu_waves_per_eu(4)))
+void pc_batch_kernel(const PcJob *jobs, int njobs, int64_t items, const float2 *twf,
+                     const float2 *twi) {
+  __shared__ float2 lds[Layout<M, 1, false>::LDS_ELEMS];
+  if constexpr (!PERSIST) {
+    const int64_t it = blockIdx.x;
+    ols_block<M>(job_block(uniform_copy(&jobs[find_job(jobs, njobs, it)]), it), twf, twi, lds);
+    return;
+  }
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const PcJob jb = uniform_copy(&jobs[find_job(jobs, njobs, it)]);
+    if (it != blockIdx.x) wg_barrier();  // previous item's LDS reads are done
+    ols_block<M>(job_block(jb, it), opaque(twf), opaque(twi), lds);
+  }
+}
+
+template <bool PERSIST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+void pc4096_batch_kernel(const PcJob *jobs, int njobs, int64_t items, const float4 *base1,
+                         const float4 *base2) {
+  __shared__ float2 lds[Layout<4096, 1, false>::LDS_ELEMS];
+  if constexpr (!PERSIST) {
+    const int64_t it = blockIdx.x;
+    ols_block4096(job_block(uniform_copy(&jobs[find_job(jobs, njobs, it)]), it), base1, base2, lds);
+    return;
+  }
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const PcJob jb = uniform_copy(&jobs[find_job(jobs, njobs, it)]);
+    if (it != blockIdx.x) wg_barrier();
+    ols_block4096(job_block(jb, it), opaque(base1), opaque(base2), lds);
+  }
+}
+
+mxr_status launch_wfspec_batch(int M, const WfArgs *d_args, int njobs, hipStream_t s) {
+  if (njobs == 0) return MXR_SUCCESS;
+  ProfScope ps("wfspec_batch_kernel", s);
+  switch (M) {
+#define MXR_WB(MM) \
+  case MM: wfspec_batch_kernel<MM><<<dim3(njobs), dim3(PcCfg<MM>::NT), 0, s>>>(d_args); break;
+    MXR_WB(1024)
+    MXR_WB(2048)
+    MXR_WB(4096)
+    MXR_WB(8192)
+    MXR_WB(16384)
+#undef MXR_WB
+    default:
+      set_error(MXR_NOT_SUPPORTED, "ragged batch: block size %d unsupported", M);
+      return MXR_NOT_SUPPORTED;
+  }
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
+// Persistent grid: as many workgroups as stay resident at once.
+int pc_batch_grid(int M) {
+  int per_cu = 1;
+  hipError_t e = hipSuccess;
+  switch (M) {
+    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc_batch_kernel<1024, true>, PcCfg<1024>::NT, 0); break;
+    case 2048: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc_batch_kernel<2048, true>, PcCfg<2048>::NT, 0); break;
+    case 4096: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc4096_batch_kernel<true>, 256, 0); break;
+    case 8192: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc_batch_kernel<8192, true>, PcCfg<8192>::NT, 0); break;
+    case 16384: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc_batch_kernel<16384, true>, PcCfg<16384>::NT, 0); break;
+    default: break;
+  }
+  if (e != hipSuccess || per_cu < 1) per_cu = 1;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_id()) != hipSuccess || cus < 1)
+    cus = 256;
+  return per_cu * cus;
+}
+
+// grid == items: one workgroup per item; otherwise the persistent form.
+mxr_status launch_pc_batch(int M, const PcJob *d_jobs, int njobs, int64_t items, int grid,
+                           hipStream_t s) {
+  if (items == 0) return MXR_SUCCESS;
+  MXR_REQUIRE(grid >= 1 && grid <= items && items <= 0x7fffffff, MXR_INVALID_SIZE,
+              "ragged batch: bad grid %d for %lld items", grid, (long long)items);
+  const bool persist = grid < items;
+  const float2 *twf = twiddles_device(M, false), *twi = twiddles_device(M, true);
+  MXR_REQUIRE(twf && twi, MXR_OUT_OF_MEMORY, "ragged batch: twiddle table allocation failed");
+  ProfScope ps("pc_batch_kernel", s);
+  switch (M) {
+    case 4096: {
+      const float4 *b1 = pow_base_device(4096, 16), *b2 = pow_base_device(4096, 256);
+      MXR_REQUIRE(b1 && b2, MXR_OUT_OF_MEMORY, "ragged batch: twiddle table allocation failed");
+      if (persist) pc4096_batch_kernel<true><<<dim3(grid), dim3(256), 0, s>>>(d_jobs, njobs, items, b1, b2);
+      else pc4096_batch_kernel<false><<<dim3(grid), dim3(256), 0, s>>>(d_jobs, njobs, items, b1, b2);
+      break;
+    }
+#define MXR_PB(MM)                                                                               \
+  case MM:                                                                                       \
+    if (persist)                                                                                 \
+      pc_batch_kernel<MM, true><<<dim3(grid), dim3(PcCfg<MM>::NT), 0, s>>>(d_jobs, njobs, items, twf, twi); \
+    else                                                                                         \
+      pc_batch_kernel<MM, false><<<dim3(grid), dim3(PcCfg<MM>::NT), 0, s>>>(d_jobs, njobs, items, twf, twi); \
+    break;
+    MXR_PB(1024)
+    MXR_PB(2048)
+    MXR_PB(8192)
+    MXR_PB(16384)
+#undef MXR_PB
+    default:
+      set_error(MXR_NOT_SUPPORTED, "ragged batch: block size %d unsupported", M);
+      return MXR_NOT_SUPPORTED;
+  }
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
 // ------------------------------------------------------------------ host
 namespace {
 
@@ -344,7 +513,7 @@ int forced_block() {
 // Chooses the overlap-save block size.  allow_ols is false for convolution
 // (conj_w = 0), where only the single n-point block is exact.
 mxr_status plan_pc(int64_t seff, int64_t L, int64_t n, int64_t nout, bool allow_ols, bool in_place,
-                   PcPlan *out) {
+                   PcPlan *out, int force_M = 0) {
   PcPlan best{0, 0, 0};
   double best_cost = 1e300;
   auto consider = [&](int M) {
@@ -372,7 +541,7 @@ mxr_status plan_pc(int64_t seff, int64_t L, int64_t n, int64_t nout, bool allow_
       best = p;
     }
   };
-  const int fb = forced_block();
+  const int fb = force_M > 0 ? force_M : forced_block();
   if (fb > 0) {
     consider(fb);
   } else {
@@ -462,6 +631,19 @@ mxr_status dispatch_pc(int M, const WfArgs *wa, const PcArgs *pa, hipStream_t s)
 }
 
 }  // namespace
+
+// Overlap-save block plan of an out-of-place correlation (the planner's
+// choice, or the given block size M_req when it fits).
+mxr_status pc_plan_blocks(int64_t seff, int64_t L, int64_t n, int64_t nout, int M_req, int *M,
+                          int64_t *V, int64_t *nblk) {
+  PcPlan p;
+  mxr_status st = plan_pc(seff, L, n, nout, true, false, &p, M_req);
+  if (st) return st;
+  *M = p.M;
+  *V = p.V;
+  *nblk = p.nblk;
+  return MXR_SUCCESS;
+}
 
 // Validates the signal/output pair of a pulse-compression call and fills
 // the row fields of PcArgs (x, y, rows, batch strides, seff, n, nout).

@@ -162,6 +162,28 @@ def test_doppler_cfar_checks(L):
     assert r == 8
 
 
+def test_ragged_plan_checks(L):
+    import ctypes
+    lib = L.lib()
+    jobs = (L.mxr_ragged_job * 1)()
+    j = jobs[0]
+    j.x, j.C, j.P, j.S, j.x_cs, j.x_ps = 0x10000, 2, 16, 1000, 16000, 1000
+    j.w, j.L = 0x20000, 1200                      # waveform longer than the pulse
+    j.pw, j.pw_cs, j.pw_ks = 0x30000, 16 * 1000, 1000
+    h = ctypes.c_void_p()
+    r, msg = status(L, lib.mxr_ragged_plan_create, ctypes.byref(h), jobs, 1, 0, None)
+    assert r == 4 and "L <= S" in msg
+    j.L = 100
+    j.pw = 0                                      # no output
+    r, _ = status(L, lib.mxr_ragged_plan_create, ctypes.byref(h), jobs, 1, 0, None)
+    assert r == 2
+    j.pw, j.x_ps = 0x30000, 500                   # pulses overlap
+    r, _ = status(L, lib.mxr_ragged_plan_create, ctypes.byref(h), jobs, 1, 0, None)
+    assert r == 2
+    r, _ = status(L, lib.mxr_ragged_plan_create, ctypes.byref(h), jobs, 1, 8, None)
+    assert r == 2                                 # unknown flag
+
+
 def test_header_matches_python_constants(L):
     text = open(os.path.join(ROOT, "include", "mxr.h")).read()
     assert "MXR_TPC_NONE 3" in text and L.TPC_NONE == 3
