@@ -109,13 +109,15 @@ def test_doppler_cfar_equals_unfused(mx, orc):
     p2, _ = mx.doppler(y, nd, tpc="valid")
     d2, b2, _ = mx.cfar(p2, guard=(3, 5), want_ba=True)
     P1, P2 = host(p1), host(p2)
-    assert np.max(np.abs(P1 - P2) / np.maximum(np.abs(P2), 1e-30)) < 1e-6
-    assert rel(host(b1), host(b2)) < 1e-6
+    # the same transform with the canceller and window applied in a different
+    # order: agreement at rounding level (cells cancelled to near zero differ
+    # relatively, not absolutely)
+    assert rel(P1, P2) < 2e-6
+    assert rel(host(b1), host(b2)) < 2e-6
     D1, D2 = host(d1), host(d2)
     assert D1.sum() > 0
-    # identical arithmetic up to float contraction: at most a rare flip at a
-    # cell sitting on its threshold
-    assert int((D1 != D2).sum()) <= 2
+    # a flip needs a cell within rounding of its threshold: rare
+    assert int((D1 != D2).sum()) <= max(2, D1.size // 100000)
 
 
 def test_doppler_cfar_strided_unaligned_outputs(mx, orc):
