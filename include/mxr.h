@@ -121,6 +121,13 @@ mxr_status mxr_fft_conv_adjoint(const mxr_tensor *gx, const mxr_tensor *gw, cons
                                 const mxr_tensor *x, const mxr_tensor *w, int64_t n, int conj_w,
                                 hipStream_t stream);
 
+/* Gradient w.r.t. s of r = fft_conv(s, s, n) (each row's matched filter to
+ * itself, correlation form), i.e. gx + gw of mxr_fft_conv_adjoint with
+ * x = w = s, in one pass: gs = IFFT_n(FFT_n(s) .* 2 Re(FFT_n(gy)))[0..S).
+ * gy: [..., nout] c64; s, gs: [..., S] c64; n in [1024, 8192]. */
+mxr_status mxr_autocorr_adjoint(const mxr_tensor *gs, const mxr_tensor *gy, const mxr_tensor *s,
+                                int64_t n, hipStream_t stream);
+
 /* Pulse compression stage, examples/simple_radar_pipeline.h:273-301:
  *   w1 = w .* hamming(L) (if MXR_PC_WINDOW); w2 = w1/||w1|| (if
  *   MXR_PC_NORMALIZE); y[..., r] = (1/n) ifft(fft(x, n) .* conj(fft(w2, n)))[r]

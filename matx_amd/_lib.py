@@ -26,7 +26,7 @@ EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_ena
            "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar", "mxr_doppler_cfar",
            "mxr_conv1d", "mxr_conv2d", "mxr_ew_eval", "mxr_sum", "mxr_pc_plan_create",
            "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy",
-           "mxr_fft_conv_adjoint", "mxr_doppler_taps", "mxr_doppler_taps_adjoint",
+           "mxr_fft_conv_adjoint", "mxr_autocorr_adjoint", "mxr_doppler_taps", "mxr_doppler_taps_adjoint",
            "mxr_ragged_plan_create", "mxr_ragged_execute", "mxr_ragged_plan_block_size",
            "mxr_ragged_plan_destroy")
 
@@ -111,6 +111,7 @@ def lib():
         "mxr_pc_plan_execute": [V, T, T, V],
         "mxr_pc_plan_destroy": [V],
         "mxr_fft_conv_adjoint": [T, T, T, T, T, I64, I, V],
+        "mxr_autocorr_adjoint": [T, T, T, I64, V],
         "mxr_doppler_taps": [T, T, T, T, I64, V],
         "mxr_doppler_taps_adjoint": [T, T, T, T, T, I64, V],
         "mxr_ragged_plan_create": [ctypes.POINTER(V), ctypes.POINTER(mxr_ragged_job), I, I, V],

@@ -39,6 +39,23 @@ class FFTConv(torch.autograd.Function):
         return gx, gw, None, None
 
 
+class AutoCorr(torch.autograd.Function):
+    """r = fft_conv(s, s, n): each row's matched filter to itself.  The two
+    adjoint terms of FFTConv add up to one pass (mxr_autocorr_adjoint)."""
+
+    @staticmethod
+    def forward(ctx, s, n: int):
+        s = s.contiguous()
+        ctx.save_for_backward(s)
+        ctx.n = n
+        return ops.fft_conv(s, s, n)
+
+    @staticmethod
+    def backward(ctx, gr):
+        (s,) = ctx.saved_tensors
+        return ops.autocorr_adjoint(gr.contiguous(), s, ctx.n), None
+
+
 class DopplerPower(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, taps, nd: int):
@@ -66,6 +83,10 @@ def doppler_power(x, taps, nd: int):
     return DopplerPower.apply(x, taps, nd)
 
 
+def autocorr(s, n: int):
+    return AutoCorr.apply(s, n)
+
+
 def sidelobe_mask(nd: int, kd: torch.Tensor, ml: int) -> torch.Tensor:
     """[T, nd] bool: Doppler bins farther than ml (circularly) from kd."""
     k = torch.arange(nd, device=kd.device)[None, :]
@@ -79,7 +100,7 @@ def design_loss(phi, taps, cube, targets, n: int, nd: int, lam: float = 1.0, ml:
     targets: [T, 3] int64 (channel, range bin, Doppler bin).  Returns
     mean ISL + lam * mean PSL (the definitions of oracle/grad.py)."""
     s = torch.polar(torch.ones_like(phi), phi)
-    r = fft_conv(s, s, n)
+    r = autocorr(s, n)
     pr = r.real ** 2 + r.imag ** 2
     isl = (pr.sum(-1) - pr[:, 0]) / pr[:, 0]
     pw = doppler_power(cube, taps, nd)

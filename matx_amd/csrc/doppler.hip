@@ -369,19 +369,21 @@ __global__ __launch_bounds__(256) void doppler_taps_adj_kernel(DopAdjArgs a) {
   }
 }
 
-// gh[p] = sum over parts of partial[part][p], in part order (double).
+// gh[p] = sum over parts of partial[part][p]: one workgroup per tap, a
+// fixed-order tree in double (deterministic).
 __global__ __launch_bounds__(256) void taps_reduce_kernel(const float *partial, int nparts, int P,
                                                           float *gh, int64_t ghs) {
-  __shared__ double acc[4][64];
-  const int p = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int q = threadIdx.x >> 6;  // four interleaved part streams
+  __shared__ double acc[256];
+  const int p = blockIdx.x;
   double s = 0.0;
-  if (p < P)
-    for (int part = q; part < nparts; part += 4) s += (double)partial[(int64_t)part * P + p];
-  acc[q][threadIdx.x & 63] = s;
+  for (int part = threadIdx.x; part < nparts; part += 256) s += (double)partial[(int64_t)part * P + p];
+  acc[threadIdx.x] = s;
   __syncthreads();
-  if (q == 0 && p < P) gh[(int64_t)p * ghs] = (float)(((acc[0][threadIdx.x] + acc[1][threadIdx.x]) +
-                                                      acc[2][threadIdx.x]) + acc[3][threadIdx.x]);
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) acc[threadIdx.x] += acc[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gh[(int64_t)p * ghs] = (float)acc[0];
 }
 
 template <int ND>
@@ -400,7 +402,7 @@ static mxr_status launch_taps_adj(const DopAdjArgs &a, const mxr_tensor *gtaps, 
   MXR_HIP_TRY(hipGetLastError());
   if (gtaps) {
     ProfScope ps("taps_reduce_kernel", s);
-    taps_reduce_kernel<<<dim3((a.P + 63) / 64), dim3(256), 0, s>>>(
+    taps_reduce_kernel<<<dim3(a.P), dim3(256), 0, s>>>(
         a.partial, a.C * a.tiles, a.P, (float *)gtaps->data, gtaps->stride[0]);
     MXR_HIP_TRY(hipGetLastError());
   }

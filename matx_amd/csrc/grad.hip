@@ -29,7 +29,7 @@ struct RowOp {
 
 struct XcArgs {
   RowOp a, b;        // out_b = IFFT(FFT(a_b) .* f(FFT(b_b)))
-  int mode;          // 0: A.*conj(B), 1: A.*B, 3: |A|^2 (b unused)
+  int mode;          // 0: A.*conj(B), 1: A.*B, 3: |A|^2 (b unused), 4: A.*2Re(B)
   float2 *out;
   int64_t nout, os;  // kept lags, out element stride
   BatchIdx obi;      // sa: out offsets per batch dim
@@ -78,7 +78,8 @@ __global__ __launch_bounds__(XcCfg<M>::NT) void xcorr_rows_kernel(XcArgs a) {
       r = make_float2(cabs2(x), 0.f);
     } else {
       const float2 y = B[LY::addr(0, i)];
-      r = a.mode == 0 ? cmulc(x, y) : cmul(x, y);
+      if (a.mode == 4) r = cscale(x, 2.0f * y.x);
+      else r = a.mode == 0 ? cmulc(x, y) : cmul(x, y);
     }
     A[LY::addr(0, i)] = r;
   }
@@ -229,7 +230,34 @@ static mxr_status fft_conv_adjoint_impl(const mxr_tensor *gx, const mxr_tensor *
   return MXR_SUCCESS;
 }
 
+// Gradient w.r.t. s of r = fft_conv(s, s, n) (a waveform's matched filter
+// to itself): gx + gw = IFFT(GY .* S) + IFFT(S .* conj(GY)) = IFFT(S .* 2Re(GY)).
+static mxr_status autocorr_adjoint_impl(const mxr_tensor *gs, const mxr_tensor *gy,
+                                        const mxr_tensor *s, int64_t n, hipStream_t st_) {
+  const char *fn = "autocorr_adjoint";
+  mxr_status st;
+  if ((st = check_tensor(gy, "gy", MXR_C64, 1, MXR_MAX_RANK))) return st;
+  if ((st = check_tensor(gs, "gs", MXR_C64, 1, MXR_MAX_RANK))) return st;
+  RowOp S{}, GY{};
+  if ((st = row_op(s, gy, fn, "s", &S))) return st;
+  if ((st = row_op(gy, gy, fn, "gy", &GY))) return st;
+  MXR_REQUIRE(S.len <= n && GY.len <= n, MXR_INVALID_SIZE,
+              "%s: rows longer than the transform size %lld", fn, (long long)n);
+  MXR_REQUIRE(gs->rank == s->rank, MXR_INVALID_SIZE, "%s: gs must have s's shape", fn);
+  for (int d = 0; d < gs->rank; ++d)
+    MXR_REQUIRE(gs->shape[d] == s->shape[d], MXR_INVALID_SIZE, "%s: gs must have s's shape", fn);
+  MXR_REQUIRE(!views_overlap(*gs, *gy) && !views_overlap(*gs, *s), MXR_ALIAS_ERROR,
+              "%s: gs overlaps an input", fn);
+  return xc_run(fn, gs, S, GY, 4, n, st_);
+}
+
 }  // namespace mxr
+
+extern "C" mxr_status mxr_autocorr_adjoint(const mxr_tensor *gs, const mxr_tensor *gy,
+                                           const mxr_tensor *s, int64_t n, hipStream_t stream) {
+  return mxr::guarded("mxr_autocorr_adjoint",
+                      [&] { return mxr::autocorr_adjoint_impl(gs, gy, s, n, stream); });
+}
 
 extern "C" mxr_status mxr_fft_conv_adjoint(const mxr_tensor *gx, const mxr_tensor *gw,
                                            const mxr_tensor *gy, const mxr_tensor *x,

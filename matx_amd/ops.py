@@ -183,6 +183,15 @@ def fft_conv_adjoint(gy, x, w, n: int, conj: bool = True, want_gx: bool = True,
     return gx, gw
 
 
+def autocorr_adjoint(gy, s, n: int, stream=None):
+    """Gradient w.r.t. s of r = fft_conv(s, s, n) (rows' matched filters to
+    themselves) for gy = dL/dRe(r) + i dL/dIm(r)."""
+    gs = torch.empty_like(s)
+    L.check(L.lib().mxr_autocorr_adjoint(L.ref(L.desc(gs)), L.ref(L.desc(gy)), L.ref(L.desc(s)),
+                                         int(n), L.stream_handle(stream)))
+    return gs
+
+
 def doppler_taps(x, taps, nd: int, want_rd: bool = False, want_pow: bool = True, stream=None):
     """Doppler stage with an arbitrary real window: pow = |FFT_nd(taps .* x)|^2
     along pulses (x: [C, P, R]).  Returns (pow, rd)."""

@@ -127,3 +127,14 @@ def test_design_loss_gradient(mx, size):
     assert abs(float(L) - L64) / abs(L64) < 1e-5
     assert rel(host(p.grad), gphi64) < GTOL
     assert rel(host(t.grad), gh64) < GTOL
+
+
+def test_autocorr_adjoint_is_sum_of_both_terms(mx):
+    rng = np.random.default_rng(9)
+    B, S, n = 5, 1000, 4096
+    s, gy = crandn(rng, B, S), crandn(rng, B, n)
+    gs = mx.ops.autocorr_adjoint(dev(gy), dev(s), n)
+    Sf = np.fft.fft(s.astype(complex), n, axis=-1)
+    GY = np.fft.fft(gy.astype(complex), n, axis=-1)
+    ref = (np.fft.ifft(GY * Sf) + np.fft.ifft(Sf * np.conj(GY)))[:, :S]
+    assert rel(host(gs), ref) < RTOL
