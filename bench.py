@@ -121,7 +121,12 @@ def cpu_baseline_leg(x0, w, pow0, dets0, nd, budget_s, C, mask):
     cpu = {"value": reps / C / el, "unit": "cubes/s", "cores": cores, "kind": "port",
            "sample": f"FP64 oracle (oracle/mxr_oracle.c, OpenMP) on channel 0 of the "
                      f"same cube (1/{C} cube), {reps} runs in {el:.1f} s"}
+    # the CFAR stage alone: the FP64 CFAR applied to the GPU's own power map
+    cf = orc.cfar(pow0[None].astype(np.float64), mask)
+    near_cf = np.abs(cf["margin"][0]) < 1e-5
+    mism_cf = int(((dets0 != cf["dets"][0]) & ~near_cf).sum())
     parity = {"channel": 0, "pow_rel_l2_vs_fp64": rel, "det_mismatch_outside_1e-5": mism,
+              "det_mismatch_cfar_stage": mism_cf,
               "dets_gpu": int(dets0.sum()), "dets_oracle": int(st["dets"][0].sum())}
     return cpu, parity
 
