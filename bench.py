@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall budget of the CPU baseline leg (0 disables it)")
+    ap.add_argument("--overlap", action="store_true",
+                    help="radar/large: overlap consecutive cubes on two streams (measured slower "
+                         "on MI355X: the co-resident kernels starve each other of LDS)")
     ap.add_argument("--no-profile", action="store_true",
                     help="time without per-kernel events (roofline omitted)")
     ap.add_argument("--workload", default="radar", choices=["radar", "fftconv", "large", "grad", "ragged"],
@@ -542,11 +545,13 @@ def bench_pipeline(args, cfg):
     x, w, targets = synth_cube(C, P, S, L, nd, cfg["targets"], cfg["snr"], args.seed, dev,
                                bw=cfg["bw"])
     fused = nd <= 128
+    overlap = args.overlap
     pipe = matx_amd.FusedRadarPipeline(C, P, S, L, nd=nd, tpc="valid", win=win, guard=guard,
-                                       want_pow=not fused)
+                                       want_pow=not fused, overlap=overlap)
     pipe.set_waveform(w)
     for _ in range(warmup):
         pipe.step(x)
+    pipe.sync()
     torch.cuda.synchronize()
 
     lib = _lib.lib()
@@ -561,6 +566,7 @@ def bench_pipeline(args, cfg):
     e0.record(stream)
     for _ in range(steps):
         pipe.step(x)
+    pipe.sync()
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
@@ -602,6 +608,7 @@ def bench_pipeline(args, cfg):
     # one more cube with the power map written, for the parity check
     pipe.set_outputs(want_pow=True, want_ba=False)
     pipe.step(x)
+    pipe.sync()
     torch.cuda.synchronize()
     dets = pipe.dets
     det_cells = int(dets.sum().item())
@@ -629,6 +636,8 @@ def bench_pipeline(args, cfg):
         "data": f"synthetic (noise + {len(targets)} LFM point targets, seeded)",
         "config": {"workload": cfg["desc"],
                    "channels": C, "pulses": P, "samples": S, "waveform": L, "doppler": nd,
+                   "cube_overlap": "consecutive cubes on two streams, double-buffered compressed "
+                                   "cube" if overlap else "one stream",
                    "algorithmic_bytes_per_cube": B,
                    "pipeline_hbm_frac": round(B * value / (HBM_PEAK_GBS * 1e9), 4),
                    "per_kernel_us": {k: round(v["avg_us"], 2) for k, v in kern.items() if v["launches"]},

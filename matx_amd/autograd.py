@@ -101,7 +101,7 @@ def design_loss(phi, taps, cube, targets, n: int, nd: int, lam: float = 1.0, ml:
     mean ISL + lam * mean PSL (the definitions of oracle/grad.py)."""
     s = torch.polar(torch.ones_like(phi), phi)
     r = autocorr(s, n)
-    pr = r.real ** 2 + r.imag ** 2
+    pr = torch.view_as_real(r).square().sum(-1)   # |r|^2 on contiguous floats
     isl = (pr.sum(-1) - pr[:, 0]) / pr[:, 0]
     pw = doppler_power(cube, taps, nd)
     c, rb, kd = targets[:, 0], targets[:, 1], targets[:, 2]

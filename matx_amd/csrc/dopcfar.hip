@@ -76,6 +76,7 @@ __device__ __forceinline__ void st32(T *base, uint32_t byte_off, T v) {
 
 constexpr int DC_TB = 64;   // computed range columns per tile
 constexpr int DC_NT = 512;  // threads per workgroup
+constexpr int DC_PS = 64;   // power-tile row stride (floats)
 
 template <int HX>
 struct DcGeom {
@@ -120,7 +121,8 @@ void dopcfar_kernel(DcArgs a) {
   constexpr int HXA = DcGeom<HX>::HXA, OUT = DcGeom<HX>::OUT;
   static_assert(OUT >= 16, "window too wide for a 64-column tile");
   static_assert(ND >= NW, "Doppler size below one row per wave");
-  static_assert(2 * HY <= ND, "power tile and its zero rows must fit the complex tile");
+  constexpr int PS = DC_PS;
+  static_assert((ND + 2 * HY) * PS <= 2 * ND * TB, "power tile and its zero rows must fit the complex tile");
   constexpr int SR = ND / NW;  // Doppler-input rows per wave in the load phase
   using LY = Layout<ND, TB, true>;
   using SQ = Seq<ND, false>;
@@ -181,7 +183,7 @@ void dopcfar_kernel(DcArgs a) {
   // ---- 2. Doppler FFT down the columns; the last pass leaves |X|^2 in LDS
   // as rows 0..ND-1 of a power tile with HY zero rows above and below (the
   // CFAR's zero padding along Doppler)
-  float *const pwt = reinterpret_cast<float *>(lds) + HY * TB;  // row 0
+  float *const pwt = reinterpret_cast<float *>(lds) + HY * PS;  // row 0
   if constexpr (K > 1) lds_passes<LY, SQ, ND, NT, TB, false, 0, K - 1>(lds, a.tw);
   {
     constexpr int RL = SQ::template R<K - 1>;
@@ -191,10 +193,10 @@ void dopcfar_kernel(DcArgs a) {
     wg_barrier();  // the power map overwrites the tile
     for (int i = threadIdx.x; i < 2 * HY * TB; i += NT) {
       const int row = i / TB < HY ? i / TB - HY : ND + i / TB - HY;
-      pwt[row * TB + (i % TB)] = 0.f;
+      pwt[row * PS + (i % TB)] = 0.f;
     }
     pl.template compute<LSL, false>(a.tw + SQ::template TWOFF<K - 1>);
-    pl.template store<LSL>([&](int t, int k, float2 x) { pwt[k * TB + t] = cabs2(x); });
+    pl.template store<LSL>([&](int t, int k, float2 x) { pwt[k * PS + t] = cabs2(x); });
   }
   wg_barrier();
 
@@ -242,7 +244,7 @@ void dopcfar_kernel(DcArgs a) {
     if (ly >= NLY) break;
     const int k = k0 - HY + ly;  // in [-HY, ND + HY): zero rows outside the map
     float u[NV];
-    const float *row = pwt + k * TB + tc0 - HXA;
+    const float *row = pwt + k * PS + tc0 - HXA;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const float4 z = *reinterpret_cast<const float4 *>(row + 4 * q);
@@ -279,7 +281,7 @@ void dopcfar_kernel(DcArgs a) {
         thr[j] = thr_tab[min(max(nrm[j], 0), NCELL)];
       }
     }
-    const float4 cut = *reinterpret_cast<const float4 *>(pwt + kc * TB + tc0);
+    const float4 cut = *reinterpret_cast<const float4 *>(pwt + kc * PS + tc0);
     const float cv[4] = {cut.x, cut.y, cut.z, cut.w};
     int det[4];
     float bav[4];

@@ -19,9 +19,10 @@ namespace mxr {
 template <int ND>
 struct DopCfg {
   static constexpr int NT = 256;
-  // 32 range bins (256-byte rows) per tile up to 128 Doppler bins, 32 KiB of
-  // LDS for 128 pulses: four workgroups per CU keep enough loads in flight
-  static constexpr int TR = ND <= 128 ? 32 : 4096 / ND;
+  // 32 range bins (256-byte rows) per tile up to 256 Doppler bins (32 KiB of
+  // LDS at 128, 64 KiB at 256), then 64 KiB tiles (16 bins at 512, 8 at
+  // 1024): narrower tiles waste most of every 128-byte line they touch
+  static constexpr int TR = ND <= 256 ? 32 : 8192 / ND;
 };
 
 // One tile: TR range bins from r0 of channel c.  lds holds rows_buf * TR
@@ -195,7 +196,7 @@ static mxr_status launch_doppler(const DopArgs &a, hipStream_t s) {
   const size_t lds = (size_t)rows_buf * TR * sizeof(float2);
   MXR_REQUIRE(lds + ND * sizeof(float) <= (size_t)max_lds_per_block(), MXR_NOT_SUPPORTED,
               "doppler: %d pulses x %d range bins do not fit in LDS", rows_buf, TR);
-  if (lds > 65536)
+  if (lds + ND * sizeof(float) > 65536)
     MXR_HIP_TRY(hipFuncSetAttribute((const void *)doppler_kernel<ND>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   dim3 grid((a.R + TR - 1) / TR, a.C);
@@ -392,7 +393,7 @@ static mxr_status launch_taps_adj(const DopAdjArgs &a, const mxr_tensor *gtaps, 
   const size_t lds = (size_t)ND * TR * sizeof(float2);
   MXR_REQUIRE(lds + ND * sizeof(float) <= (size_t)max_lds_per_block(), MXR_NOT_SUPPORTED,
               "doppler_taps_adjoint: tile does not fit in LDS");
-  if (lds > 65536)
+  if (lds + ND * sizeof(float) > 65536)
     MXR_HIP_TRY(hipFuncSetAttribute((const void *)doppler_taps_adj_kernel<ND>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   {

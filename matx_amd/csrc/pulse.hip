@@ -12,8 +12,9 @@
 // (4096 for a 9000-sample pulse and a 1000-sample waveform, 36% fewer flops
 // than one 16384-point transform), and M = n reproduces the plain n-point
 // form.  Each block is one fused kernel pass: load (zero-padding implicit),
-// forward Stockham passes, conj-multiply, inverse passes with the 1/M scale
-// folded in, store of the kept lags.  The last forward pass and the first
+// forward Stockham passes, conj-multiply (the waveform spectrum carries the
+// 1/M of the inverse, an exact power-of-two scale), inverse passes, store of
+// the kept lags.  The last forward pass and the first
 // inverse pass share one radix, so they run back to back in registers.
 #include <cmath>
 #include <map>
@@ -38,9 +39,8 @@ struct PcArgs {
   int64_t nblk;     // blocks per row
   int64_t bpw;      // blocks per workgroup (nblk when y aliases x)
   int64_t groups;   // total workgroups
-  const float2 *wc; // M-point waveform spectrum
+  const float2 *wc; // M-point waveform spectrum, scaled by 1/M
   const float2 *twf, *twi;
-  float scale;
 };
 
 template <int M>
@@ -55,8 +55,7 @@ struct OlsBlock {
   const float2 *xr;
   float2 *yr;
   int lo, n, seff, nout, V;
-  const float2 *wc;
-  float scale;
+  const float2 *wc;  // M-point spectrum, scaled by 1/M
 };
 
 // Body shared by the per-call kernel and the ragged-batch kernels.  Pre: the
@@ -110,19 +109,105 @@ __device__ __forceinline__ void ols_block(const OlsBlock &ob, const float2 *twf,
   wg_barrier();
   lds_passes<LY, SI, M, NT, 1, true, 1, K - 1>(lds, twi);
 
-  // last inverse pass straight to HBM with the 1/M scale
+  // last inverse pass straight to HBM
   {
     constexpr int RI = SI::template R<K - 1>;
     constexpr int LSI = SI::template LS<K - 1>;
     Pass<LY, RI, M, NT, 1> ps;
     ps.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
     ps.template compute<LSI, true>(twi + SI::template TWOFF<K - 1>);
-    const float s = ob.scale;
     const int lim = (ob.nout - lo) < ob.V ? (ob.nout - lo) : ob.V;
     ps.template store<LSI>([&](int, int i, float2 v) {
-      if (i < lim) yr[lo + i] = cscale(v, s);
+      if (i < lim) yr[lo + i] = v;  // the spectrum carries the 1/M
     });
   }
+}
+
+// TB = 4096/M small blocks per 256-thread workgroup (M <= 2048): transform
+// t runs the block obs[t] (kept in LDS).  Same passes as ols_block; an
+// inactive transform has seff = 0 (reads nothing) and nout = lo (stores
+// nothing).
+template <int M>
+__device__ __forceinline__ void ols_multi(const OlsBlock *obs, const float2 *twf,
+                                          const float2 *twi, float2 *lds) {
+  constexpr int NT = 256, TB = 4096 / M;
+  using LY = Layout<M, TB, false>;
+  using SF = Seq<M, false>;
+  using SI = Seq<M, true>;
+  constexpr int K = SF::n;
+  constexpr int R0 = SF::template R<0>;
+  constexpr int RL = SF::template R<K - 1>;
+  constexpr int LSL = SF::template LS<K - 1>;
+  constexpr int NBL = M / RL;
+  static_assert(K >= 2 && R0 == 16, "small-block kernel expects a radix-16 first pass");
+  {
+    Pass<LY, R0, M, NT, TB> ps;
+    ps.load([&](int t, int i) {
+      const OlsBlock &ob = obs[t];
+      int gi = ob.lo + i;
+      if (gi >= ob.n) gi -= ob.n;
+      return gi < ob.seff ? ob.xr[gi] : make_float2(0.f, 0.f);
+    });
+    ps.template compute<1, false>(nullptr);
+    ps.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
+  }
+  wg_barrier();
+  lds_passes<LY, SF, M, NT, TB, false, 1, K - 1>(lds, twf);
+  {
+    Pass<LY, RL, M, NT, TB> ps;
+    ps.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
+    wg_barrier();
+    ps.template compute<LSL, false>(twf + SF::template TWOFF<K - 1>);
+#pragma unroll
+    for (int u = 0; u < ps.BPT; ++u) {
+      if (!ps.active(u)) continue;
+      const int g = threadIdx.x + u * NT;
+      const int t = g / NBL, j = g % NBL;  // k == j in the last pass
+      const float2 *wc = obs[t].wc;
+#pragma unroll
+      for (int q = 0; q < RL; ++q) ps.v[u][q] = cmul(ps.v[u][q], wc[j + q * LSL]);
+    }
+    ps.template compute<1, true>(nullptr);
+    ps.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
+  }
+  wg_barrier();
+  lds_passes<LY, SI, M, NT, TB, true, 1, K - 1>(lds, twi);
+  {
+    constexpr int RI = SI::template R<K - 1>;
+    constexpr int LSI = SI::template LS<K - 1>;
+    Pass<LY, RI, M, NT, TB> ps;
+    ps.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
+    ps.template compute<LSI, true>(twi + SI::template TWOFF<K - 1>);
+    ps.template store<LSI>([&](int t, int i, float2 v) {
+      const OlsBlock &ob = obs[t];
+      const int lim = (ob.nout - ob.lo) < ob.V ? (ob.nout - ob.lo) : ob.V;
+      if (i < lim) ob.yr[ob.lo + i] = v;  // the spectrum carries the 1/M
+    });
+  }
+}
+
+// Per-call small blocks: 4096/M (row, block) groups per workgroup.
+template <int M>
+__global__ __launch_bounds__(256) void pc_ols_multi_kernel(PcArgs a) {
+  constexpr int TB = 4096 / M;
+  __shared__ float2 lds[Layout<M, TB, false>::LDS_ELEMS];
+  __shared__ OlsBlock obs[TB];
+  if (threadIdx.x < TB) {
+    const int64_t g0 = (int64_t)blockIdx.x * TB;
+    const int64_t g = g0 + threadIdx.x < a.groups ? g0 + threadIdx.x : g0;
+    const int64_t row = g / a.nblk;
+    int64_t ox, oy;
+    batch_offsets(a.bi, row, &ox, &oy);
+    OlsBlock ob{a.x + ox, a.y + oy, (int)((g % a.nblk) * a.V), (int)a.n, (int)a.seff,
+                (int)a.nout, (int)a.V, a.wc};
+    if (g0 + threadIdx.x >= a.groups) {  // past the last group: idle transform
+      ob.seff = 0;
+      ob.nout = ob.lo;
+    }
+    obs[threadIdx.x] = ob;
+  }
+  __syncthreads();
+  ols_multi<M>(obs, a.twf, a.twi, lds);
 }
 
 // ROWLOOP: one workgroup walks blocks [b0, b1) of its row in order (needed
@@ -140,7 +225,7 @@ void pc_ols_kernel(PcArgs a) {
   const int64_t b1 = ROWLOOP ? (b0 + a.bpw < a.nblk ? b0 + a.bpw : a.nblk) : b0 + 1;
   int64_t ox, oy;
   batch_offsets(a.bi, row, &ox, &oy);
-  OlsBlock ob{a.x + ox, a.y + oy, 0, (int)a.n, (int)a.seff, (int)a.nout, (int)a.V, a.wc, a.scale};
+  OlsBlock ob{a.x + ox, a.y + oy, 0, (int)a.n, (int)a.seff, (int)a.nout, (int)a.V, a.wc};
   for (int64_t b = b0; b < b1; ++b) {
     if (b != b0) wg_barrier();  // previous block's LDS reads are done
     ob.lo = (int)(b * a.V);
@@ -252,7 +337,7 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int i = j + q * 256;
-    if (i < lim) yr[lo + i] = cscale(v[q], ob.scale);
+    if (i < lim) yr[lo + i] = v[q];  // the spectrum carries the 1/M
   }
 }
 
@@ -266,7 +351,7 @@ void pc4096_kernel(Pc4096Args pa) {
   int64_t ox, oy;
   batch_offsets(a.bi, row, &ox, &oy);
   const OlsBlock ob{a.x + ox, a.y + oy, (int)((g % a.nblk) * a.V), (int)a.n, (int)a.seff,
-                    (int)a.nout, (int)a.V, a.wc, a.scale};
+                    (int)a.nout, (int)a.V, a.wc};
   ols_block4096(ob, pa.base1, pa.base2, lds);
 }
 
@@ -331,9 +416,10 @@ __device__ __forceinline__ void wfspec_body(const WfArgs &a, float2 *lds, double
   Pass<LY, RL, M, NT, 1> pl;
   pl.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
   pl.template compute<LSL, false>(a.tw + SF::template TWOFF<K - 1>);
+  const float os = a.out_scale;
   pl.template store<LSL>([&](int t, int i, float2 x) {
     if (a.conj) x.y = -x.y;
-    a.wc[i] = x;
+    a.wc[i] = cscale(x, os);
   });
 }
 
@@ -370,7 +456,7 @@ __device__ __forceinline__ OlsBlock job_block(const PcJob &jb, int64_t it) {
   const int b = (int)(local % jb.nblk);
   const int c = (int)(row / jb.P), p = (int)(row % jb.P);
   return OlsBlock{jb.x + c * jb.x_cs + p * jb.x_ps, jb.y + c * jb.y_cs + p * jb.y_ps, b * jb.V,
-                  jb.n, jb.seff, jb.nout, jb.V, jb.wc, jb.scale};
+                  jb.n, jb.seff, jb.nout, jb.V, jb.wc};
 }
 
 // PERSIST: a fixed grid of resident workgroups loops over the items;
@@ -391,6 +477,29 @@ void pc_batch_kernel(const PcJob *jobs, int njobs, int64_t items, const float2 *
     if (it != blockIdx.x) wg_barrier();  // previous item's LDS reads are done
     ols_block<M>(job_block(jb, it), opaque(twf), opaque(twi), lds);
   }
+}
+
+// Ragged batch, small blocks: 4096/M consecutive items per workgroup (items
+// of different jobs may share one).
+template <int M>
+__global__ __launch_bounds__(256) void pc_batch_multi_kernel(const PcJob *jobs, int njobs,
+                                                             int64_t items, const float2 *twf,
+                                                             const float2 *twi) {
+  constexpr int TB = 4096 / M;
+  __shared__ float2 lds[Layout<M, TB, false>::LDS_ELEMS];
+  __shared__ OlsBlock obs[TB];
+  if (threadIdx.x < TB) {
+    const int64_t it0 = (int64_t)blockIdx.x * TB;
+    const int64_t it = it0 + threadIdx.x < items ? it0 + threadIdx.x : it0;
+    OlsBlock ob = job_block(jobs[find_job(jobs, njobs, it)], it);
+    if (it0 + threadIdx.x >= items) {  // past the last item: idle transform
+      ob.seff = 0;
+      ob.nout = ob.lo;
+    }
+    obs[threadIdx.x] = ob;
+  }
+  __syncthreads();
+  ols_multi<M>(obs, twf, twi, lds);
 }
 
 template <bool PERSIST>
@@ -474,11 +583,20 @@ mxr_status launch_pc_batch(int M, const PcJob *d_jobs, int njobs, int64_t items,
     else                                                                                         \
       pc_batch_kernel<MM, false><<<dim3(grid), dim3(PcCfg<MM>::NT), 0, s>>>(d_jobs, njobs, items, twf, twi); \
     break;
-    MXR_PB(1024)
-    MXR_PB(2048)
+#define MXR_PBM(MM)                                                                              \
+  case MM:                                                                                       \
+    if (persist)                                                                                 \
+      pc_batch_kernel<MM, true><<<dim3(grid), dim3(PcCfg<MM>::NT), 0, s>>>(d_jobs, njobs, items, twf, twi); \
+    else                                                                                         \
+      pc_batch_multi_kernel<MM><<<dim3((unsigned)((items + 4096 / MM - 1) / (4096 / MM))), dim3(256), 0, s>>>( \
+          d_jobs, njobs, items, twf, twi);                                                       \
+    break;
+    MXR_PBM(1024)
+    MXR_PBM(2048)
     MXR_PB(8192)
     MXR_PB(16384)
 #undef MXR_PB
+#undef MXR_PBM
     default:
       set_error(MXR_NOT_SUPPORTED, "ragged batch: block size %d unsupported", M);
       return MXR_NOT_SUPPORTED;
@@ -608,10 +726,14 @@ mxr_status launch_pc(const WfArgs *wa, const PcArgs *pa, hipStream_t s) {
   }
   {
     ProfScope ps("pc_ols_kernel", s);
-    if (pa->bpw > 1)
+    if (pa->bpw > 1) {
       pc_ols_kernel<M, true><<<dim3((unsigned)pa->groups), dim3(PcCfg<M>::NT), 0, s>>>(*pa);
-    else
+    } else if constexpr (M <= 2048) {
+      constexpr int TB = 4096 / M;
+      pc_ols_multi_kernel<M><<<dim3((unsigned)((pa->groups + TB - 1) / TB)), dim3(256), 0, s>>>(*pa);
+    } else {
       pc_ols_kernel<M, false><<<dim3((unsigned)pa->groups), dim3(PcCfg<M>::NT), 0, s>>>(*pa);
+    }
   }
   MXR_HIP_TRY(hipGetLastError());
   return MXR_SUCCESS;
@@ -701,7 +823,6 @@ static mxr_status pc_blocks(const char *fn, const PcPlan &plan, bool in_place, c
   pa->wc = wc;
   pa->twf = twf;
   pa->twi = twi;
-  pa->scale = (float)(1.0 / (double)plan.M);
   return MXR_SUCCESS;
 }
 
@@ -742,6 +863,7 @@ mxr_status pc_common(const char *fn, const mxr_tensor *y, const mxr_tensor *x, c
   wa.conj = conj_w;
   wa.wc = wc;
   wa.tw = twf;
+  wa.out_scale = (float)(1.0 / (double)plan.M);  // exact: a power of two
   if ((st = pc_blocks(fn, plan, in_place, wc, twf, twi, &pa))) return st;
   return dispatch_pc(plan.M, &wa, &pa, stream);
 }
@@ -811,6 +933,7 @@ static mxr_status pc_plan_set_waveform(mxr_pc_plan h, const mxr_tensor *w, hipSt
   wa.conj = 1;
   wa.wc = p->wc;
   wa.tw = p->twf;
+  wa.out_scale = (float)(1.0 / (double)p->plan.M);
   return dispatch_pc(p->plan.M, &wa, nullptr, s);
 }
 

@@ -120,7 +120,6 @@ mxr_status autotune_block(int64_t Lc, int64_t n, hipStream_t s, int *bestM) {
     j.V = (int)V;
     j.nblk = (int)nblk;
     j.wc = wc;
-    j.scale = 1.0f / (float)M;
     j.item0 = 0;
     if ((e = hipMemcpyAsync(dj, &j, sizeof(PcJob), hipMemcpyHostToDevice, s)) != hipSuccess) {
       fail(e);
@@ -274,7 +273,6 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
       pj.V = (int)V;
       pj.nblk = (int)nblk;
       pj.wc = p->spec + soff[i];
-      pj.scale = (float)(1.0 / (double)M);
       pj.item0 = cl.items;
       cl.items += j.C * j.P * nblk;
       hpc.push_back(pj);
@@ -288,6 +286,7 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
       wa.conj = 1;
       wa.wc = p->spec + soff[i];
       wa.tw = tw;
+      wa.out_scale = (float)(1.0 / (double)M);
       hwf.push_back(wa);
     }
     cl.njobs = (int)(hpc.size() - first);

@@ -34,6 +34,7 @@ struct WfArgs {
   int window, normalize, writeback, conj;
   float2 *wc;
   const float2 *tw;
+  float out_scale;  // applied to the stored spectrum (the block kernels' 1/M)
 };
 
 // One job of a ragged pulse-compression batch: rows (c, p) of a [C, P, S]
@@ -45,8 +46,7 @@ struct PcJob {
   int64_t y_cs, y_ps;
   int C, P;
   int seff, n, nout, V, nblk;
-  const float2 *wc;
-  float scale;
+  const float2 *wc;  // the job's spectrum, scaled by 1/M
   int64_t item0;  // first batch item (row-block) of this job
 };
 
