@@ -166,6 +166,21 @@ mxr_status mxr_pc_plan_execute(mxr_pc_plan plan, const mxr_tensor *y, const mxr_
                                hipStream_t stream);
 mxr_status mxr_pc_plan_destroy(mxr_pc_plan plan);
 
+/* Gradient of mxr_pc_plan_execute with respect to the raw waveform, summed
+ * over every row (no MatX counterpart: the backward of PulseCompression,
+ * examples/simple_radar_pipeline.h:276-300, for waveform design).  gy:
+ * [..., nout] c64 = dL/dRe y + i dL/dIm y for the rows of x ([..., S]); w:
+ * [L] the waveform the forward used; gw: [L] c64 receives dL/dw through the
+ * plan's window and normalisation.  With w2 the filter the forward applied,
+ *   g2[l] = sum_rows sum_{m < nout} x[(m + l) mod n] conj(gy[m]),
+ *   w1 = h .* w, nu = ||w1||, gw = h .* (g2/nu - w1 Re<g2, w1>/nu^3)
+ * (h = 1 without MXR_PC_WINDOW; gw = h .* g2 without MXR_PC_NORMALIZE).
+ * Runs over the plan's overlap-save blocks, recomputing the spectra of x
+ * and gy; the per-workgroup partial sums are reduced in a fixed order, so
+ * the result is deterministic. */
+mxr_status mxr_pc_plan_waveform_grad(mxr_pc_plan plan, const mxr_tensor *gw, const mxr_tensor *gy,
+                                     const mxr_tensor *x, const mxr_tensor *w, hipStream_t stream);
+
 /* ------------------------------------------------------------ pulse axis */
 /* tpc_mode value for mxr_doppler: y already holds the canceller output; all
  * of its rows feed the FFT (MatX's DopplerProcessing stage on tpcView,
@@ -199,6 +214,18 @@ mxr_status mxr_doppler_taps(const mxr_tensor *pow_out, const mxr_tensor *rd_out,
  *   gx = taps .* gu  ([C, P, R] c64, g = dL/dRe + i dL/dIm)
  *   gtaps[p] = sum_{c,r} Re(conj(gu[c,p,r]) x[c,p,r])  ([P] f32)
  * D is recomputed from x (not stored).  Either output may be NULL. */
+/* Adjoint of mxr_doppler's power output with respect to y (no MatX
+ * counterpart; the gradient of ThreePulseCanceller + DopplerProcessing +
+ * abs2, examples/simple_radar_pipeline.h:321-357,398).  Same tpc_mode,
+ * win_rows and nd as the forward call.  gpow: [C, nd, R] f32 = dL/dpow (any
+ * strides, 0 included); gy: [C, P, R] c64 = dL/dRe y + i dL/dIm y.  With
+ * t = window .* canceller(y), D = FFT_nd(t), s = window .* F^H(2 gpow .* D):
+ *   gy[p] = s[p-o+1] - 2 s[p-o] + s[p-o-1]   (o = 1 for VALID, else 0)
+ * or gy = s without a canceller.  D is recomputed from y (checkpointing:
+ * no spectrum is stored between forward and backward). */
+mxr_status mxr_doppler_adjoint(const mxr_tensor *gy, const mxr_tensor *gpow, const mxr_tensor *y,
+                               int tpc_mode, int64_t win_rows, int64_t nd, hipStream_t stream);
+
 mxr_status mxr_doppler_taps_adjoint(const mxr_tensor *gtaps, const mxr_tensor *gx,
                                     const mxr_tensor *gpow, const mxr_tensor *x,
                                     const mxr_tensor *taps, int64_t nd, hipStream_t stream);

@@ -28,7 +28,7 @@ EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_ena
            "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy",
            "mxr_fft_conv_adjoint", "mxr_autocorr_adjoint", "mxr_doppler_taps", "mxr_doppler_taps_adjoint",
            "mxr_ragged_plan_create", "mxr_ragged_execute", "mxr_ragged_plan_block_size",
-           "mxr_ragged_plan_destroy")
+           "mxr_ragged_plan_destroy", "mxr_doppler_adjoint", "mxr_pc_plan_waveform_grad")
 
 STATUS_NAMES = {0: "SUCCESS", 1: "NOT_SUPPORTED", 2: "INVALID_PARAMETER",
                 3: "INVALID_DIM", 4: "INVALID_SIZE", 5: "OUT_OF_MEMORY",
@@ -118,6 +118,8 @@ def lib():
         "mxr_ragged_execute": [V, V],
         "mxr_ragged_plan_block_size": [V, I, ctypes.POINTER(I)],
         "mxr_ragged_plan_destroy": [V],
+        "mxr_doppler_adjoint": [T, T, T, I, I64, I64, V],
+        "mxr_pc_plan_waveform_grad": [V, T, T, T, T, V],
     }
     for name, argt in sigs.items():
         fn = getattr(L, name)

@@ -10,6 +10,16 @@ DopplerPower: pow = |FFT_nd(taps .* x)|^2 along pulses; backward runs
               instead of keeping it (the Doppler-stage recomputation that keeps
               the saved state at the size of the input).
 
+RadarPower:   pow = doppler(pulse_compress(x, w), nd) for a whole cube and
+              one waveform (the PulseCompression -> ThreePulseCanceller ->
+              DopplerProcessing chain); backward runs mxr_doppler_adjoint
+              (Doppler spectrum recomputed from the compressed cube, the only
+              activation kept) and mxr_pc_plan_waveform_grad (the waveform
+              gradient summed over every pulse).
+WeightedPowerSum: L = sum_{c,k,r} wmap[k, r] pow[c, k, r]; its backward hands
+              on wmap expanded over channels (a stride-0 view, never
+              materialised), which the pulse-axis adjoint reads as is.
+
 `design_loss` is the waveform/window design objective: integrated sidelobe
 level of every waveform's matched-filter output plus a peak-sidelobe penalty
 on the Doppler spectra of a target cube.
@@ -73,6 +83,66 @@ class DopplerPower(torch.autograd.Function):
                                           want_gx=ctx.needs_input_grad[0],
                                           want_gtaps=ctx.needs_input_grad[1])
         return gx, gt, None
+
+
+class RadarPower(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, w, x, plan, nd: int, tpc: str, win_rows):
+        w = w.contiguous()
+        plan.set_waveform(w)
+        y = plan(x)
+        pw, _ = ops.doppler(y, nd, tpc=tpc, win_rows=win_rows)
+        ctx.save_for_backward(w, x, y)
+        ctx.plan, ctx.nd, ctx.tpc, ctx.win_rows = plan, nd, tpc, win_rows
+        return pw
+
+    @staticmethod
+    def backward(ctx, gpow):
+        w, x, y = ctx.saved_tensors
+        gy = ops.doppler_adjoint(gpow, y, ctx.nd, tpc=ctx.tpc, win_rows=ctx.win_rows)
+        gw = ctx.plan.waveform_grad(gy, x, w)
+        return gw, None, None, None, None, None
+
+
+class WeightedPowerSum(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, pw, wmap):
+        ctx.save_for_backward(wmap)
+        ctx.shape = pw.shape
+        C = pw.shape[0]
+        # one matrix-vector product over the map (no [C, nd, R] temporary)
+        return torch.mv(pw.reshape(C, -1), wmap.reshape(-1)).sum()
+
+    @staticmethod
+    def backward(ctx, g):
+        (wmap,) = ctx.saved_tensors
+        return (g * wmap).expand(ctx.shape), None
+
+
+def radar_power(w, x, plan, nd: int, tpc: str = "valid", win_rows=None):
+    """Doppler power map of cube x ([C, P, S] c64) filtered with waveform w
+    ([L] c64) by `plan` (ops.PulseCompressionPlan(S, L, n), windowed and
+    normalised); differentiable with respect to w."""
+    return RadarPower.apply(w, x, plan, nd, tpc, win_rows)
+
+
+def weighted_power_sum(pw, wmap):
+    return WeightedPowerSum.apply(pw, wmap)
+
+
+def contrast_weights(nd: int, R: int, cells, C: int = 1, device=None):
+    """[nd, R] f32 map: 1/(C n_clutter) on clutter cells, -1/(C n_target) on
+    the target cells (k, r), so weighted_power_sum(pw, map) = mean clutter
+    power - mean target power."""
+    wm = torch.empty((nd, R), dtype=torch.float32, device=device)
+    k = torch.as_tensor([c[0] for c in cells], dtype=torch.int64, device=device)
+    r = torch.as_tensor([c[1] for c in cells], dtype=torch.int64, device=device)
+    tgt = torch.zeros((nd, R), dtype=torch.bool, device=device)
+    tgt[k, r] = True
+    nt = int(tgt.sum())
+    wm.fill_(1.0 / (C * (nd * R - nt)))
+    wm[tgt] = -1.0 / (C * nt)
+    return wm
 
 
 def fft_conv(x, w, n: int, conj: bool = True):

@@ -493,7 +493,193 @@ static mxr_status doppler_taps_adjoint_impl(const mxr_tensor *gtaps, const mxr_t
   return MXR_NOT_SUPPORTED;
 }
 
+// ------------------------------------------------ pulse-axis adjoint
+// Backward of mxr_doppler's power output with respect to its input y:
+//   t_i = h_i (y[i+o-1] - 2 y[i+o] + y[i+o+1])   i < rows_t   (o = p_off)
+//   D = FFT_nd(t),  pow = |D|^2
+//   gD = 2 gpow D;  gu = F^H gD (unnormalised inverse DFT);  s_i = h_i gu_i
+//   gy[p] = s[p-o+1] - 2 s[p-o] + s[p-o-1]   (s = 0 outside [0, rows_t))
+// (no canceller: t_i = h_i y[i], gy[p] = s_p).  The canceller coefficients
+// are real, so its adjoint is the same symmetric stencil.  D is recomputed
+// from y inside the tile (checkpointing: no nd x R complex spectrum is kept
+// between the passes), so a tile reads y and gpow once and writes gy once.
+struct DopGradArgs {
+  const float2 *y;
+  int64_t ys_c, ys_p, ys_r;
+  const float *gp;
+  int64_t gs_c, gs_k, gs_r;
+  float2 *gy;
+  int64_t gys_c, gys_p, gys_r;
+  int C, P, R;
+  int rows_t, p_off, no_tpc, win_rows;
+  const float2 *tw;
+};
+
+template <int ND>
+__global__ __launch_bounds__(256) void doppler_adj_kernel(DopGradArgs a) {
+  constexpr int NT = DopCfg<ND>::NT;
+  constexpr int TR = DopCfg<ND>::TR;
+  using LY = Layout<ND, TR, true>;
+  using SQ = Seq<ND, false>;
+  constexpr int K = SQ::n;
+  extern __shared__ float2 lds[];  // max(P, ND) * TR: y tile, then D, then gu
+  __shared__ float win[ND];
+  const int c = blockIdx.y;
+  const int r0 = blockIdx.x * TR;
+  for (int i = threadIdx.x; i < ND; i += NT)
+    win[i] = i < a.rows_t ? (i < a.win_rows ? hamming_f32(i, a.win_rows) : 1.0f) : 0.0f;
+  const float2 *yc = a.y + (int64_t)c * a.ys_c;
+  for (int idx = threadIdx.x; idx < a.P * TR; idx += NT) {
+    const int p = idx / TR, t = idx % TR, r = r0 + t;
+    lds[p * TR + t] = r < a.R ? yc[(int64_t)p * a.ys_p + (int64_t)r * a.ys_r] : make_float2(0.f, 0.f);
+  }
+  wg_barrier();
+
+  // forward: first pass fed by the windowed canceller, the rest in LDS
+  constexpr int R0 = SQ::template R<0>;
+  {
+    Pass<LY, R0, ND, NT, TR> p0;
+    p0.load([&](int t, int i) -> float2 {
+      if (i >= a.rows_t) return make_float2(0.f, 0.f);
+      const int p = i + a.p_off;
+      const float2 m = lds[p * TR + t];
+      if (a.no_tpc) return cscale(m, win[i]);
+      const float2 s = p >= 1 ? lds[(p - 1) * TR + t] : make_float2(0.f, 0.f);
+      const float2 u = p + 1 < a.P ? lds[(p + 1) * TR + t] : make_float2(0.f, 0.f);
+      return cscale(make_float2(s.x + u.x - 2.0f * m.x, s.y + u.y - 2.0f * m.y), win[i]);
+    });
+    wg_barrier();  // every canceller read is done before the tile is overwritten
+    p0.template compute<1, false>(nullptr);
+    p0.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
+  }
+  wg_barrier();
+  lds_passes<LY, SQ, ND, NT, TR, false, 1, K>(lds, a.tw);
+
+  // gD = 2 gpow D
+  const float *gpc = a.gp + (int64_t)c * a.gs_c;
+  for (int idx = threadIdx.x; idx < ND * TR; idx += NT) {
+    const int k = idx / TR, t = idx % TR, r = r0 + t;
+    const float g = r < a.R ? gpc[(int64_t)k * a.gs_k + (int64_t)r * a.gs_r] : 0.f;
+    lds[LY::addr(t, k)] = cscale(lds[LY::addr(t, k)], 2.0f * g);
+  }
+  wg_barrier();
+  // gu = F^H gD: the same passes with conjugate twiddles, unscaled
+  lds_passes<LY, SQ, ND, NT, TR, true, 0, K>(lds, a.tw);
+
+  // window, canceller transpose, coalesced store
+  float2 *gyc = a.gy + (int64_t)c * a.gys_c;
+  auto s_at = [&](int t, int i) -> float2 {
+    return (i >= 0 && i < a.rows_t) ? cscale(lds[LY::addr(t, i)], win[i]) : make_float2(0.f, 0.f);
+  };
+  for (int idx = threadIdx.x; idx < a.P * TR; idx += NT) {
+    const int p = idx / TR, t = idx % TR, r = r0 + t;
+    if (r >= a.R) continue;
+    const int i = p - a.p_off;
+    float2 g;
+    if (a.no_tpc) {
+      g = s_at(t, i);
+    } else {
+      const float2 lo = s_at(t, i - 1), mid = s_at(t, i), hi = s_at(t, i + 1);
+      g = make_float2(lo.x + hi.x - 2.0f * mid.x, lo.y + hi.y - 2.0f * mid.y);
+    }
+    gyc[(int64_t)p * a.gys_p + (int64_t)r * a.gys_r] = g;
+  }
+}
+
+template <int ND>
+static mxr_status launch_doppler_adj(const DopGradArgs &a, hipStream_t s) {
+  constexpr int TR = DopCfg<ND>::TR;
+  const int rows_buf = a.P > ND ? a.P : ND;
+  const size_t lds = (size_t)rows_buf * TR * sizeof(float2);
+  MXR_REQUIRE(lds + ND * sizeof(float) <= (size_t)max_lds_per_block(), MXR_NOT_SUPPORTED,
+              "doppler_adjoint: %d pulses x %d range bins do not fit in LDS", rows_buf, TR);
+  if (lds + ND * sizeof(float) > 65536)
+    MXR_HIP_TRY(hipFuncSetAttribute((const void *)doppler_adj_kernel<ND>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  {
+    ProfScope ps("doppler_adj_kernel", s);
+    doppler_adj_kernel<ND><<<dim3((a.R + TR - 1) / TR, a.C), dim3(DopCfg<ND>::NT), lds, s>>>(a);
+  }
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
+static mxr_status doppler_adjoint_impl(const mxr_tensor *gy, const mxr_tensor *gpow,
+                                       const mxr_tensor *y, int tpc_mode, int64_t win_rows,
+                                       int64_t nd, hipStream_t stream) {
+  const char *fn = "doppler_adjoint";
+  mxr_status st;
+  if ((st = check_tensor(y, "doppler_adjoint y", MXR_C64, 3, 3))) return st;
+  if ((st = check_tensor(gpow, "doppler_adjoint gpow", MXR_F32, 3, 3))) return st;
+  if ((st = check_tensor(gy, "doppler_adjoint gy", MXR_C64, 3, 3))) return st;
+  MXR_REQUIRE(tpc_mode == MXR_CONV_SAME || tpc_mode == MXR_CONV_VALID || tpc_mode == MXR_TPC_NONE,
+              MXR_INVALID_PARAMETER, "%s: canceller mode must be SAME, VALID or NONE", fn);
+  const int64_t C = y->shape[0], P = y->shape[1], R = y->shape[2];
+  MXR_REQUIRE(P >= 3 || tpc_mode == MXR_TPC_NONE, MXR_INVALID_SIZE,
+              "%s: need at least 3 pulses, got %lld", fn, (long long)P);
+  const int64_t rows_t = tpc_mode == MXR_CONV_VALID ? P - 2 : P;
+  MXR_REQUIRE(nd >= rows_t, MXR_INVALID_SIZE,
+              "%s: %lld-point FFT is shorter than the %lld canceller rows", fn, (long long)nd,
+              (long long)rows_t);
+  MXR_REQUIRE(win_rows >= 0 && win_rows <= rows_t, MXR_INVALID_SIZE, "%s: bad window length", fn);
+  MXR_REQUIRE(gpow->shape[0] == C && gpow->shape[1] == nd && gpow->shape[2] == R, MXR_INVALID_SIZE,
+              "%s: gpow must be [%lld, %lld, %lld]", fn, (long long)C, (long long)nd, (long long)R);
+  MXR_REQUIRE(gy->shape[0] == C && gy->shape[1] == P && gy->shape[2] == R, MXR_INVALID_SIZE,
+              "%s: gy must have y's shape [%lld, %lld, %lld]", fn, (long long)C, (long long)P,
+              (long long)R);
+  MXR_REQUIRE(!views_overlap(*gy, *y) && !views_overlap(*gy, *gpow), MXR_ALIAS_ERROR,
+              "%s: gy overlaps an input", fn);
+  MXR_REQUIRE(nd >= 2 && nd <= 1024 && (nd & (nd - 1)) == 0, MXR_NOT_SUPPORTED,
+              "%s: FFT size %lld must be a power of two in [2, 1024]", fn, (long long)nd);
+  if (C == 0 || R == 0 || P == 0) return MXR_SUCCESS;
+  MXR_REQUIRE(C <= 65535 && R < (1ll << 30) && P < (1 << 20), MXR_INVALID_SIZE,
+              "%s: shape too large", fn);
+  DopGradArgs a{};
+  a.y = (const float2 *)y->data;
+  a.ys_c = y->stride[0];
+  a.ys_p = y->stride[1];
+  a.ys_r = y->stride[2];
+  a.gp = (const float *)gpow->data;
+  a.gs_c = gpow->stride[0];
+  a.gs_k = gpow->stride[1];
+  a.gs_r = gpow->stride[2];
+  a.gy = (float2 *)gy->data;
+  a.gys_c = gy->stride[0];
+  a.gys_p = gy->stride[1];
+  a.gys_r = gy->stride[2];
+  a.C = (int)C;
+  a.P = (int)P;
+  a.R = (int)R;
+  a.rows_t = (int)rows_t;
+  a.p_off = tpc_mode == MXR_CONV_VALID ? 1 : 0;
+  a.no_tpc = tpc_mode == MXR_TPC_NONE;
+  a.win_rows = (int)win_rows;
+  a.tw = twiddles_device((int)nd, false);
+  MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
+  switch (nd) {
+    case 2: return launch_doppler_adj<2>(a, stream);
+    case 4: return launch_doppler_adj<4>(a, stream);
+    case 8: return launch_doppler_adj<8>(a, stream);
+    case 16: return launch_doppler_adj<16>(a, stream);
+    case 32: return launch_doppler_adj<32>(a, stream);
+    case 64: return launch_doppler_adj<64>(a, stream);
+    case 128: return launch_doppler_adj<128>(a, stream);
+    case 256: return launch_doppler_adj<256>(a, stream);
+    case 512: return launch_doppler_adj<512>(a, stream);
+    case 1024: return launch_doppler_adj<1024>(a, stream);
+  }
+  return MXR_NOT_SUPPORTED;
+}
+
 }  // namespace mxr
+
+extern "C" mxr_status mxr_doppler_adjoint(const mxr_tensor *gy, const mxr_tensor *gpow,
+                                          const mxr_tensor *y, int tpc_mode, int64_t win_rows,
+                                          int64_t nd, hipStream_t stream) {
+  return mxr::guarded("mxr_doppler_adjoint", [&] {
+    return mxr::doppler_adjoint_impl(gy, gpow, y, tpc_mode, win_rows, nd, stream);
+  });
+}
 
 extern "C" mxr_status mxr_doppler(const mxr_tensor *pow_out, const mxr_tensor *rd_out,
                                   const mxr_tensor *y, int tpc_mode, int64_t win_rows, int64_t nd,

@@ -23,6 +23,7 @@
 
 #include "fft_engine.hpp"
 #include "mxr_cache.hpp"
+#include "pcgrad.hpp"
 #include "ragged.hpp"
 
 namespace mxr {
@@ -955,6 +956,55 @@ static mxr_status pc_plan_execute(mxr_pc_plan h, const mxr_tensor *y, const mxr_
   return dispatch_pc(p->plan.M, nullptr, &pa, s);
 }
 
+static mxr_status pc_plan_waveform_grad(mxr_pc_plan h, const mxr_tensor *gw, const mxr_tensor *gy,
+                                        const mxr_tensor *x, const mxr_tensor *w, hipStream_t s) {
+  const char *fn = "pc_plan_waveform_grad";
+  const PcPlanObj *p = reinterpret_cast<const PcPlanObj *>(h);
+  mxr_status st;
+  if ((st = plan_on_device(p, fn))) return st;
+  PcArgs pa{};
+  bool in_place = false;
+  if ((st = pc_rows(fn, gy, x, p->n, &pa, &in_place))) return st;
+  MXR_REQUIRE(!in_place, MXR_ALIAS_ERROR, "%s: gy overlaps x", fn);
+  MXR_REQUIRE(x->shape[x->rank - 1] == p->S && pa.nout == p->nout, MXR_INVALID_SIZE,
+              "%s: rows of %lld samples -> %lld lags, the plan has %lld -> %lld", fn,
+              (long long)x->shape[x->rank - 1], (long long)pa.nout, (long long)p->S,
+              (long long)p->nout);
+  if ((st = check_tensor(w, "waveform", MXR_C64, 1, 1))) return st;
+  if ((st = check_tensor(gw, "waveform gradient", MXR_C64, 1, 1))) return st;
+  MXR_REQUIRE(w->shape[0] == p->L && gw->shape[0] == p->L, MXR_INVALID_SIZE,
+              "%s: waveform and its gradient need %lld samples", fn, (long long)p->L);
+  for (const mxr_tensor *i : {x, gy, w})
+    MXR_REQUIRE(!views_overlap(*gw, *i), MXR_ALIAS_ERROR, "%s: gw overlaps an input", fn);
+  MXR_REQUIRE(pa.rows <= 0x7fffffffll, MXR_INVALID_SIZE, "%s: too many rows", fn);
+  WgradJob j{};
+  j.x = pa.x;
+  j.gy = pa.y;
+  j.w = (const float2 *)w->data;
+  j.gw = (float2 *)gw->data;
+  j.wstride = w->stride[0];
+  j.gwstride = gw->stride[0];
+  j.bi = pa.bi;
+  j.rows = pa.nout > 0 ? pa.rows : 0;
+  j.V = p->plan.V;
+  j.nblk = p->plan.nblk;
+  j.M = p->plan.M;
+  if (j.M > 8192 && p->L <= 8192) {
+    // two spectra and their running product stay in registers: at 16384
+    // points (1024 threads) they no longer fit, so use 8192-point blocks
+    int M2 = 0;
+    if ((st = pc_plan_blocks(pa.seff, p->L, p->n, pa.nout, 8192, &M2, &j.V, &j.nblk))) return st;
+    j.M = M2;
+  }
+  j.n = (int)p->n;
+  j.seff = (int)pa.seff;
+  j.nout = (int)pa.nout;
+  j.L = (int)p->L;
+  j.window = (p->flags & MXR_PC_WINDOW) != 0;
+  j.normalize = (p->flags & MXR_PC_NORMALIZE) != 0;
+  return pc_waveform_grad_run(j, s);
+}
+
 static mxr_status pc_plan_destroy(mxr_pc_plan h) {
   PcPlanObj *p = reinterpret_cast<PcPlanObj *>(h);
   if (!p) return MXR_SUCCESS;
@@ -1004,6 +1054,12 @@ mxr_status mxr_pc_plan_execute(mxr_pc_plan plan, const mxr_tensor *y, const mxr_
                                hipStream_t stream) {
   return mxr::guarded("mxr_pc_plan_execute",
                       [&] { return mxr::pc_plan_execute(plan, y, x, stream); });
+}
+
+mxr_status mxr_pc_plan_waveform_grad(mxr_pc_plan plan, const mxr_tensor *gw, const mxr_tensor *gy,
+                                     const mxr_tensor *x, const mxr_tensor *w, hipStream_t stream) {
+  return mxr::guarded("mxr_pc_plan_waveform_grad",
+                      [&] { return mxr::pc_plan_waveform_grad(plan, gw, gy, x, w, stream); });
 }
 
 mxr_status mxr_pc_plan_destroy(mxr_pc_plan plan) {

@@ -101,6 +101,16 @@ class PulseCompressionPlan:
                                               L_.stream_handle(stream)))
         return out
 
+    def waveform_grad(self, gy, x, w, out=None, stream=None):
+        """dL/dw of a real loss through y = self(x) with waveform w, summed
+        over every row (mxr_pc_plan_waveform_grad); gy = dL/dy."""
+        if out is None:
+            out = torch.empty((self.L,), dtype=torch.complex64, device=x.device)
+        L_.check(L_.lib().mxr_pc_plan_waveform_grad(self._h, L_.ref(L_.desc(out)),
+                                                    L_.ref(L_.desc(gy)), L_.ref(L_.desc(x)),
+                                                    L_.ref(L_.desc(w)), L_.stream_handle(stream)))
+        return out
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             L_.check(L_.lib().mxr_pc_plan_destroy(self._h))
@@ -129,6 +139,22 @@ def doppler(y, nd: int, tpc: str = "same", win_rows: int | None = None, rd=None,
     L.check(L.lib().mxr_doppler(L.ref(L.desc(pw)), L.ref(L.desc(rd)), L.ref(L.desc(y)), mode,
                                 int(win_rows), int(nd), L.stream_handle(stream)))
     return pw, rd
+
+
+def doppler_adjoint(gpow, y, nd: int, tpc: str = "same", win_rows: int | None = None, gy=None,
+                    stream=None):
+    """dL/dy of a real loss through pow = doppler(y, nd, tpc, win_rows)[0]
+    (mxr_doppler_adjoint); gpow = dL/dpow may be any strided view (an
+    expanded one included).  The Doppler spectrum is recomputed from y."""
+    C, P, R = y.shape
+    mode = {"same": L.SAME, "valid": L.VALID, "none": L.TPC_NONE}[tpc]
+    rows = P - 2 if mode == L.VALID else P
+    win_rows = (P - 2 if mode != L.TPC_NONE else rows) if win_rows is None else win_rows
+    if gy is None:
+        gy = torch.empty_like(y)
+    L.check(L.lib().mxr_doppler_adjoint(L.ref(L.desc(gy)), L.ref(L.desc(gpow)), L.ref(L.desc(y)),
+                                        mode, int(win_rows), int(nd), L.stream_handle(stream)))
+    return gy
 
 
 def cfar(inp, win=(5, 13), guard=(3, 3), pfa: float = 1e-5, dets=None, ba=None, pw=None,

@@ -77,3 +77,63 @@ def test_planted_adjoint_mistakes_fail(mistake):
     idx = np.arange(0, phi.size, 5)
     fd = og.central_fd(lambda p: og.loss(p, h, X, targets, n, nd), phi, idx)
     assert rel(g_phi.flat[idx], fd) > 1e-2
+
+
+# ------------------------------------------------------ radar-chain gradient
+def radar_problem(seed=0):
+    rng = np.random.default_rng(seed)
+    C, P, S, Lw, n, nd = 2, 8, 48, 12, 64, 8
+    R = S - Lw + 1
+    X = (rng.standard_normal((C, P, S)) + 1j * rng.standard_normal((C, P, S))) / np.sqrt(2)
+    phi = rng.uniform(0, 2 * np.pi, Lw)
+    wmap = rng.standard_normal((nd, R))
+    return phi, X, wmap, n, nd
+
+
+@pytest.mark.parametrize("tpc", ["valid", "same", "none"])
+def test_radar_gradient_matches_central_differences(tpc):
+    """dL/dphi (phase-only waveform) and dL/dw (free complex waveform, which
+    also exercises the normalisation term that phase-only waveforms cancel)
+    against FP64 central differences with h = 1e-3."""
+    phi, X, wmap, n, nd = radar_problem(1)
+    L, g_phi = og.radar_loss_and_grad(phi, X, wmap, n, nd, tpc)
+    assert np.isclose(L, og.radar_loss(phi, X, wmap, n, nd, tpc))
+    fd = og.central_fd(lambda p: og.radar_loss(p, X, wmap, n, nd, tpc), phi, range(phi.size))
+    assert rel(g_phi, fd) < 1e-5
+    rng = np.random.default_rng(2)
+    w = og.waveforms(phi) * (1 + 0.3 * rng.standard_normal(phi.size))
+    _, gw = og.radar_grad_w(w, X, wmap, n, nd, tpc)
+    Lw = w.size
+    fd = og.central_fd(lambda v: og.radar_loss_w(v[:Lw] + 1j * v[Lw:], X, wmap, n, nd, tpc),
+                       np.concatenate([w.real, w.imag]), range(2 * Lw))
+    assert rel(gw, fd[:Lw] + 1j * fd[Lw:]) < 1e-5
+
+
+@pytest.mark.parametrize("tpc,mistake", [("valid", "no_norm"), ("same", "no_norm"),
+                                         ("valid", "tpc_shift"), ("valid", "conj_corr"),
+                                         ("none", "conj_corr"), ("same", "window_all")])
+def test_radar_planted_mistakes_fail(tpc, mistake):
+    phi, X, wmap, n, nd = radar_problem(3)
+    rng = np.random.default_rng(4)
+    w = og.waveforms(phi) * (1 + 0.3 * rng.standard_normal(phi.size))
+    _, gw = og.radar_grad_w(w, X, wmap, n, nd, tpc, mistake=mistake)
+    Lw = w.size
+    fd = og.central_fd(lambda v: og.radar_loss_w(v[:Lw] + 1j * v[Lw:], X, wmap, n, nd, tpc),
+                       np.concatenate([w.real, w.imag]), range(2 * Lw))
+    assert rel(gw, fd[:Lw] + 1j * fd[Lw:]) > 1e-2
+
+
+def test_radar_gradient_is_additive_over_channels():
+    """The loss is a sum over channels, so the gradient of the whole cube is
+    the sum of the per-channel gradients (the size-independent check the GPU
+    path is held to at full size)."""
+    phi, X, wmap, n, nd = radar_problem(5)
+    _, g = og.radar_loss_and_grad(phi, X, wmap, n, nd)
+    parts = [og.radar_loss_and_grad(phi, X[c:c + 1], wmap, n, nd)[1] for c in range(X.shape[0])]
+    assert rel(sum(parts), g) < 1e-12
+
+
+def test_contrast_weights_sum():
+    wm = og.contrast_weights(8, 10, [(1, 2), (3, 4)], C=4)
+    assert np.isclose(wm.sum(), 0.0)                      # mean clutter - mean target of 1s
+    assert np.isclose(wm[1, 2], -1.0 / 8) and np.isclose(wm[0, 0], 1.0 / (4 * 78))

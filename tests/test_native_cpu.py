@@ -188,3 +188,29 @@ def test_header_matches_python_constants(L):
     text = open(os.path.join(ROOT, "include", "mxr.h")).read()
     assert "MXR_TPC_NONE 3" in text and L.TPC_NONE == 3
     assert "MXR_PC_WRITEBACK 4" in text and L.PC_WRITEBACK == 4
+
+
+def test_radar_gradient_checks(L):
+    lib = L.lib()
+    y = fake(L, [2, 8, 100], L.C64)
+    gy = fake(L, [2, 8, 100], L.C64, addr=0x900000)
+    gp4 = fake(L, [2, 4, 100], L.F32, addr=0xA00000)
+    r, msg = status(L, lib.mxr_doppler_adjoint, ctypes.byref(gy), ctypes.byref(gp4),
+                    ctypes.byref(y), 1, 6, 4, None)
+    assert r == 4 and "shorter" in msg              # 4-point FFT, 8 canceller rows
+    gp = fake(L, [2, 8, 100], L.F32, addr=0xA00000)
+    gy_bad = fake(L, [2, 8, 99], L.C64, addr=0x900000)
+    r, _ = status(L, lib.mxr_doppler_adjoint, ctypes.byref(gy_bad), ctypes.byref(gp),
+                  ctypes.byref(y), 1, 6, 8, None)
+    assert r == 4                                   # gy must have y's shape
+    gy_alias = fake(L, [2, 8, 100], L.C64)
+    r, _ = status(L, lib.mxr_doppler_adjoint, ctypes.byref(gy_alias), ctypes.byref(gp),
+                  ctypes.byref(y), 1, 6, 8, None)
+    assert r == 8
+    r, _ = status(L, lib.mxr_doppler_adjoint, ctypes.byref(gy), ctypes.byref(gp),
+                  ctypes.byref(y), 7, 6, 8, None)
+    assert r == 2                                   # unknown canceller mode
+    gw = fake(L, [10], L.C64, addr=0xB00000)
+    r, _ = status(L, lib.mxr_pc_plan_waveform_grad, None, ctypes.byref(gw), ctypes.byref(gy),
+                  ctypes.byref(y), ctypes.byref(gw), None)
+    assert r == 2                                   # null plan
