@@ -620,6 +620,11 @@ def bench_pipeline(args, cfg):
         cpu, parity = cpu_baseline_leg(x[0].cpu().numpy(), w.cpu().numpy(),
                                        pipe.pw[0].cpu().numpy(), dets[0].cpu().numpy(), nd,
                                        args.cpu_seconds, C, cfar_mask(win, guard))
+    fwd_bwd = None
+    if args.workload == "large":
+        del pipe
+        torch.cuda.empty_cache()
+        fwd_bwd = bench_large_grad(args, cfg, x, w, targets)
 
     out = {
         "metric": cfg["metric"],
@@ -643,11 +648,114 @@ def bench_pipeline(args, cfg):
                    "per_kernel_us": {k: round(v["avg_us"], 2) for k, v in kern.items() if v["launches"]},
                    "detections": det_cells, "targets_detected": f"{hit}/{len(targets)}",
                    "kernel_rooflines": {k: v for k, v in rooflines.items() if k != dominant},
-                   "parity": parity},
+                   "parity": parity,
+                   **({"fwd_bwd": fwd_bwd} if fwd_bwd else {})},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def bench_large_grad(args, cfg, x, w, targets):
+    """Forward + backward of the large cube with respect to the waveform
+    phases: pow = doppler(pulse_compress(x, exp(i phi)), 512) (valid
+    canceller), L = mean clutter power - mean target power over the
+    injected target cells.  The backward recomputes the Doppler spectrum
+    from the compressed cube (the only activation kept) and sums the
+    waveform gradient over all 32768 pulses.  The FP64 oracle gradient of
+    channel 0 is compared with the GPU gradient of that channel."""
+    import numpy as np
+    import torch
+    import matx_amd
+    from matx_amd import _lib
+    from matx_amd import autograd as ag
+
+    C, P, S, L, nd = cfg["C"], cfg["P"], cfg["S"], cfg["L"], cfg["nd"]
+    R = S - L + 1
+    n = 1
+    while n < S:
+        n *= 2
+    cells = [(dop, rb) for rb, dop, _ in targets]
+    wmap = ag.contrast_weights(nd, R, cells, C, device=x.device)
+    phi0 = torch.angle(w).to(torch.float32)
+    plan = matx_amd.ops.PulseCompressionPlan(S, L, n)
+
+    def step(xc=x, p0=phi0):
+        p = p0.clone().requires_grad_()
+        wv = torch.polar(torch.ones_like(p), p)
+        loss = ag.weighted_power_sum(ag.radar_power(wv, xc, plan, nd, tpc="valid"), wmap)
+        loss.backward()
+        return loss.detach(), p.grad
+
+    def fwd_only():
+        with torch.no_grad():
+            wv = torch.polar(torch.ones_like(phi0), phi0)
+            return ag.weighted_power_sum(ag.radar_power(wv, x, plan, nd, tpc="valid"), wmap)
+
+    def timed(fn, k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / k * 1000.0
+
+    k = max(1, min(args.steps, 5))
+    print("large fwd+bwd: warm-up", file=sys.stderr, flush=True)
+    step()
+    fwd_only()
+    lib = _lib.lib()
+    prof = not args.no_profile
+    if prof:
+        _lib.check(lib.mxr_profile_reset())
+        _lib.check(lib.mxr_profile_enable(1))
+    ms_fb = timed(step, k)
+    if prof:
+        _lib.check(lib.mxr_profile_enable(0))
+    ms_f = timed(fwd_only, k)
+    kern = {}
+    if prof:
+        for name in ["pc_ols_kernel", "doppler_kernel", "doppler_adj_kernel", "pc_wgrad_kernel",
+                     "wgrad_reduce_kernel", "wgrad_finish_kernel", "wfspec_kernel"]:
+            cnt, tot = _lib.profile_query(name)
+            if cnt:
+                kern[name] = round(1000.0 * tot / cnt, 1)
+    # algorithmic bytes of one forward + backward: forward reads x, writes
+    # and reads the compressed cube, writes and reads the power map; the
+    # backward reads the compressed cube, writes the lag gradient, then reads
+    # x and the lag gradient (the weight map is 1/C of a power map, resident)
+    B = 2 * 8 * C * P * S + 5 * 8 * C * P * R + 2 * 4 * C * nd * R
+    out = {"grad_steps_per_s": round(1000.0 / ms_fb, 3), "ms_fwd_bwd": round(ms_fb, 2),
+           "ms_fwd": round(ms_f, 2), "fwd_bwd_over_fwd": round(ms_fb / ms_f, 3),
+           "algorithmic_bytes_fwd_bwd": B,
+           "hbm_gbs_fwd_bwd": round(B / (ms_fb * 1e-3) / 1e9, 1),
+           "hbm_frac_fwd_bwd": round(B / (ms_fb * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+           "per_kernel_us": kern, "steps": k,
+           "loss": "mean clutter power - mean target power over the injected target cells",
+           "checkpointing": "Doppler spectrum recomputed in the backward; saved: compressed cube"}
+    # size-independent checks on the GPU: determinism, and the gradient of
+    # the cube equals the sum of its channel-block gradients
+    print(f"large fwd+bwd: {ms_fb:.1f} ms/step; checks", file=sys.stderr, flush=True)
+    _, g_all = step()
+    _, g_again = step()
+    blocks = [(c0, min(c0 + 16, C)) for c0 in range(0, C, 16)]
+    g_sum = sum(step(x[a:b])[1].double() for a, b in blocks)
+    out["deterministic"] = bool(torch.equal(g_all, g_again))
+    out["channel_block_sum_rel"] = float((g_all.double() - g_sum).norm() / g_sum.norm())
+    if args.cpu_seconds > 0:
+        sys.path.insert(0, ROOT)
+        from oracle import grad as og
+        print("large fwd+bwd: FP64 oracle on channel 0", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        _, g0 = step(x[0:1])
+        L64, g64 = og.radar_loss_and_grad(phi0.double().cpu().numpy(), x[0:1].cpu().numpy(),
+                                          wmap.double().cpu().numpy(), n, nd, "valid")
+        g0 = g0.double().cpu().numpy()
+        out["fp64_oracle"] = {"channel": 0,
+                              "grad_rel_l2": float(np.linalg.norm(g0 - g64) / np.linalg.norm(g64)),
+                              "oracle_s": round(time.perf_counter() - t0, 1)}
+    plan.close()
+    return out
 
 
 if __name__ == "__main__":

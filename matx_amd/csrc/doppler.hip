@@ -25,6 +25,34 @@ struct DopCfg {
   static constexpr int TR = ND <= 256 ? 32 : 8192 / ND;
 };
 
+// Coalesced load of a pulse tile: rows p < P of TR range bins from r0 into
+// lds[p * TR + t] (zero past R), 16 loads in flight per lane.
+template <int ND>
+__device__ __forceinline__ void load_pulse_tile(const float2 *yc, int64_t ys_p, int64_t ys_r,
+                                                int P, int R, int r0, float2 *lds) {
+  constexpr int NT = DopCfg<ND>::NT;
+  constexpr int TR = DopCfg<ND>::TR;
+  constexpr int RPI = NT / TR;  // pulses covered by one sweep of the workgroup
+  constexpr int BATCH = 16;
+  const int t = threadIdx.x % TR, p0 = threadIdx.x / TR;
+  const int r = r0 + t;
+  const bool rok = r < R;
+  const float2 *col = yc + (int64_t)r * ys_r;
+  for (int pb = 0; pb < P; pb += RPI * BATCH) {
+    float2 v[BATCH];
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u) {
+      const int p = pb + p0 + u * RPI;
+      v[u] = (rok && p < P) ? col[(int64_t)p * ys_p] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < BATCH; ++u) {
+      const int p = pb + p0 + u * RPI;
+      if (p < P) lds[p * TR + t] = v[u];
+    }
+  }
+}
+
 // One tile: TR range bins from r0 of channel c.  lds holds rows_buf * TR
 // elements, win ND floats.  Pre: the workgroup's previous use of both is
 // finished.
@@ -42,29 +70,7 @@ __device__ __forceinline__ void doppler_tile(const DopArgs &a, int c, int r0, in
       h = a.taps ? a.taps[(int64_t)i * a.taps_s] : (i < a.win_rows ? hamming_f32(i, a.win_rows) : 1.0f);
     win[i] = h;
   }
-  // coalesced tile load: rows of TR range bins, 16 loads in flight per lane
-  const float2 *yc = a.y + (int64_t)c * a.ys_c;
-  {
-    constexpr int RPI = NT / TR;  // pulses covered by one sweep of the workgroup
-    constexpr int BATCH = 16;
-    const int t = threadIdx.x % TR, p0 = threadIdx.x / TR;
-    const int r = r0 + t;
-    const bool rok = r < a.R;
-    const float2 *col = yc + (int64_t)r * a.ys_r;
-    for (int pb = 0; pb < a.P; pb += RPI * BATCH) {
-      float2 v[BATCH];
-#pragma unroll
-      for (int u = 0; u < BATCH; ++u) {
-        const int p = pb + p0 + u * RPI;
-        v[u] = (rok && p < a.P) ? col[(int64_t)p * a.ys_p] : make_float2(0.f, 0.f);
-      }
-#pragma unroll
-      for (int u = 0; u < BATCH; ++u) {
-        const int p = pb + p0 + u * RPI;
-        if (p < a.P) lds[p * TR + t] = v[u];
-      }
-    }
-  }
+  load_pulse_tile<ND>(a.y + (int64_t)c * a.ys_c, a.ys_p, a.ys_r, a.P, a.R, r0, lds);
   wg_barrier();
 
   auto tpc = [&](int t, int i) -> float2 {
@@ -297,79 +303,6 @@ mxr_status doppler_impl(const mxr_tensor *pow_out, const mxr_tensor *rd_out, con
   return doppler_run(pow_out, rd_out, y, tpc_mode, win_rows, nullptr, nd, stream);
 }
 
-// ------------------------------------------------------- window adjoint
-// Backward of pow = |FFT_nd(h .* x)|^2 along pulses, for a real window h:
-//   gD = 2 gpow D                      (torch convention g = dL/dRe + i dL/dIm)
-//   gu = F^H gD, rows p < P            (unnormalised inverse DFT)
-//   gx[c,p,r] = h[p] gu[c,p,r]
-//   gh[p] = sum_{c,r} Re(conj(gu[c,p,r]) x[c,p,r])
-// D is recomputed from x and h in the tile (no stored spectrum: the backward
-// costs one more read of x instead of a C x nd x R complex buffer).  A
-// workgroup reduces its tile's gh contributions over the range bins; a
-// second kernel sums the per-tile partials in a fixed order.
-struct DopAdjArgs {
-  const float2 *x;
-  int64_t xs_c, xs_p, xs_r;
-  const float *taps;
-  int64_t taps_s;
-  const float *gp;
-  int64_t gs_c, gs_k, gs_r;
-  float2 *gx;
-  int64_t gxs_c, gxs_p, gxs_r;
-  float *partial;  // [C * tiles][P]
-  int C, P, R, tiles;
-  const float2 *tw;
-};
-
-template <int ND>
-__global__ __launch_bounds__(256) void doppler_taps_adj_kernel(DopAdjArgs a) {
-  constexpr int NT = 256;
-  constexpr int TR = DopCfg<ND>::TR;
-  static_assert(TR <= 64 && 64 % TR == 0, "range tile must divide a wave");
-  using LY = Layout<ND, TR, true>;
-  using SQ = Seq<ND, false>;
-  constexpr int K = SQ::n;
-  extern __shared__ float2 lds[];  // ND * TR
-  __shared__ float hw[ND];
-  const int c = blockIdx.y;
-  const int r0 = blockIdx.x * TR;
-  for (int i = threadIdx.x; i < ND; i += NT) hw[i] = i < a.P ? a.taps[(int64_t)i * a.taps_s] : 0.f;
-  __syncthreads();
-  const float2 *xc = a.x + (int64_t)c * a.xs_c;
-  for (int idx = threadIdx.x; idx < ND * TR; idx += NT) {
-    const int i = idx / TR, t = idx % TR, r = r0 + t;
-    float2 v = make_float2(0.f, 0.f);
-    if (i < a.P && r < a.R) v = cscale(xc[(int64_t)i * a.xs_p + (int64_t)r * a.xs_r], hw[i]);
-    lds[LY::addr(t, i)] = v;
-  }
-  wg_barrier();
-  lds_passes<LY, SQ, ND, NT, TR, false, 0, K>(lds, a.tw);  // D
-  const float *gpc = a.gp + (int64_t)c * a.gs_c;
-  for (int idx = threadIdx.x; idx < ND * TR; idx += NT) {
-    const int k = idx / TR, t = idx % TR, r = r0 + t;
-    const float g = r < a.R ? gpc[(int64_t)k * a.gs_k + (int64_t)r * a.gs_r] : 0.f;
-    lds[LY::addr(t, k)] = cscale(lds[LY::addr(t, k)], 2.0f * g);
-  }
-  wg_barrier();
-  lds_passes<LY, SQ, ND, NT, TR, true, 0, K>(lds, a.tw);  // gu (unnormalised inverse)
-  float *part = a.partial + ((int64_t)c * a.tiles + blockIdx.x) * a.P;
-  for (int base = 0; base < a.P * TR; base += NT) {
-    const int idx = base + threadIdx.x;
-    const int p = idx / TR, t = idx % TR, r = r0 + t;
-    float contrib = 0.f;
-    if (p < a.P && r < a.R) {
-      const float2 gu = lds[LY::addr(t, p)];
-      const float2 xv = xc[(int64_t)p * a.xs_p + (int64_t)r * a.xs_r];
-      contrib = gu.x * xv.x + gu.y * xv.y;
-      if (a.gx)
-        a.gx[(int64_t)c * a.gxs_c + (int64_t)p * a.gxs_p + (int64_t)r * a.gxs_r] = cscale(gu, hw[p]);
-    }
-#pragma unroll
-    for (int o = TR / 2; o > 0; o >>= 1) contrib += __shfl_xor(contrib, o);
-    if (t == 0 && p < a.P) part[p] = contrib;
-  }
-}
-
 // gh[p] = sum over parts of partial[part][p]: one workgroup per tap, a
 // fixed-order tree in double (deterministic).
 __global__ __launch_bounds__(256) void taps_reduce_kernel(const float *partial, int nparts, int P,
@@ -387,130 +320,33 @@ __global__ __launch_bounds__(256) void taps_reduce_kernel(const float *partial, 
   if (threadIdx.x == 0) gh[(int64_t)p * ghs] = (float)acc[0];
 }
 
-template <int ND>
-static mxr_status launch_taps_adj(const DopAdjArgs &a, const mxr_tensor *gtaps, hipStream_t s) {
-  constexpr int TR = DopCfg<ND>::TR;
-  const size_t lds = (size_t)ND * TR * sizeof(float2);
-  MXR_REQUIRE(lds + ND * sizeof(float) <= (size_t)max_lds_per_block(), MXR_NOT_SUPPORTED,
-              "doppler_taps_adjoint: tile does not fit in LDS");
-  if (lds + ND * sizeof(float) > 65536)
-    MXR_HIP_TRY(hipFuncSetAttribute((const void *)doppler_taps_adj_kernel<ND>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  {
-    ProfScope ps("doppler_taps_adj_kernel", s);
-    doppler_taps_adj_kernel<ND><<<dim3(a.tiles, a.C), dim3(256), lds, s>>>(a);
-  }
-  MXR_HIP_TRY(hipGetLastError());
-  if (gtaps) {
-    ProfScope ps("taps_reduce_kernel", s);
-    taps_reduce_kernel<<<dim3(a.P), dim3(256), 0, s>>>(
-        a.partial, a.C * a.tiles, a.P, (float *)gtaps->data, gtaps->stride[0]);
-    MXR_HIP_TRY(hipGetLastError());
-  }
-  return MXR_SUCCESS;
-}
-
-static mxr_status doppler_taps_adjoint_impl(const mxr_tensor *gtaps, const mxr_tensor *gx,
-                                            const mxr_tensor *gpow, const mxr_tensor *x,
-                                            const mxr_tensor *taps, int64_t nd, hipStream_t s) {
-  const char *fn = "doppler_taps_adjoint";
-  mxr_status st;
-  if ((st = check_tensor(x, "x", MXR_C64, 3, 3))) return st;
-  if ((st = check_tensor(taps, "taps", MXR_F32, 1, 1))) return st;
-  if ((st = check_tensor(gpow, "gpow", MXR_F32, 3, 3))) return st;
-  MXR_REQUIRE(gtaps || gx, MXR_INVALID_PARAMETER, "%s: no gradient requested", fn);
-  const int64_t C = x->shape[0], P = x->shape[1], R = x->shape[2];
-  MXR_REQUIRE(taps->shape[0] == P, MXR_INVALID_SIZE, "%s: %lld taps for %lld pulses", fn,
-              (long long)taps->shape[0], (long long)P);
-  MXR_REQUIRE(nd >= 2 && nd <= 1024 && (nd & (nd - 1)) == 0 && nd >= P, MXR_NOT_SUPPORTED,
-              "%s: FFT size %lld must be a power of two in [max(2, P), 1024]", fn, (long long)nd);
-  MXR_REQUIRE(gpow->shape[0] == C && gpow->shape[1] == nd && gpow->shape[2] == R,
-              MXR_INVALID_SIZE, "%s: gpow must be [%lld, %lld, %lld]", fn, (long long)C,
-              (long long)nd, (long long)R);
-  if (gtaps) {
-    if ((st = check_tensor(gtaps, "gtaps", MXR_F32, 1, 1))) return st;
-    MXR_REQUIRE(gtaps->shape[0] == P, MXR_INVALID_SIZE, "%s: gtaps must have %lld entries", fn,
-                (long long)P);
-  }
-  if (gx) {
-    if ((st = check_tensor(gx, "gx", MXR_C64, 3, 3))) return st;
-    MXR_REQUIRE(gx->shape[0] == C && gx->shape[1] == P && gx->shape[2] == R, MXR_INVALID_SIZE,
-                "%s: gx must have x's shape", fn);
-  }
-  for (const mxr_tensor *o : {gtaps, gx}) {
-    if (!o) continue;
-    for (const mxr_tensor *i : {x, taps, gpow})
-      MXR_REQUIRE(!views_overlap(*o, *i), MXR_ALIAS_ERROR, "%s: an output overlaps an input", fn);
-  }
-  if (gtaps && gx) MXR_REQUIRE(!views_overlap(*gtaps, *gx), MXR_ALIAS_ERROR, "%s: outputs overlap", fn);
-  if (C == 0 || R == 0 || P == 0) {
-    return MXR_SUCCESS;
-  }
-  MXR_REQUIRE(C <= 65535 && R < (1ll << 30), MXR_INVALID_SIZE, "%s: shape too large", fn);
-  DopAdjArgs a{};
-  a.x = (const float2 *)x->data;
-  a.xs_c = x->stride[0];
-  a.xs_p = x->stride[1];
-  a.xs_r = x->stride[2];
-  a.taps = (const float *)taps->data;
-  a.taps_s = taps->stride[0];
-  a.gp = (const float *)gpow->data;
-  a.gs_c = gpow->stride[0];
-  a.gs_k = gpow->stride[1];
-  a.gs_r = gpow->stride[2];
-  if (gx) {
-    a.gx = (float2 *)gx->data;
-    a.gxs_c = gx->stride[0];
-    a.gxs_p = gx->stride[1];
-    a.gxs_r = gx->stride[2];
-  }
-  a.C = (int)C;
-  a.P = (int)P;
-  a.R = (int)R;
-  a.tw = twiddles_device((int)nd, false);
-  MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
-  switch (nd) {
-#define MXR_TA(N)                                                                              \
-  case N: {                                                                                    \
-    a.tiles = (int)((R + DopCfg<N>::TR - 1) / DopCfg<N>::TR);                                  \
-    a.partial = (float *)stream_workspace("doppler_taps_partial",                              \
-                                          sizeof(float) * (size_t)C * a.tiles * P, s);         \
-    MXR_REQUIRE(a.partial != nullptr, MXR_OUT_OF_MEMORY, "%s: workspace allocation failed", fn); \
-    return launch_taps_adj<N>(a, gtaps, s);                                                    \
-  }
-    MXR_TA(2)
-    MXR_TA(4)
-    MXR_TA(8)
-    MXR_TA(16)
-    MXR_TA(32)
-    MXR_TA(64)
-    MXR_TA(128)
-    MXR_TA(256)
-    MXR_TA(512)
-    MXR_TA(1024)
-#undef MXR_TA
-  }
-  return MXR_NOT_SUPPORTED;
-}
-
-// ------------------------------------------------ pulse-axis adjoint
+// ------------------------------------------------ pulse-axis adjoints
 // Backward of mxr_doppler's power output with respect to its input y:
 //   t_i = h_i (y[i+o-1] - 2 y[i+o] + y[i+o+1])   i < rows_t   (o = p_off)
 //   D = FFT_nd(t),  pow = |D|^2
 //   gD = 2 gpow D;  gu = F^H gD (unnormalised inverse DFT);  s_i = h_i gu_i
 //   gy[p] = s[p-o+1] - 2 s[p-o] + s[p-o-1]   (s = 0 outside [0, rows_t))
 // (no canceller: t_i = h_i y[i], gy[p] = s_p).  The canceller coefficients
-// are real, so its adjoint is the same symmetric stencil.  D is recomputed
-// from y inside the tile (checkpointing: no nd x R complex spectrum is kept
-// between the passes), so a tile reads y and gpow once and writes gy once.
+// are real, so its adjoint is the same symmetric stencil.  With free taps h
+// (mxr_doppler_taps, no canceller) the same kernel also gives the tap
+// gradient gh[p] = sum_{c,r} Re(conj(gu[c,p,r]) y[c,p,r]): each tile reduces
+// its range bins into one partial row, and taps_reduce_kernel sums the rows
+// in a fixed order.  D is recomputed from y inside the tile (checkpointing:
+// no nd x R complex spectrum is kept between the passes), so a tile reads y
+// and gpow once and writes gy once.  A tile whose gpow is all zero (a loss on
+// a few cells leaves most of it zero) contributes nothing: it writes zeros
+// and skips the transforms.
 struct DopGradArgs {
   const float2 *y;
   int64_t ys_c, ys_p, ys_r;
   const float *gp;
   int64_t gs_c, gs_k, gs_r;
-  float2 *gy;
+  float2 *gy;  // may be null (tap gradient only)
   int64_t gys_c, gys_p, gys_r;
-  int C, P, R;
+  const float *taps;  // null: Hamming(win_rows) then 1
+  int64_t taps_s;
+  float *partial;  // [C * tiles][P] tap-gradient rows, or null
+  int C, P, R, tiles;
   int rows_t, p_off, no_tpc, win_rows;
   const float2 *tw;
 };
@@ -519,6 +355,7 @@ template <int ND>
 __global__ __launch_bounds__(256) void doppler_adj_kernel(DopGradArgs a) {
   constexpr int NT = DopCfg<ND>::NT;
   constexpr int TR = DopCfg<ND>::TR;
+  static_assert(TR <= 64 && 64 % TR == 0, "range tile must divide a wave");
   using LY = Layout<ND, TR, true>;
   using SQ = Seq<ND, false>;
   constexpr int K = SQ::n;
@@ -526,82 +363,154 @@ __global__ __launch_bounds__(256) void doppler_adj_kernel(DopGradArgs a) {
   __shared__ float win[ND];
   const int c = blockIdx.y;
   const int r0 = blockIdx.x * TR;
-  for (int i = threadIdx.x; i < ND; i += NT)
-    win[i] = i < a.rows_t ? (i < a.win_rows ? hamming_f32(i, a.win_rows) : 1.0f) : 0.0f;
-  const float2 *yc = a.y + (int64_t)c * a.ys_c;
-  for (int idx = threadIdx.x; idx < a.P * TR; idx += NT) {
-    const int p = idx / TR, t = idx % TR, r = r0 + t;
-    lds[p * TR + t] = r < a.R ? yc[(int64_t)p * a.ys_p + (int64_t)r * a.ys_r] : make_float2(0.f, 0.f);
-  }
-  wg_barrier();
-
-  // forward: first pass fed by the windowed canceller, the rest in LDS
-  constexpr int R0 = SQ::template R<0>;
-  {
-    Pass<LY, R0, ND, NT, TR> p0;
-    p0.load([&](int t, int i) -> float2 {
-      if (i >= a.rows_t) return make_float2(0.f, 0.f);
-      const int p = i + a.p_off;
-      const float2 m = lds[p * TR + t];
-      if (a.no_tpc) return cscale(m, win[i]);
-      const float2 s = p >= 1 ? lds[(p - 1) * TR + t] : make_float2(0.f, 0.f);
-      const float2 u = p + 1 < a.P ? lds[(p + 1) * TR + t] : make_float2(0.f, 0.f);
-      return cscale(make_float2(s.x + u.x - 2.0f * m.x, s.y + u.y - 2.0f * m.y), win[i]);
-    });
-    wg_barrier();  // every canceller read is done before the tile is overwritten
-    p0.template compute<1, false>(nullptr);
-    p0.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
-  }
-  wg_barrier();
-  lds_passes<LY, SQ, ND, NT, TR, false, 1, K>(lds, a.tw);
-
-  // gD = 2 gpow D
+  float *part = a.partial ? a.partial + ((int64_t)c * a.tiles + blockIdx.x) * a.P : nullptr;
+  float2 *gyc = a.gy ? a.gy + (int64_t)c * a.gys_c : nullptr;
   const float *gpc = a.gp + (int64_t)c * a.gs_c;
-  for (int idx = threadIdx.x; idx < ND * TR; idx += NT) {
-    const int k = idx / TR, t = idx % TR, r = r0 + t;
+  {
+    int nz = 0;
+    for (int idx = threadIdx.x; idx < ND * TR; idx += NT) {
+      const int k = idx / TR, r = r0 + idx % TR;
+      if (r < a.R && gpc[(int64_t)k * a.gs_k + (int64_t)r * a.gs_r] != 0.f) nz = 1;
+    }
+    if (!__syncthreads_or(nz)) {
+      if (gyc)
+        for (int idx = threadIdx.x; idx < a.P * TR; idx += NT) {
+          const int p = idx / TR, r = r0 + idx % TR;
+          if (r < a.R) gyc[(int64_t)p * a.gys_p + (int64_t)r * a.gys_r] = make_float2(0.f, 0.f);
+        }
+      if (part)
+        for (int p = threadIdx.x; p < a.P; p += NT) part[p] = 0.f;
+      return;
+    }
+  }
+  for (int i = threadIdx.x; i < ND; i += NT) {
+    float h = 0.f;
+    if (i < a.rows_t)
+      h = a.taps ? a.taps[(int64_t)i * a.taps_s] : (i < a.win_rows ? hamming_f32(i, a.win_rows) : 1.0f);
+    win[i] = h;
+  }
+  load_pulse_tile<ND>(a.y + (int64_t)c * a.ys_c, a.ys_p, a.ys_r, a.P, a.R, r0, lds);
+  wg_barrier();
+
+  // forward: first pass fed by the windowed canceller, the rest in LDS; the
+  // last forward pass stores gD = 2 gpow D
+  auto store_gd = [&](int t, int k, float2 v) {
+    const int r = r0 + t;
     const float g = r < a.R ? gpc[(int64_t)k * a.gs_k + (int64_t)r * a.gs_r] : 0.f;
-    lds[LY::addr(t, k)] = cscale(lds[LY::addr(t, k)], 2.0f * g);
+    lds[LY::addr(t, k)] = cscale(v, 2.0f * g);
+  };
+  constexpr int R0 = SQ::template R<0>;
+  Pass<LY, R0, ND, NT, TR> p0;
+  p0.load([&](int t, int i) -> float2 {
+    if (i >= a.rows_t) return make_float2(0.f, 0.f);
+    const int p = i + a.p_off;
+    const float2 m = lds[p * TR + t];
+    if (a.no_tpc) return cscale(m, win[i]);
+    const float2 s = p >= 1 ? lds[(p - 1) * TR + t] : make_float2(0.f, 0.f);
+    const float2 u = p + 1 < a.P ? lds[(p + 1) * TR + t] : make_float2(0.f, 0.f);
+    return cscale(make_float2(s.x + u.x - 2.0f * m.x, s.y + u.y - 2.0f * m.y), win[i]);
+  });
+  wg_barrier();  // every canceller read is done before the tile is overwritten
+  p0.template compute<1, false>(nullptr);
+  if constexpr (K == 1) {
+    p0.template store<1>(store_gd);
+  } else {
+    p0.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
+    wg_barrier();
+    lds_passes<LY, SQ, ND, NT, TR, false, 1, K - 1>(lds, a.tw);
+    constexpr int RL = SQ::template R<K - 1>;
+    constexpr int LSL = SQ::template LS<K - 1>;
+    Pass<LY, RL, ND, NT, TR> pl;
+    pl.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
+    wg_barrier();
+    pl.template compute<LSL, false>(a.tw + SQ::template TWOFF<K - 1>);
+    pl.template store<LSL>(store_gd);
   }
   wg_barrier();
   // gu = F^H gD: the same passes with conjugate twiddles, unscaled
   lds_passes<LY, SQ, ND, NT, TR, true, 0, K>(lds, a.tw);
 
-  // window, canceller transpose, coalesced store
-  float2 *gyc = a.gy + (int64_t)c * a.gys_c;
+  // window, canceller transpose, coalesced store; tap-gradient partials
   auto s_at = [&](int t, int i) -> float2 {
     return (i >= 0 && i < a.rows_t) ? cscale(lds[LY::addr(t, i)], win[i]) : make_float2(0.f, 0.f);
   };
-  for (int idx = threadIdx.x; idx < a.P * TR; idx += NT) {
+  const float2 *yc = a.y + (int64_t)c * a.ys_c;
+  for (int base = 0; base < a.P * TR; base += NT) {  // uniform trip count (shuffles)
+    const int idx = base + threadIdx.x;
     const int p = idx / TR, t = idx % TR, r = r0 + t;
-    if (r >= a.R) continue;
-    const int i = p - a.p_off;
-    float2 g;
-    if (a.no_tpc) {
-      g = s_at(t, i);
-    } else {
-      const float2 lo = s_at(t, i - 1), mid = s_at(t, i), hi = s_at(t, i + 1);
-      g = make_float2(lo.x + hi.x - 2.0f * mid.x, lo.y + hi.y - 2.0f * mid.y);
+    const bool ok = p < a.P && r < a.R;
+    float contrib = 0.f;
+    if (ok && gyc) {
+      const int i = p - a.p_off;
+      float2 g;
+      if (a.no_tpc) {
+        g = s_at(t, i);
+      } else {
+        const float2 lo = s_at(t, i - 1), mid = s_at(t, i), hi = s_at(t, i + 1);
+        g = make_float2(lo.x + hi.x - 2.0f * mid.x, lo.y + hi.y - 2.0f * mid.y);
+      }
+      gyc[(int64_t)p * a.gys_p + (int64_t)r * a.gys_r] = g;
     }
-    gyc[(int64_t)p * a.gys_p + (int64_t)r * a.gys_r] = g;
+    if (part) {  // no canceller here: row p of gu pairs with pulse p
+      if (ok) {
+        const float2 gu = lds[LY::addr(t, p)];
+        const float2 yv = yc[(int64_t)p * a.ys_p + (int64_t)r * a.ys_r];
+        contrib = gu.x * yv.x + gu.y * yv.y;
+      }
+#pragma unroll
+      for (int o = TR / 2; o > 0; o >>= 1) contrib += __shfl_xor(contrib, o);
+      if (t == 0 && p < a.P) part[p] = contrib;
+    }
   }
 }
 
 template <int ND>
-static mxr_status launch_doppler_adj(const DopGradArgs &a, hipStream_t s) {
+static mxr_status launch_doppler_adj(DopGradArgs a, const mxr_tensor *gtaps, hipStream_t s) {
   constexpr int TR = DopCfg<ND>::TR;
   const int rows_buf = a.P > ND ? a.P : ND;
   const size_t lds = (size_t)rows_buf * TR * sizeof(float2);
   MXR_REQUIRE(lds + ND * sizeof(float) <= (size_t)max_lds_per_block(), MXR_NOT_SUPPORTED,
-              "doppler_adjoint: %d pulses x %d range bins do not fit in LDS", rows_buf, TR);
+              "doppler adjoint: %d pulses x %d range bins do not fit in LDS", rows_buf, TR);
   if (lds + ND * sizeof(float) > 65536)
     MXR_HIP_TRY(hipFuncSetAttribute((const void *)doppler_adj_kernel<ND>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  a.tiles = (a.R + TR - 1) / TR;
+  a.partial = nullptr;
+  if (gtaps) {
+    a.partial = (float *)stream_workspace("doppler_taps_partial",
+                                          sizeof(float) * (size_t)a.C * a.tiles * a.P, s);
+    MXR_REQUIRE(a.partial != nullptr, MXR_OUT_OF_MEMORY, "doppler adjoint: workspace allocation failed");
+  }
   {
     ProfScope ps("doppler_adj_kernel", s);
-    doppler_adj_kernel<ND><<<dim3((a.R + TR - 1) / TR, a.C), dim3(DopCfg<ND>::NT), lds, s>>>(a);
+    doppler_adj_kernel<ND><<<dim3(a.tiles, a.C), dim3(DopCfg<ND>::NT), lds, s>>>(a);
   }
   MXR_HIP_TRY(hipGetLastError());
+  if (gtaps) {
+    ProfScope ps("taps_reduce_kernel", s);
+    taps_reduce_kernel<<<dim3(a.P), dim3(256), 0, s>>>(a.partial, a.C * a.tiles, a.P,
+                                                       (float *)gtaps->data, gtaps->stride[0]);
+    MXR_HIP_TRY(hipGetLastError());
+  }
   return MXR_SUCCESS;
+}
+
+static mxr_status dispatch_doppler_adj(int64_t nd, const DopGradArgs &a, const mxr_tensor *gtaps,
+                                       hipStream_t s) {
+  switch (nd) {
+    case 2: return launch_doppler_adj<2>(a, gtaps, s);
+    case 4: return launch_doppler_adj<4>(a, gtaps, s);
+    case 8: return launch_doppler_adj<8>(a, gtaps, s);
+    case 16: return launch_doppler_adj<16>(a, gtaps, s);
+    case 32: return launch_doppler_adj<32>(a, gtaps, s);
+    case 64: return launch_doppler_adj<64>(a, gtaps, s);
+    case 128: return launch_doppler_adj<128>(a, gtaps, s);
+    case 256: return launch_doppler_adj<256>(a, gtaps, s);
+    case 512: return launch_doppler_adj<512>(a, gtaps, s);
+    case 1024: return launch_doppler_adj<1024>(a, gtaps, s);
+  }
+  set_error(MXR_NOT_SUPPORTED, "doppler adjoint: FFT size %lld unsupported", (long long)nd);
+  return MXR_NOT_SUPPORTED;
 }
 
 static mxr_status doppler_adjoint_impl(const mxr_tensor *gy, const mxr_tensor *gpow,
@@ -656,19 +565,71 @@ static mxr_status doppler_adjoint_impl(const mxr_tensor *gy, const mxr_tensor *g
   a.win_rows = (int)win_rows;
   a.tw = twiddles_device((int)nd, false);
   MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
-  switch (nd) {
-    case 2: return launch_doppler_adj<2>(a, stream);
-    case 4: return launch_doppler_adj<4>(a, stream);
-    case 8: return launch_doppler_adj<8>(a, stream);
-    case 16: return launch_doppler_adj<16>(a, stream);
-    case 32: return launch_doppler_adj<32>(a, stream);
-    case 64: return launch_doppler_adj<64>(a, stream);
-    case 128: return launch_doppler_adj<128>(a, stream);
-    case 256: return launch_doppler_adj<256>(a, stream);
-    case 512: return launch_doppler_adj<512>(a, stream);
-    case 1024: return launch_doppler_adj<1024>(a, stream);
+  return dispatch_doppler_adj(nd, a, nullptr, stream);
+}
+
+static mxr_status doppler_taps_adjoint_impl(const mxr_tensor *gtaps, const mxr_tensor *gx,
+                                            const mxr_tensor *gpow, const mxr_tensor *x,
+                                            const mxr_tensor *taps, int64_t nd, hipStream_t s) {
+  const char *fn = "doppler_taps_adjoint";
+  mxr_status st;
+  if ((st = check_tensor(x, "x", MXR_C64, 3, 3))) return st;
+  if ((st = check_tensor(taps, "taps", MXR_F32, 1, 1))) return st;
+  if ((st = check_tensor(gpow, "gpow", MXR_F32, 3, 3))) return st;
+  MXR_REQUIRE(gtaps || gx, MXR_INVALID_PARAMETER, "%s: no gradient requested", fn);
+  const int64_t C = x->shape[0], P = x->shape[1], R = x->shape[2];
+  MXR_REQUIRE(taps->shape[0] == P, MXR_INVALID_SIZE, "%s: %lld taps for %lld pulses", fn,
+              (long long)taps->shape[0], (long long)P);
+  MXR_REQUIRE(nd >= 2 && nd <= 1024 && (nd & (nd - 1)) == 0 && nd >= P, MXR_NOT_SUPPORTED,
+              "%s: FFT size %lld must be a power of two in [max(2, P), 1024]", fn, (long long)nd);
+  MXR_REQUIRE(gpow->shape[0] == C && gpow->shape[1] == nd && gpow->shape[2] == R,
+              MXR_INVALID_SIZE, "%s: gpow must be [%lld, %lld, %lld]", fn, (long long)C,
+              (long long)nd, (long long)R);
+  if (gtaps) {
+    if ((st = check_tensor(gtaps, "gtaps", MXR_F32, 1, 1))) return st;
+    MXR_REQUIRE(gtaps->shape[0] == P, MXR_INVALID_SIZE, "%s: gtaps must have %lld entries", fn,
+                (long long)P);
   }
-  return MXR_NOT_SUPPORTED;
+  if (gx) {
+    if ((st = check_tensor(gx, "gx", MXR_C64, 3, 3))) return st;
+    MXR_REQUIRE(gx->shape[0] == C && gx->shape[1] == P && gx->shape[2] == R, MXR_INVALID_SIZE,
+                "%s: gx must have x's shape", fn);
+  }
+  for (const mxr_tensor *o : {gtaps, gx}) {
+    if (!o) continue;
+    for (const mxr_tensor *i : {x, taps, gpow})
+      MXR_REQUIRE(!views_overlap(*o, *i), MXR_ALIAS_ERROR, "%s: an output overlaps an input", fn);
+  }
+  if (gtaps && gx) MXR_REQUIRE(!views_overlap(*gtaps, *gx), MXR_ALIAS_ERROR, "%s: outputs overlap", fn);
+  if (C == 0 || R == 0 || P == 0) return MXR_SUCCESS;
+  MXR_REQUIRE(C <= 65535 && R < (1ll << 30), MXR_INVALID_SIZE, "%s: shape too large", fn);
+  DopGradArgs a{};
+  a.y = (const float2 *)x->data;
+  a.ys_c = x->stride[0];
+  a.ys_p = x->stride[1];
+  a.ys_r = x->stride[2];
+  a.gp = (const float *)gpow->data;
+  a.gs_c = gpow->stride[0];
+  a.gs_k = gpow->stride[1];
+  a.gs_r = gpow->stride[2];
+  if (gx) {
+    a.gy = (float2 *)gx->data;
+    a.gys_c = gx->stride[0];
+    a.gys_p = gx->stride[1];
+    a.gys_r = gx->stride[2];
+  }
+  a.taps = (const float *)taps->data;
+  a.taps_s = taps->stride[0];
+  a.C = (int)C;
+  a.P = (int)P;
+  a.R = (int)R;
+  a.rows_t = (int)P;
+  a.p_off = 0;
+  a.no_tpc = 1;
+  a.win_rows = 0;
+  a.tw = twiddles_device((int)nd, false);
+  MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
+  return dispatch_doppler_adj(nd, a, gtaps, s);
 }
 
 }  // namespace mxr
