@@ -8,10 +8,12 @@
 // Tile: a workgroup owns TB = 64 consecutive range bins (one wave wide) of
 // one channel across all pulses.  The CFAR window needs HX range bins on
 // either side of a cell, so a tile computes the Doppler spectrum of all 64
-// columns and emits detections for the OUT = 64 - 2*HXA inner columns
-// (HXA = HX rounded up to a multiple of 4, so output quads stay 16-byte
-// aligned); neighbouring tiles overlap by 2*HXA columns, which the XCD-local
-// L2 serves.
+// columns and emits detections for the OUT = 64 - 2*HXA inner columns (HXA =
+// HX rounded up to even, so OUT is a multiple of 4 and every output quad
+// starts on a 16-byte boundary of the detection rows); neighbouring tiles
+// overlap by 2*HXA columns, which the XCD-local L2 serves.  The narrowest
+// halo matters: at 16 x 8001 range bins and two resident workgroups per CU,
+// 52 outputs per tile fill 4.8 rounds of the machine where 48 need 5.2.
 //
 //  1. load: lane = range column, wave = a strip of Doppler-input rows; each
 //     lane reads its strip's pulses plus one on either side (512-byte
@@ -80,7 +82,7 @@ constexpr int DC_PS = 64;   // power-tile row stride (floats)
 
 template <int HX>
 struct DcGeom {
-  static constexpr int HXA = (HX + 3) & ~3;
+  static constexpr int HXA = (HX + 1) & ~1;
   static constexpr int OUT = DC_TB - 2 * HXA;
 };
 
@@ -208,9 +210,9 @@ void dopcfar_kernel(DcArgs a) {
   constexpr int NQ = 1 + 2 * HXA / 4;         // quads read per row
   constexpr int NV = 4 * NQ;
   const int grp = threadIdx.x % GR, band = threadIdx.x / GR;
-  const int tc0 = 4 * grp;  // first tile column of the quad
+  const int tc0 = HXA + 4 * grp;  // first tile column of the quad
   const int k0 = band * RB;
-  if (band >= NB || tc0 < HXA || tc0 >= HXA + OUT) return;  // no barrier follows
+  if (band >= NB || grp >= OUT / 4) return;  // no barrier follows
   const int gr0 = r0 + tc0;  // its range bin (a multiple of 4, >= 0)
   if (gr0 >= a.R) return;
   const int R = a.R;
@@ -281,7 +283,14 @@ void dopcfar_kernel(DcArgs a) {
         thr[j] = thr_tab[min(max(nrm[j], 0), NCELL)];
       }
     }
-    const float4 cut = *reinterpret_cast<const float4 *>(pwt + kc * PS + tc0);
+    float4 cut;
+    if constexpr (HXA % 4 == 0) {
+      cut = *reinterpret_cast<const float4 *>(pwt + kc * PS + tc0);
+    } else {  // 8-byte aligned: two 64-bit LDS reads
+      const float2 c01 = *reinterpret_cast<const float2 *>(pwt + kc * PS + tc0);
+      const float2 c23 = *reinterpret_cast<const float2 *>(pwt + kc * PS + tc0 + 2);
+      cut = make_float4(c01.x, c01.y, c23.x, c23.y);
+    }
     const float cv[4] = {cut.x, cut.y, cut.z, cut.w};
     int det[4];
     float bav[4];
