@@ -258,7 +258,7 @@ def bench_grad(args):
     p = torch.tensor(phi0, device=dev, requires_grad=True)
     t = torch.tensor(h0, device=dev, requires_grad=True)
     X = torch.tensor(X0, device=dev)
-    tg = torch.tensor(np.array(tg0), dtype=torch.int64, device=dev)
+    tg = torch.tensor(np.array(tg0), dtype=torch.int32, device=dev)
 
     def fwd():
         return ag.design_loss(p, t, X, tg, n, nd)
@@ -354,7 +354,7 @@ def bench_grad(args):
                    "fwd_ms_graph": round(graph_fwd, 4), "step_ms_graph": round(graph_step, 4),
                    "fwd_bwd_over_fwd": round(best / fwd_best, 3),
                    "graph_replay_vs_eager_rel": replay_dev,
-                   "loss_rel_vs_fp64": abs(float(L_e) - L64) / abs(L64),
+                   "loss_rel_vs_fp64": abs(float(L_e.detach()) - L64) / abs(L64),
                    "grad_phases_rel_vs_fp64": float(np.linalg.norm(gp - gp64) / np.linalg.norm(gp64)),
                    "grad_taps_rel_vs_fp64": float(np.linalg.norm(gt - gt64) / np.linalg.norm(gt64))},
         "cpu_baseline": {"value": round(1.0 / cpu_s, 3), "unit": "steps/s",

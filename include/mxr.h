@@ -230,6 +230,26 @@ mxr_status mxr_doppler_taps_adjoint(const mxr_tensor *gtaps, const mxr_tensor *g
                                     const mxr_tensor *gpow, const mxr_tensor *x,
                                     const mxr_tensor *taps, int64_t nd, hipStream_t stream);
 
+/* Sidelobe loss heads for waveform/window design, forward and adjoint (no
+ * MatX counterpart).  All reductions in double.
+ * mxr_isl: r [B, n] c64 (unit row stride) -> isl [B] f32,
+ *   isl_b = (sum_l |r_b[l]|^2 - |r_b[0]|^2) / |r_b[0]|^2.
+ * mxr_isl_adjoint: gr [B, n] c64 = d(sum_b gisl_b isl_b)/dr.
+ * mxr_psl: pw [C, nd, R] f32, targets [T, 3] i32 (channel, range bin,
+ *   Doppler bin kd) -> psl [T] f32,
+ *   psl_t = (sum_{k: circular |k - kd| > ml} pw[c, k, r]^q)^(1/q) / pw[c, kd, r]
+ *   (NaN for a target outside pw).
+ * mxr_psl_adjoint: gpow [C, nd, R] f32 (contiguous) = d(sum_t gpsl_t psl_t)/dpw;
+ *   zero outside the target columns, columns shared by targets summed in
+ *   target order. */
+mxr_status mxr_isl(const mxr_tensor *isl, const mxr_tensor *r, hipStream_t stream);
+mxr_status mxr_isl_adjoint(const mxr_tensor *gr, const mxr_tensor *gisl, const mxr_tensor *r,
+                           hipStream_t stream);
+mxr_status mxr_psl(const mxr_tensor *psl, const mxr_tensor *pw, const mxr_tensor *targets, int ml,
+                   int q, hipStream_t stream);
+mxr_status mxr_psl_adjoint(const mxr_tensor *gpow, const mxr_tensor *gpsl, const mxr_tensor *pw,
+                           const mxr_tensor *targets, int ml, int q, hipStream_t stream);
+
 /* --------------------------------------------------------- ragged batch */
 /* Many independent pulse-compression + Doppler jobs of uneven shapes in a
  * few persistent launches (one per block-size class and stage), for capture

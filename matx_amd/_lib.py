@@ -28,7 +28,8 @@ EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_ena
            "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy",
            "mxr_fft_conv_adjoint", "mxr_autocorr_adjoint", "mxr_doppler_taps", "mxr_doppler_taps_adjoint",
            "mxr_ragged_plan_create", "mxr_ragged_execute", "mxr_ragged_plan_block_size",
-           "mxr_ragged_plan_destroy", "mxr_doppler_adjoint", "mxr_pc_plan_waveform_grad")
+           "mxr_ragged_plan_destroy", "mxr_doppler_adjoint", "mxr_pc_plan_waveform_grad",
+           "mxr_isl", "mxr_isl_adjoint", "mxr_psl", "mxr_psl_adjoint")
 
 STATUS_NAMES = {0: "SUCCESS", 1: "NOT_SUPPORTED", 2: "INVALID_PARAMETER",
                 3: "INVALID_DIM", 4: "INVALID_SIZE", 5: "OUT_OF_MEMORY",
@@ -120,6 +121,10 @@ def lib():
         "mxr_ragged_plan_destroy": [V],
         "mxr_doppler_adjoint": [T, T, T, I, I64, I64, V],
         "mxr_pc_plan_waveform_grad": [V, T, T, T, T, V],
+        "mxr_isl": [T, T, V],
+        "mxr_isl_adjoint": [T, T, T, V],
+        "mxr_psl": [T, T, T, I, I, V],
+        "mxr_psl_adjoint": [T, T, T, T, I, I, V],
     }
     for name, argt in sigs.items():
         fn = getattr(L, name)

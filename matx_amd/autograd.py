@@ -145,6 +145,37 @@ def contrast_weights(nd: int, R: int, cells, C: int = 1, device=None):
     return wm
 
 
+class Isl(torch.autograd.Function):
+    """Per-row integrated sidelobe level (mxr_isl / mxr_isl_adjoint)."""
+
+    @staticmethod
+    def forward(ctx, r):
+        r = r.contiguous()
+        ctx.save_for_backward(r)
+        return ops.isl(r)
+
+    @staticmethod
+    def backward(ctx, g):
+        (r,) = ctx.saved_tensors
+        return ops.isl_adjoint(g.contiguous(), r)
+
+
+class Psl(torch.autograd.Function):
+    """Peak-sidelobe ratio of Doppler columns at targets (mxr_psl /
+    mxr_psl_adjoint); the gradient is zero outside the target columns."""
+
+    @staticmethod
+    def forward(ctx, pw, targets, ml: int, q: int):
+        ctx.save_for_backward(pw, targets)
+        ctx.ml, ctx.q = ml, q
+        return ops.psl(pw, targets, ml, q)
+
+    @staticmethod
+    def backward(ctx, g):
+        pw, targets = ctx.saved_tensors
+        return ops.psl_adjoint(g.contiguous(), pw, targets, ctx.ml, ctx.q), None, None, None
+
+
 def fft_conv(x, w, n: int, conj: bool = True):
     return FFTConv.apply(x, w, n, conj)
 
@@ -167,19 +198,15 @@ def sidelobe_mask(nd: int, kd: torch.Tensor, ml: int) -> torch.Tensor:
 def design_loss(phi, taps, cube, targets, n: int, nd: int, lam: float = 1.0, ml: int = 2,
                 q: int = 8):
     """phi: [B, Lw] phases; taps: [P] Doppler window; cube: [C, P, R] c64;
-    targets: [T, 3] int64 (channel, range bin, Doppler bin).  Returns
-    mean ISL + lam * mean PSL (the definitions of oracle/grad.py)."""
+    targets: [T, 3] int (channel, range bin, Doppler bin; int32 avoids a
+    conversion per call).  Returns mean ISL + lam * mean PSL (the
+    definitions of oracle/grad.py); both heads are single HIP kernels."""
     s = torch.polar(torch.ones_like(phi), phi)
-    r = autocorr(s, n)
-    pr = torch.view_as_real(r).square().sum(-1)   # |r|^2 on contiguous floats
-    isl = (pr.sum(-1) - pr[:, 0]) / pr[:, 0]
+    isl = Isl.apply(autocorr(s, n))
     pw = doppler_power(cube, taps, nd)
-    c, rb, kd = targets[:, 0], targets[:, 1], targets[:, 2]
-    cols = pw[c, :, rb]                                    # [T, nd]
-    side = torch.where(sidelobe_mask(nd, kd, ml), cols, torch.zeros_like(cols))
-    A = (side ** q).sum(-1) ** (1.0 / q)
-    M = cols.gather(1, kd[:, None])[:, 0]
-    return isl.mean() + lam * (A / M).mean()
+    tg = targets if targets.dtype == torch.int32 else targets.to(torch.int32)
+    psl = Psl.apply(pw, tg, ml, q)
+    return isl.mean() + lam * psl.mean()
 
 
 def make_design_problem(seed: int, B: int = 64, Lw: int = 1024, C: int = 16, P: int = 128,

@@ -242,6 +242,37 @@ def doppler_taps_adjoint(gpow, x, taps, nd: int, want_gx: bool = False, want_gta
     return gx, gt
 
 
+def isl(r, stream=None):
+    """Integrated sidelobe level of each row of r ([B, n] c64): [B] f32."""
+    out = torch.empty((r.shape[0],), dtype=torch.float32, device=r.device)
+    L.check(L.lib().mxr_isl(L.ref(L.desc(out)), L.ref(L.desc(r)), L.stream_handle(stream)))
+    return out
+
+
+def isl_adjoint(gisl, r, stream=None):
+    gr = torch.empty_like(r)
+    L.check(L.lib().mxr_isl_adjoint(L.ref(L.desc(gr)), L.ref(L.desc(gisl)), L.ref(L.desc(r)),
+                                    L.stream_handle(stream)))
+    return gr
+
+
+def psl(pw, targets, ml: int = 2, q: int = 8, stream=None):
+    """q-norm peak-sidelobe ratio of the Doppler columns pw[c, :, r] at
+    targets ([T, 3] int32: channel, range bin, Doppler bin): [T] f32."""
+    out = torch.empty((targets.shape[0],), dtype=torch.float32, device=pw.device)
+    L.check(L.lib().mxr_psl(L.ref(L.desc(out)), L.ref(L.desc(pw)), L.ref(L.desc(targets)),
+                            int(ml), int(q), L.stream_handle(stream)))
+    return out
+
+
+def psl_adjoint(gpsl, pw, targets, ml: int = 2, q: int = 8, stream=None):
+    gp = torch.empty_like(pw, memory_format=torch.contiguous_format)
+    L.check(L.lib().mxr_psl_adjoint(L.ref(L.desc(gp)), L.ref(L.desc(gpsl)), L.ref(L.desc(pw)),
+                                    L.ref(L.desc(targets)), int(ml), int(q),
+                                    L.stream_handle(stream)))
+    return gp
+
+
 def conv1d(x, f, mode="full", out=None, stream=None):
     """Direct conv1d along the last axis (include/matx/operators/conv.h:244-266)."""
     m = _mode(m=mode)

@@ -214,3 +214,28 @@ def test_radar_gradient_checks(L):
     r, _ = status(L, lib.mxr_pc_plan_waveform_grad, None, ctypes.byref(gw), ctypes.byref(gy),
                   ctypes.byref(y), ctypes.byref(gw), None)
     assert r == 2                                   # null plan
+
+
+def test_loss_head_checks(L):
+    lib = L.lib()
+    r = fake(L, [4, 100], L.C64)
+    isl_bad = fake(L, [3], L.F32, addr=0x900000)
+    r_, _ = status(L, lib.mxr_isl, ctypes.byref(isl_bad), ctypes.byref(r), None)
+    assert r_ == 4                                  # one value per row
+    r_strided = fake(L, [4, 100], L.C64, strides=[200, 2])
+    isl = fake(L, [4], L.F32, addr=0x900000)
+    r_, _ = status(L, lib.mxr_isl, ctypes.byref(isl), ctypes.byref(r_strided), None)
+    assert r_ == 2                                  # rows need unit stride
+    pw = fake(L, [2, 8, 10], L.F32)
+    tg = fake(L, [3, 2], L.I32, addr=0x900000)
+    out = fake(L, [3], L.F32, addr=0xA00000)
+    r_, _ = status(L, lib.mxr_psl, ctypes.byref(out), ctypes.byref(pw), ctypes.byref(tg), 2, 8, None)
+    assert r_ == 4                                  # targets must be [T, 3]
+    tg = fake(L, [3, 3], L.I32, addr=0x900000)
+    r_, _ = status(L, lib.mxr_psl, ctypes.byref(out), ctypes.byref(pw), ctypes.byref(tg), 2, 0, None)
+    assert r_ == 2                                  # q >= 1
+    gp = fake(L, [2, 8, 10], L.F32, addr=0xB00000, strides=[160, 20, 2])
+    g = fake(L, [3], L.F32, addr=0xC00000)
+    r_, _ = status(L, lib.mxr_psl_adjoint, ctypes.byref(gp), ctypes.byref(g), ctypes.byref(pw),
+                   ctypes.byref(tg), 2, 8, None)
+    assert r_ == 2                                  # gpow must be contiguous
