@@ -19,19 +19,20 @@ namespace mxr {
 template <int ND>
 struct DopCfg {
   static constexpr int NT = 256;
-  // 32 range bins (256-byte rows) per tile up to 256 Doppler bins (32 KiB of
-  // LDS at 128, 64 KiB at 256), then 64 KiB tiles (16 bins at 512, 8 at
-  // 1024): narrower tiles waste most of every 128-byte line they touch
-  static constexpr int TR = ND <= 256 ? 32 : 8192 / ND;
+  // range bins per tile (measured on MI355X): 32 (256-byte rows, 32 KiB of
+  // LDS at 128 Doppler bins) up to 128; 16 (one 128-byte line per pulse row)
+  // at 256 and 512, where 32 would hold a 64 KiB tile and halve the resident
+  // workgroups (40% slower at 256); 8 at 1024.  Narrower tiles waste part of
+  // every 128-byte line they touch.
+  static constexpr int TR = ND <= 128 ? 32 : (ND <= 512 ? 16 : 8);
 };
 
 // Coalesced load of a pulse tile: rows p < P of TR range bins from r0 into
 // lds[p * TR + t] (zero past R), 16 loads in flight per lane.
-template <int ND>
+template <int ND, int TR = DopCfg<ND>::TR>
 __device__ __forceinline__ void load_pulse_tile(const float2 *yc, int64_t ys_p, int64_t ys_r,
                                                 int P, int R, int r0, float2 *lds) {
   constexpr int NT = DopCfg<ND>::NT;
-  constexpr int TR = DopCfg<ND>::TR;
   constexpr int RPI = NT / TR;  // pulses covered by one sweep of the workgroup
   constexpr int BATCH = 16;
   const int t = threadIdx.x % TR, p0 = threadIdx.x / TR;
@@ -56,11 +57,10 @@ __device__ __forceinline__ void load_pulse_tile(const float2 *yc, int64_t ys_p, 
 // One tile: TR range bins from r0 of channel c.  lds holds rows_buf * TR
 // elements, win ND floats.  Pre: the workgroup's previous use of both is
 // finished.
-template <int ND>
+template <int ND, int TR = DopCfg<ND>::TR>
 __device__ __forceinline__ void doppler_tile(const DopArgs &a, int c, int r0, int rows_buf,
                                              float2 *lds, float *win) {
   constexpr int NT = DopCfg<ND>::NT;
-  constexpr int TR = DopCfg<ND>::TR;
   using LY = Layout<ND, TR, true>;
   using SQ = Seq<ND, false>;
   constexpr int K = SQ::n;
@@ -70,7 +70,7 @@ __device__ __forceinline__ void doppler_tile(const DopArgs &a, int c, int r0, in
       h = a.taps ? a.taps[(int64_t)i * a.taps_s] : (i < a.win_rows ? hamming_f32(i, a.win_rows) : 1.0f);
     win[i] = h;
   }
-  load_pulse_tile<ND>(a.y + (int64_t)c * a.ys_c, a.ys_p, a.ys_r, a.P, a.R, r0, lds);
+  load_pulse_tile<ND, TR>(a.y + (int64_t)c * a.ys_c, a.ys_p, a.ys_r, a.P, a.R, r0, lds);
   wg_barrier();
 
   auto tpc = [&](int t, int i) -> float2 {
@@ -118,24 +118,29 @@ __global__ __launch_bounds__(256) void doppler_kernel(DopArgs a, int rows_buf) {
   doppler_tile<ND>(a, blockIdx.y, blockIdx.x * DopCfg<ND>::TR, rows_buf, lds, win);
 }
 
-// Ragged batch: one persistent grid walks the (job, channel, tile) items of
-// every job of one Doppler size (jobs[j].item0 are prefix sums of C*tiles).
+// Ragged batch: the (job, channel, tile) items of every job of one Doppler
+// size (jobs[j].item0 are prefix sums of C*tiles); item_job[it] is the job
+// of item it (a table built with the plan).  PERSIST: a resident grid loops
+// over the items; otherwise one workgroup per item.
+// Batch tiles stay 32 range bins wide up to 256 Doppler bins: the per-item
+// decode (job lookup, record copy) is paid per tile, and at 256 the wider
+// tile measured faster in the batch although the single call prefers 16.
+template <int ND>
+struct DopBatchCfg {
+  static constexpr int TR = ND <= 256 ? 32 : DopCfg<ND>::TR;
+};
+
 template <int ND, bool PERSIST>
-__global__ __launch_bounds__(256) void doppler_batch_kernel(const DopJob *jobs, int njobs,
+__global__ __launch_bounds__(256) void doppler_batch_kernel(const DopJob *jobs, const int *item_job,
                                                             int64_t items) {
-  constexpr int TR = DopCfg<ND>::TR;
+  constexpr int TR = DopBatchCfg<ND>::TR;
   __shared__ float2 lds[ND * TR];
   __shared__ float win[ND];
   auto item = [&](int64_t it) {
-    int lo = 0, hi = njobs - 1;  // last job with item0 <= it
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (jobs[mid].item0 <= it) lo = mid;
-      else hi = mid - 1;
-    }
-    const DopJob jb = uniform_copy(&jobs[lo]);
+    const int ji = item_job ? __builtin_amdgcn_readfirstlane(item_job[it]) : 0;
+    const DopJob jb = uniform_copy(&jobs[ji]);
     const int64_t local = it - jb.item0;
-    doppler_tile<ND>(jb.a, (int)(local / jb.tiles), (int)(local % jb.tiles) * TR, ND, lds, win);
+    doppler_tile<ND, TR>(jb.a, (int)(local / jb.tiles), (int)(local % jb.tiles) * TR, ND, lds, win);
   };
   if constexpr (!PERSIST) {
     item(blockIdx.x);
@@ -147,7 +152,7 @@ __global__ __launch_bounds__(256) void doppler_batch_kernel(const DopJob *jobs, 
   }
 }
 
-mxr_status launch_doppler_batch(int nd, const DopJob *jobs, int njobs, int64_t items, int grid,
+mxr_status launch_doppler_batch(int nd, const DopJob *jobs, const int *item_job, int64_t items, int grid,
                                 hipStream_t s) {
   if (items == 0) return MXR_SUCCESS;
   MXR_REQUIRE(grid >= 1 && grid <= items && items <= 0x7fffffff, MXR_INVALID_SIZE,
@@ -157,8 +162,8 @@ mxr_status launch_doppler_batch(int nd, const DopJob *jobs, int njobs, int64_t i
   switch (nd) {
 #define MXR_DB(N)                                                                          \
   case N:                                                                                  \
-    if (persist) doppler_batch_kernel<N, true><<<dim3(grid), dim3(256), 0, s>>>(jobs, njobs, items); \
-    else doppler_batch_kernel<N, false><<<dim3(grid), dim3(256), 0, s>>>(jobs, njobs, items); \
+    if (persist) doppler_batch_kernel<N, true><<<dim3(grid), dim3(256), 0, s>>>(jobs, item_job, items); \
+    else doppler_batch_kernel<N, false><<<dim3(grid), dim3(256), 0, s>>>(jobs, item_job, items); \
     break;
     MXR_DB(2)
     MXR_DB(4)
@@ -181,16 +186,16 @@ mxr_status launch_doppler_batch(int nd, const DopJob *jobs, int njobs, int64_t i
 
 int doppler_batch_tile(int nd) {
   switch (nd) {
-    case 2: return DopCfg<2>::TR;
-    case 4: return DopCfg<4>::TR;
-    case 8: return DopCfg<8>::TR;
-    case 16: return DopCfg<16>::TR;
-    case 32: return DopCfg<32>::TR;
-    case 64: return DopCfg<64>::TR;
-    case 128: return DopCfg<128>::TR;
-    case 256: return DopCfg<256>::TR;
-    case 512: return DopCfg<512>::TR;
-    case 1024: return DopCfg<1024>::TR;
+    case 2: return DopBatchCfg<2>::TR;
+    case 4: return DopBatchCfg<4>::TR;
+    case 8: return DopBatchCfg<8>::TR;
+    case 16: return DopBatchCfg<16>::TR;
+    case 32: return DopBatchCfg<32>::TR;
+    case 64: return DopBatchCfg<64>::TR;
+    case 128: return DopBatchCfg<128>::TR;
+    case 256: return DopBatchCfg<256>::TR;
+    case 512: return DopBatchCfg<512>::TR;
+    case 1024: return DopBatchCfg<1024>::TR;
   }
   return 0;
 }

@@ -357,6 +357,125 @@ void pc4096_kernel(Pc4096Args pa) {
 }
 
 
+// Dedicated M = 8192 block kernel: radix sequence 2 x 16 x 16 x 16 over 512
+// threads (one radix-16 butterfly per thread per pass) with the power-base
+// twiddles of the 4096-point kernel for the passes Ls = 2, 32, 512
+// (conjugated for the inverse).  The forward radix-2 pass (Ls = 1, no
+// twiddles) is folded into the load: radix-16 butterfly j of the next pass
+// takes X1[j + 512m] = x[j0] +- x[j0 + 4096] with j0 = j/2 + 256m and the
+// sign given by j's parity, so every thread reads its 16 input pairs straight
+// from HBM (its neighbour reads the same pairs for the other sign, from L1).
+// The inverse radix-2 pass pairs bins k and k + 4096, which the forward's
+// last pass leaves in one thread (slots q and q + 8), so it runs in
+// registers before the first inverse exchange: five LDS exchanges per block.
+struct Pc8192Args {
+  PcArgs a;
+  const float4 *bA, *bB, *bC;  // power-base rows for Ls = 2, 32, 512
+};
+
+__device__ __forceinline__ void ols_block8192(const OlsBlock ob, const float4 *bA, const float4 *bB,
+                                              const float4 *bC, float2 *lds) {
+  constexpr int NT = 512;
+  using LY = Layout<8192, 1, false>;
+  const int j = threadIdx.x;
+  const float2 *__restrict__ xr = ob.xr;
+  float2 *__restrict__ yr = ob.yr;
+  const int lo = ob.lo, n32 = ob.n, seff32 = ob.seff;
+
+  float2 v[16];
+  auto xchg = [&](int ls_out) {
+    const int k = j & (ls_out - 1), base = (j - k) * 16 + k;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) lds[LY::addr(0, base + q * ls_out)] = v[q];
+    wg_barrier();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = lds[LY::addr(0, j + m * NT)];
+    wg_barrier();
+  };
+  auto ldx = [&](int i) {
+    int gi = lo + i;
+    if (gi >= n32) gi -= n32;
+    return gi < seff32 ? xr[gi] : make_float2(0.f, 0.f);
+  };
+  const float4 *pA = bA + (j & 1) * 2;
+  const float4 *pB = bB + (j & 31) * 2;
+  const float4 *pC = bC + j * 2;
+
+  // forward radix-2 pass folded into the load
+  {
+    const int jh = j >> 1;
+    const bool odd = (j & 1) != 0;
+    float2 a[16], b[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      a[m] = ldx(jh + 256 * m);
+      b[m] = ldx(jh + 256 * m + 4096);
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = odd ? csub(a[m], b[m]) : cadd(a[m], b[m]);
+  }
+  float4 t0 = pA[0], t1 = pA[1];
+  tw_from_base(v, t0, t1, false);  // forward pass Ls = 2
+  dft16<false>(v);
+  t0 = pB[0];
+  t1 = pB[1];
+  xchg(2);
+  tw_from_base(v, t0, t1, false);  // forward pass Ls = 32
+  dft16<false>(v);
+  t0 = pC[0];
+  t1 = pC[1];
+  xchg(32);
+  tw_from_base(v, t0, t1, false);  // forward pass Ls = 512: bins j + 512q
+  dft16<false>(v);
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], ob.wc[j + q * 512]);
+  // inverse radix-2 pass (Ls = 1) on bins j + 512q, j + 512q + 4096
+  t0 = pA[0];
+  t1 = pA[1];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int e = 2 * (j + 512 * q);
+    lds[LY::addr(0, e)] = cadd(v[q], v[q + 8]);
+    lds[LY::addr(0, e + 1)] = csub(v[q], v[q + 8]);
+  }
+  wg_barrier();
+#pragma unroll
+  for (int m = 0; m < 16; ++m) v[m] = lds[LY::addr(0, j + m * NT)];
+  wg_barrier();
+  tw_from_base(v, t0, t1, true);  // inverse pass Ls = 2
+  dft16<true>(v);
+  t0 = pB[0];
+  t1 = pB[1];
+  xchg(2);
+  tw_from_base(v, t0, t1, true);  // inverse pass Ls = 32
+  dft16<true>(v);
+  t0 = pC[0];
+  t1 = pC[1];
+  xchg(32);
+  tw_from_base(v, t0, t1, true);  // inverse pass Ls = 512: lags j + 512q
+  dft16<true>(v);
+  const int lim = (ob.nout - lo) < ob.V ? (ob.nout - lo) : ob.V;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = j + q * 512;
+    if (i < lim) yr[lo + i] = v[q];  // the spectrum carries the 1/M
+  }
+}
+
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+void pc8192_kernel(Pc8192Args pa) {
+  using LY = Layout<8192, 1, false>;
+  __shared__ float2 lds[LY::LDS_ELEMS];
+  const PcArgs &a = pa.a;
+  const int64_t g = xcd_remap(blockIdx.x, a.groups);
+  const int64_t row = g / a.nblk;
+  int64_t ox, oy;
+  batch_offsets(a.bi, row, &ox, &oy);
+  const OlsBlock ob{a.x + ox, a.y + oy, (int)((g % a.nblk) * a.V), (int)a.n, (int)a.seff,
+                    (int)a.nout, (int)a.V, a.wc};
+  ols_block8192(ob, pa.bA, pa.bB, pa.bC, lds);
+}
+
 // One workgroup of M/16 threads.  Thread j loads samples j + m*M/16,
 // m < 16 (all loads in flight at once), which are exactly the inputs of
 // butterfly j of the first radix-16 Stockham pass, so the window, the
@@ -439,16 +558,16 @@ __global__ __launch_bounds__(PcCfg<M>::NT) void wfspec_batch_kernel(const WfArgs
   wfspec_body<M>(args[blockIdx.x], lds, red);
 }
 
-// Ragged batch: one persistent grid walks the (job, row, block) items of
-// every job of one block size (jobs[j].item0 are prefix sums of C*P*nblk).
-__device__ __forceinline__ int find_job(const PcJob *jobs, int njobs, int64_t it) {
-  int lo = 0, hi = njobs - 1;  // last job with item0 <= it
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].item0 <= it) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
+// Ragged batch: the (job, row, block) items of every job of one block size
+// (jobs[j].item0 are prefix sums of C*P*nblk).  item_job[it] is the job of
+// item it, a table built with the plan (one load, where a search over the
+// jobs costs a chain of dependent loads per workgroup); null means a batch
+// of one job.  item_job_of is for a workgroup-uniform item.
+__device__ __forceinline__ int item_job_of(const int *item_job, int64_t it) {
+  return item_job ? __builtin_amdgcn_readfirstlane(item_job[it]) : 0;
+}
+__device__ __forceinline__ int item_job_lane(const int *item_job, int64_t it) {
+  return item_job ? item_job[it] : 0;
 }
 
 __device__ __forceinline__ OlsBlock job_block(const PcJob &jb, int64_t it) {
@@ -465,16 +584,16 @@ __device__ __forceinline__ OlsBlock job_block(const PcJob &jb, int64_t it) {
 // no per-pass state stays live across items).
 template <int M, bool PERSIST>
 __global__ __launch_bounds__(PcCfg<M>::NT) __attribute__((amdgpu_waves_per_eu(4)))
-void pc_batch_kernel(const PcJob *jobs, int njobs, int64_t items, const float2 *twf,
+void pc_batch_kernel(const PcJob *jobs, const int *item_job, int64_t items, const float2 *twf,
                      const float2 *twi) {
   __shared__ float2 lds[Layout<M, 1, false>::LDS_ELEMS];
   if constexpr (!PERSIST) {
     const int64_t it = blockIdx.x;
-    ols_block<M>(job_block(uniform_copy(&jobs[find_job(jobs, njobs, it)]), it), twf, twi, lds);
+    ols_block<M>(job_block(uniform_copy(&jobs[item_job_of(item_job, it)]), it), twf, twi, lds);
     return;
   }
   for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
-    const PcJob jb = uniform_copy(&jobs[find_job(jobs, njobs, it)]);
+    const PcJob jb = uniform_copy(&jobs[item_job_of(item_job, it)]);
     if (it != blockIdx.x) wg_barrier();  // previous item's LDS reads are done
     ols_block<M>(job_block(jb, it), opaque(twf), opaque(twi), lds);
   }
@@ -483,7 +602,7 @@ void pc_batch_kernel(const PcJob *jobs, int njobs, int64_t items, const float2 *
 // Ragged batch, small blocks: 4096/M consecutive items per workgroup (items
 // of different jobs may share one).
 template <int M>
-__global__ __launch_bounds__(256) void pc_batch_multi_kernel(const PcJob *jobs, int njobs,
+__global__ __launch_bounds__(256) void pc_batch_multi_kernel(const PcJob *jobs, const int *item_job,
                                                              int64_t items, const float2 *twf,
                                                              const float2 *twi) {
   constexpr int TB = 4096 / M;
@@ -492,7 +611,7 @@ __global__ __launch_bounds__(256) void pc_batch_multi_kernel(const PcJob *jobs, 
   if (threadIdx.x < TB) {
     const int64_t it0 = (int64_t)blockIdx.x * TB;
     const int64_t it = it0 + threadIdx.x < items ? it0 + threadIdx.x : it0;
-    OlsBlock ob = job_block(jobs[find_job(jobs, njobs, it)], it);
+    OlsBlock ob = job_block(jobs[item_job_lane(item_job, it)], it);
     if (it0 + threadIdx.x >= items) {  // past the last item: idle transform
       ob.seff = 0;
       ob.nout = ob.lo;
@@ -505,18 +624,35 @@ __global__ __launch_bounds__(256) void pc_batch_multi_kernel(const PcJob *jobs, 
 
 template <bool PERSIST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-void pc4096_batch_kernel(const PcJob *jobs, int njobs, int64_t items, const float4 *base1,
+void pc4096_batch_kernel(const PcJob *jobs, const int *item_job, int64_t items, const float4 *base1,
                          const float4 *base2) {
   __shared__ float2 lds[Layout<4096, 1, false>::LDS_ELEMS];
   if constexpr (!PERSIST) {
     const int64_t it = blockIdx.x;
-    ols_block4096(job_block(uniform_copy(&jobs[find_job(jobs, njobs, it)]), it), base1, base2, lds);
+    ols_block4096(job_block(uniform_copy(&jobs[item_job_of(item_job, it)]), it), base1, base2, lds);
     return;
   }
   for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
-    const PcJob jb = uniform_copy(&jobs[find_job(jobs, njobs, it)]);
+    const PcJob jb = uniform_copy(&jobs[item_job_of(item_job, it)]);
     if (it != blockIdx.x) wg_barrier();
     ols_block4096(job_block(jb, it), opaque(base1), opaque(base2), lds);
+  }
+}
+
+template <bool PERSIST>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))
+void pc8192_batch_kernel(const PcJob *jobs, const int *item_job, int64_t items, const float4 *bA,
+                         const float4 *bB, const float4 *bC) {
+  __shared__ float2 lds[Layout<8192, 1, false>::LDS_ELEMS];
+  if constexpr (!PERSIST) {
+    const int64_t it = blockIdx.x;
+    ols_block8192(job_block(uniform_copy(&jobs[item_job_of(item_job, it)]), it), bA, bB, bC, lds);
+    return;
+  }
+  for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+    const PcJob jb = uniform_copy(&jobs[item_job_of(item_job, it)]);
+    if (it != blockIdx.x) wg_barrier();
+    ols_block8192(job_block(jb, it), opaque(bA), opaque(bB), opaque(bC), lds);
   }
 }
 
@@ -548,7 +684,7 @@ int pc_batch_grid(int M) {
     case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc_batch_kernel<1024, true>, PcCfg<1024>::NT, 0); break;
     case 2048: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc_batch_kernel<2048, true>, PcCfg<2048>::NT, 0); break;
     case 4096: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc4096_batch_kernel<true>, 256, 0); break;
-    case 8192: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc_batch_kernel<8192, true>, PcCfg<8192>::NT, 0); break;
+    case 8192: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc8192_batch_kernel<true>, 512, 0); break;
     case 16384: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc_batch_kernel<16384, true>, PcCfg<16384>::NT, 0); break;
     default: break;
   }
@@ -560,7 +696,7 @@ int pc_batch_grid(int M) {
 }
 
 // grid == items: one workgroup per item; otherwise the persistent form.
-mxr_status launch_pc_batch(int M, const PcJob *d_jobs, int njobs, int64_t items, int grid,
+mxr_status launch_pc_batch(int M, const PcJob *d_jobs, const int *d_item_job, int64_t items, int grid,
                            hipStream_t s) {
   if (items == 0) return MXR_SUCCESS;
   MXR_REQUIRE(grid >= 1 && grid <= items && items <= 0x7fffffff, MXR_INVALID_SIZE,
@@ -573,28 +709,35 @@ mxr_status launch_pc_batch(int M, const PcJob *d_jobs, int njobs, int64_t items,
     case 4096: {
       const float4 *b1 = pow_base_device(4096, 16), *b2 = pow_base_device(4096, 256);
       MXR_REQUIRE(b1 && b2, MXR_OUT_OF_MEMORY, "ragged batch: twiddle table allocation failed");
-      if (persist) pc4096_batch_kernel<true><<<dim3(grid), dim3(256), 0, s>>>(d_jobs, njobs, items, b1, b2);
-      else pc4096_batch_kernel<false><<<dim3(grid), dim3(256), 0, s>>>(d_jobs, njobs, items, b1, b2);
+      if (persist) pc4096_batch_kernel<true><<<dim3(grid), dim3(256), 0, s>>>(d_jobs, d_item_job, items, b1, b2);
+      else pc4096_batch_kernel<false><<<dim3(grid), dim3(256), 0, s>>>(d_jobs, d_item_job, items, b1, b2);
       break;
     }
 #define MXR_PB(MM)                                                                               \
   case MM:                                                                                       \
     if (persist)                                                                                 \
-      pc_batch_kernel<MM, true><<<dim3(grid), dim3(PcCfg<MM>::NT), 0, s>>>(d_jobs, njobs, items, twf, twi); \
+      pc_batch_kernel<MM, true><<<dim3(grid), dim3(PcCfg<MM>::NT), 0, s>>>(d_jobs, d_item_job, items, twf, twi); \
     else                                                                                         \
-      pc_batch_kernel<MM, false><<<dim3(grid), dim3(PcCfg<MM>::NT), 0, s>>>(d_jobs, njobs, items, twf, twi); \
+      pc_batch_kernel<MM, false><<<dim3(grid), dim3(PcCfg<MM>::NT), 0, s>>>(d_jobs, d_item_job, items, twf, twi); \
     break;
 #define MXR_PBM(MM)                                                                              \
   case MM:                                                                                       \
     if (persist)                                                                                 \
-      pc_batch_kernel<MM, true><<<dim3(grid), dim3(PcCfg<MM>::NT), 0, s>>>(d_jobs, njobs, items, twf, twi); \
+      pc_batch_kernel<MM, true><<<dim3(grid), dim3(PcCfg<MM>::NT), 0, s>>>(d_jobs, d_item_job, items, twf, twi); \
     else                                                                                         \
       pc_batch_multi_kernel<MM><<<dim3((unsigned)((items + 4096 / MM - 1) / (4096 / MM))), dim3(256), 0, s>>>( \
-          d_jobs, njobs, items, twf, twi);                                                       \
+          d_jobs, d_item_job, items, twf, twi);                                                       \
     break;
+    case 8192: {
+      const float4 *bA = pow_base_device(8192, 2), *bB = pow_base_device(8192, 32),
+                   *bC = pow_base_device(8192, 512);
+      MXR_REQUIRE(bA && bB && bC, MXR_OUT_OF_MEMORY, "ragged batch: twiddle table allocation failed");
+      if (persist) pc8192_batch_kernel<true><<<dim3(grid), dim3(512), 0, s>>>(d_jobs, d_item_job, items, bA, bB, bC);
+      else pc8192_batch_kernel<false><<<dim3(grid), dim3(512), 0, s>>>(d_jobs, d_item_job, items, bA, bB, bC);
+      break;
+    }
     MXR_PBM(1024)
     MXR_PBM(2048)
-    MXR_PB(8192)
     MXR_PB(16384)
 #undef MXR_PB
 #undef MXR_PBM
@@ -645,8 +788,9 @@ mxr_status plan_pc(int64_t seff, int64_t L, int64_t n, int64_t nout, bool allow_
       if (M < L || M > n) return;
       p.V = M - L + 1;
       // keep whole 64-lag groups per block so every block's stores start on
-      // a 512-byte boundary of an aligned row
-      if (p.V >= 64) p.V = (p.V / 64) * 64;
+      // a 512-byte boundary of an aligned row, unless that costs a block
+      if (p.V >= 64 && (nout + p.V - 1) / p.V == (nout + (p.V / 64) * 64 - 1) / ((p.V / 64) * 64))
+        p.V = (p.V / 64) * 64;
       p.nblk = (nout + p.V - 1) / p.V;
       if (in_place) {
         // a block that wraps past n would read lags already overwritten
@@ -654,7 +798,11 @@ mxr_status plan_pc(int64_t seff, int64_t L, int64_t n, int64_t nout, bool allow_
         if (last_lo + M > n && (last_lo + M - n) > 0 && seff > 0) return;
       }
     }
-    const double cost = (double)p.nblk * M * ilog2(M) * (M <= 16384 ? 1.0 : 1e9);
+    // relative cost per point: 1024..8192 have dedicated or multi-block
+    // kernels with several workgroups per CU; the 16384-point kernel holds
+    // one workgroup per CU and runs about half as fast per point (measured)
+    const double eff = M <= 8192 ? 1.0 : (M == 16384 ? 2.0 : 1e9);
+    const double cost = (double)p.nblk * M * ilog2(M) * eff;
     if (cost < best_cost || (cost == best_cost && M > best.M)) {
       best_cost = cost;
       best = p;
@@ -715,6 +863,18 @@ mxr_status launch_pc(const WfArgs *wa, const PcArgs *pa, hipStream_t s) {
   }
   MXR_HIP_TRY(hipGetLastError());
   if (!pa || pa->groups <= 0) return MXR_SUCCESS;
+  if constexpr (M == 8192) {
+    if (pa->bpw == 1) {
+      Pc8192Args p8{*pa, pow_base_device(8192, 2), pow_base_device(8192, 32),
+                    pow_base_device(8192, 512)};
+      MXR_REQUIRE(p8.bA && p8.bB && p8.bC, MXR_OUT_OF_MEMORY,
+                  "pulse compression: twiddle table allocation failed");
+      ProfScope ps("pc8192_kernel", s);
+      pc8192_kernel<<<dim3((unsigned)pa->groups), dim3(512), 0, s>>>(p8);
+      MXR_HIP_TRY(hipGetLastError());
+      return MXR_SUCCESS;
+    }
+  }
   if constexpr (M == 4096) {
     if (pa->bpw == 1) {
       Pc4096Args p4{*pa, pow_base_device(4096, 16), pow_base_device(4096, 256)};
@@ -903,6 +1063,9 @@ static mxr_status pc_plan_create(mxr_pc_plan *out, int64_t S, int64_t L, int64_t
   if (plan.M == 4096)
     MXR_REQUIRE(pow_base_device(4096, 16) && pow_base_device(4096, 256), MXR_OUT_OF_MEMORY,
                 "pc_plan_create: twiddle table allocation failed");
+  if (plan.M == 8192)
+    MXR_REQUIRE(pow_base_device(8192, 2) && pow_base_device(8192, 32) && pow_base_device(8192, 512),
+                MXR_OUT_OF_MEMORY, "pc_plan_create: twiddle table allocation failed");
   float2 *wc = nullptr;
   MXR_HIP_TRY(hipMalloc(&wc, sizeof(float2) * plan.M));
   *out = reinterpret_cast<mxr_pc_plan>(new PcPlanObj{device_id(), S, L, n, nout, flags, plan, wc, twf, twi});

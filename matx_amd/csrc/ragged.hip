@@ -42,6 +42,7 @@ struct PcClass {
   int grid = 0;
   PcJob *d_jobs = nullptr;
   WfArgs *d_wf = nullptr;
+  const int *d_item_job = nullptr;  // job (index within the class) of each item
 };
 
 struct DopClass {
@@ -50,6 +51,7 @@ struct DopClass {
   int64_t items = 0;
   int grid = 0;
   DopJob *d_jobs = nullptr;
+  const int *d_item_job = nullptr;
 };
 
 struct RaggedPlanObj {
@@ -127,12 +129,12 @@ mxr_status autotune_block(int64_t Lc, int64_t n, hipStream_t s, int *bestM) {
     }
     const int64_t items = (int64_t)rows * nblk;
     const int grid = (int)items;
-    if ((st = launch_pc_batch(M, dj, 1, items, grid, s))) break;  // warm-up
+    if ((st = launch_pc_batch(M, dj, nullptr, items, grid, s))) break;  // warm-up
     if ((e = hipEventRecord(e0, s)) != hipSuccess) {
       fail(e);
       break;
     }
-    for (int r = 0; r < 3 && st == MXR_SUCCESS; ++r) st = launch_pc_batch(M, dj, 1, items, grid, s);
+    for (int r = 0; r < 3 && st == MXR_SUCCESS; ++r) st = launch_pc_batch(M, dj, nullptr, items, grid, s);
     if (st) break;
     float ms = 0.f;
     if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
@@ -339,11 +341,28 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
     dop_ranges.push_back({first, hdop.size()});
     p->dop.push_back(cl);
   }
+  // item -> job tables (job index within its class), classes back to back
+  std::vector<int> hitem;
+  std::vector<size_t> pc_item0, dop_item0;
+  for (size_t c = 0; c < p->pc.size() && !st; ++c) {
+    pc_item0.push_back(hitem.size());
+    for (size_t k = pc_ranges[c].first; k < pc_ranges[c].second; ++k) {
+      const PcJob &pj = hpc[k];
+      hitem.insert(hitem.end(), (size_t)pj.C * pj.P * pj.nblk, (int)(k - pc_ranges[c].first));
+    }
+  }
+  for (size_t c = 0; c < p->dop.size() && !st; ++c) {
+    dop_item0.push_back(hitem.size());
+    for (size_t k = dop_ranges[c].first; k < dop_ranges[c].second; ++k) {
+      const DopJob &dj = hdop[k];
+      hitem.insert(hitem.end(), (size_t)dj.a.C * dj.tiles, (int)(k - dop_ranges[c].first));
+    }
+  }
   // one device allocation for every table
   const size_t bpc = sizeof(PcJob) * hpc.size(), bwf = sizeof(WfArgs) * hwf.size(),
-               bdop = sizeof(DopJob) * hdop.size();
+               bdop = sizeof(DopJob) * hdop.size(), bitem = sizeof(int) * hitem.size();
   if (!st) {
-    if ((e = hipMalloc(&p->tables, bpc + bwf + bdop + 64)) != hipSuccess) {
+    if ((e = hipMalloc(&p->tables, bpc + bwf + bdop + bitem + 64)) != hipSuccess) {
       set_error(MXR_OUT_OF_MEMORY, "%s: table allocation failed", fn);
       st = MXR_OUT_OF_MEMORY;
     }
@@ -353,17 +372,23 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
     PcJob *dpc = (PcJob *)base;
     WfArgs *dwf = (WfArgs *)(base + bpc);
     DopJob *ddop = (DopJob *)(base + bpc + bwf);
+    int *ditem = (int *)(base + bpc + bwf + bdop);
     if ((e = hipMemcpy(dpc, hpc.data(), bpc, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(dwf, hwf.data(), bwf, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(ddop, hdop.data(), bdop, hipMemcpyHostToDevice)) != hipSuccess) {
+        (e = hipMemcpy(ddop, hdop.data(), bdop, hipMemcpyHostToDevice)) != hipSuccess ||
+        (bitem && (e = hipMemcpy(ditem, hitem.data(), bitem, hipMemcpyHostToDevice)) != hipSuccess)) {
       set_error(MXR_HIP_ERROR, "%s: table upload failed: %s", fn, hipGetErrorString(e));
       st = MXR_HIP_ERROR;
     }
     for (size_t c = 0; c < p->pc.size(); ++c) {
       p->pc[c].d_jobs = dpc + pc_ranges[c].first;
       p->pc[c].d_wf = dwf + pc_ranges[c].first;
+      p->pc[c].d_item_job = ditem + pc_item0[c];
     }
-    for (size_t c = 0; c < p->dop.size(); ++c) p->dop[c].d_jobs = ddop + dop_ranges[c].first;
+    for (size_t c = 0; c < p->dop.size(); ++c) {
+      p->dop[c].d_jobs = ddop + dop_ranges[c].first;
+      p->dop[c].d_item_job = ditem + dop_item0[c];
+    }
   }
   if (st) {
     free_plan(p);
@@ -382,10 +407,10 @@ static mxr_status ragged_execute(mxr_ragged_plan h, hipStream_t s) {
   mxr_status st;
   for (const PcClass &c : p->pc) {
     if ((st = launch_wfspec_batch(c.M, c.d_wf, c.njobs, s))) return st;
-    if ((st = launch_pc_batch(c.M, c.d_jobs, c.njobs, c.items, c.grid, s))) return st;
+    if ((st = launch_pc_batch(c.M, c.d_jobs, c.d_item_job, c.items, c.grid, s))) return st;
   }
   for (const DopClass &c : p->dop)
-    if ((st = launch_doppler_batch(c.nd, c.d_jobs, c.njobs, c.items, c.grid, s))) return st;
+    if ((st = launch_doppler_batch(c.nd, c.d_jobs, c.d_item_job, c.items, c.grid, s))) return st;
   return MXR_SUCCESS;
 }
 

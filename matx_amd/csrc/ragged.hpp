@@ -82,10 +82,11 @@ __device__ __forceinline__ T *opaque(T *p) {
 mxr_status pc_plan_blocks(int64_t seff, int64_t L, int64_t n, int64_t nout, int M_req, int *M,
                           int64_t *V, int64_t *nblk);
 mxr_status launch_wfspec_batch(int M, const WfArgs *d_args, int njobs, hipStream_t s);
-mxr_status launch_pc_batch(int M, const PcJob *d_jobs, int njobs, int64_t items, int grid,
+mxr_status launch_pc_batch(int M, const PcJob *d_jobs, const int *d_item_job, int64_t items, int grid,
                            hipStream_t s);
 int pc_batch_grid(int M);
-mxr_status launch_doppler_batch(int nd, const DopJob *d_jobs, int njobs, int64_t items, int grid,
+mxr_status launch_doppler_batch(int nd, const DopJob *d_jobs, const int *d_item_job, int64_t items,
+                                int grid,
                                 hipStream_t s);
 int doppler_batch_tile(int nd);
 

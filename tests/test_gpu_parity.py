@@ -114,6 +114,10 @@ def test_fft_conv_config1(mx, orc):
     (16000, 64, 16384, 16384),    # overlap-save blocks that wrap circularly
     (3000, 700, 4096, 2301),
     (1500, 1500, 2048, 1),        # waveform as long as the pulse
+    (32768, 4096, 32768, 28673),  # M = 8192, 7 blocks of 4097 lags (not 64-aligned)
+    (12000, 1500, 16384, 10501),  # M = 8192, 64-aligned blocks
+    (16000, 3000, 16384, 16384),  # M = 8192 blocks that wrap circularly
+    (5000, 300, 8192, 8192),      # one 8192-point block, full circular output
 ])
 def test_fft_conv_shapes(mx, orc, S, L, n, nout):
     rng = np.random.default_rng(S + L)
