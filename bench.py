@@ -749,7 +749,7 @@ def bench_large_grad(args, cfg, x, w, targets):
     ms_f = timed(fwd_only, k)
     kern = {}
     if prof:
-        for name in ["pc8192_kernel", "pc_ols_kernel", "doppler_kernel", "doppler_adj_kernel", "pc_wgrad_kernel",
+        for name in ["pc8192_kernel", "pc_ols_kernel", "doppler_kernel", "doppler_adj_kernel", "wgrad8192_kernel", "pc_wgrad_kernel",
                      "wgrad_reduce_kernel", "wgrad_finish_kernel", "wfspec_kernel"]:
             cnt, tot = _lib.profile_query(name)
             if cnt:

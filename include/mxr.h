@@ -222,7 +222,9 @@ mxr_status mxr_doppler_taps(const mxr_tensor *pow_out, const mxr_tensor *rd_out,
  * t = window .* canceller(y), D = FFT_nd(t), s = window .* F^H(2 gpow .* D):
  *   gy[p] = s[p-o+1] - 2 s[p-o] + s[p-o-1]   (o = 1 for VALID, else 0)
  * or gy = s without a canceller.  D is recomputed from y (checkpointing:
- * no spectrum is stored between forward and backward). */
+ * no spectrum is stored between forward and backward).  Every tile is
+ * transformed (mxr_doppler_taps_adjoint, for sparse design losses, skips
+ * tiles whose gpow is all zero). */
 mxr_status mxr_doppler_adjoint(const mxr_tensor *gy, const mxr_tensor *gpow, const mxr_tensor *y,
                                int tpc_mode, int64_t win_rows, int64_t nd, hipStream_t stream);
 

@@ -204,6 +204,38 @@ void build_twiddles(int N, RadixSeq s, float2 *out);
 // device pointer owned by the library.
 const float2 *twiddles_device(int N, bool reversed_seq);
 
+// v[m] *= W^{mk}, m = 1..15, from one power-base row (W^k, W^2k, W^4k,
+// W^8k): the other powers are products of at most four of them (at most
+// three extra roundings).  inv conjugates (inverse transform).
+__device__ __forceinline__ void tw_from_base(float2 (&v)[16], float4 b0, float4 b1, bool inv) {
+  float2 w1 = make_float2(b0.x, b0.y), w2 = make_float2(b0.z, b0.w);
+  float2 w4 = make_float2(b1.x, b1.y), w8 = make_float2(b1.z, b1.w);
+  if (inv) {
+    w1.y = -w1.y;
+    w2.y = -w2.y;
+    w4.y = -w4.y;
+    w8.y = -w8.y;
+  }
+  float2 w[16];
+  w[1] = w1;
+  w[2] = w2;
+  w[4] = w4;
+  w[8] = w8;
+  w[3] = cmul(w2, w1);
+  w[5] = cmul(w4, w1);
+  w[6] = cmul(w4, w2);
+  w[7] = cmul(w[6], w1);
+  w[9] = cmul(w8, w1);
+  w[10] = cmul(w8, w2);
+  w[11] = cmul(w[10], w1);
+  w[12] = cmul(w8, w4);
+  w[13] = cmul(w[12], w1);
+  w[14] = cmul(w[12], w2);
+  w[15] = cmul(w[14], w1);
+#pragma unroll
+  for (int m = 1; m < 16; ++m) v[m] = cmul(v[m], w[m]);
+}
+
 // Per-device cache of radix-16 power-base rows for the pass with sub-length
 // Ls of an N-point transform: row k (k < Ls) holds W^k, W^2k, W^4k, W^8k with
 // W = e^{-2 pi i / (16 Ls)}, each from double precision rounded once.

@@ -120,6 +120,101 @@ __global__ __launch_bounds__(WgT<M>::NT) void pc_wgrad_kernel(WgArgs a) {
   }
 }
 
+// M = 8192: the transforms of the 8192-point pulse-compression kernel
+// (radix 2 x 16 x 16 x 16, 512 threads, the radix-2 pass folded into the
+// load, power-base twiddles for Ls = 2, 32, 512), so the x spectrum, the gy
+// spectrum and the running product take 16 bins each per thread (bins
+// j + 512q).  Those three live sets need more than the 128 VGPRs of two
+// workgroups per CU, so it runs one (up to 256 VGPRs, no spills).
+struct Wg8192Args {
+  WgArgs a;
+  const float4 *bA, *bB, *bC;
+};
+
+// Forward 8192-point transform of ld(i); bins j + 512q end in v[q].  Pre:
+// the workgroup's previous LDS reads are done.  Post: LDS is free (the last
+// exchange ends with a barrier).
+template <class LoadF>
+__device__ __forceinline__ void fwd8192_regs(float2 (&v)[16], LoadF &&ld, const float4 *pA,
+                                             const float4 *pB, const float4 *pC, float2 *lds) {
+  constexpr int NT = 512;
+  using LY = Layout<8192, 1, false>;
+  const int j = threadIdx.x;
+  auto xchg = [&](int ls_out) {
+    const int k = j & (ls_out - 1), base = (j - k) * 16 + k;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) lds[LY::addr(0, base + q * ls_out)] = v[q];
+    wg_barrier();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = lds[LY::addr(0, j + m * NT)];
+    wg_barrier();
+  };
+  {
+    const int jh = j >> 1;
+    const bool odd = (j & 1) != 0;
+    float2 a[16], b[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      a[m] = ld(jh + 256 * m);
+      b[m] = ld(jh + 256 * m + 4096);
+    }
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = odd ? csub(a[m], b[m]) : cadd(a[m], b[m]);
+  }
+  float4 t0 = pA[0], t1 = pA[1];
+  tw_from_base(v, t0, t1, false);  // Ls = 2
+  dft16<false>(v);
+  t0 = pB[0];
+  t1 = pB[1];
+  xchg(2);
+  tw_from_base(v, t0, t1, false);  // Ls = 32
+  dft16<false>(v);
+  t0 = pC[0];
+  t1 = pC[1];
+  xchg(32);
+  tw_from_base(v, t0, t1, false);  // Ls = 512
+  dft16<false>(v);
+}
+
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
+void wgrad8192_kernel(Wg8192Args p) {
+  using LY = Layout<8192, 1, false>;
+  __shared__ float2 lds[LY::LDS_ELEMS];
+  const WgArgs &a = p.a;
+  const int j = threadIdx.x;
+  const float4 *pA = p.bA + (j & 1) * 2, *pB = p.bB + (j & 31) * 2, *pC = p.bC + j * 2;
+  float2 acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = make_float2(0.f, 0.f);
+  for (int64_t it = blockIdx.x; it < a.items; it += gridDim.x) {
+    // (the previous item's last exchange ended with a barrier)
+    const int64_t row = it / a.nblk;
+    const int lo = (int)(it % a.nblk) * a.V;
+    int64_t ox, oy;
+    batch_offsets(a.bi, row, &ox, &oy);
+    const float2 *xr = a.x + ox;
+    const float2 *gr = a.gy + oy;
+    const int n32 = a.n, seff = a.seff;
+    const int lim = (a.nout - lo) < a.V ? (a.nout - lo) : a.V;
+    float2 X[16], Gv[16];
+    fwd8192_regs(X, [&](int i) {
+      int gi = lo + i;
+      if (gi >= n32) gi -= n32;
+      return gi < seff ? xr[gi] : make_float2(0.f, 0.f);
+    }, pA, pB, pC, lds);
+    fwd8192_regs(Gv, [&](int i) { return i < lim ? gr[lo + i] : make_float2(0.f, 0.f); }, pA, pB,
+                 pC, lds);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      acc[q].x += X[q].x * Gv[q].x + X[q].y * Gv[q].y;  // X conj(G)
+      acc[q].y += X[q].y * Gv[q].x - X[q].x * Gv[q].y;
+    }
+  }
+  float2 *pp = a.partial + (int64_t)blockIdx.x * 8192;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) pp[j + 512 * q] = acc[q];
+}
+
 // spec[k] = sum over workgroups g of partial[g][k], in order, in double.
 __global__ __launch_bounds__(64) void wgrad_reduce_kernel(const float2 *partial, int G, int M,
                                                           float2 *spec) {
@@ -208,9 +303,12 @@ static mxr_status wgrad_run(const WgradJob &j, hipStream_t s) {
   float2 *spec = nullptr;
   if (items > 0) {
     int per_cu = 1, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc_wgrad_kernel<M>, T::NT, 0) !=
-            hipSuccess || per_cu < 1)
-      per_cu = 1;
+    hipError_t oe;
+    if constexpr (M == 8192)
+      oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wgrad8192_kernel, 512, 0);
+    else
+      oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pc_wgrad_kernel<M>, T::NT, 0);
+    if (oe != hipSuccess || per_cu < 1) per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_id()) !=
             hipSuccess || cus < 1)
       cus = 256;
@@ -230,7 +328,13 @@ static mxr_status wgrad_run(const WgradJob &j, hipStream_t s) {
     a.V = (int)j.V;
     a.partial = ws;
     a.twf = twf;
-    {
+    if constexpr (M == 8192) {
+      Wg8192Args p{a, pow_base_device(8192, 2), pow_base_device(8192, 32), pow_base_device(8192, 512)};
+      MXR_REQUIRE(p.bA && p.bB && p.bC, MXR_OUT_OF_MEMORY,
+                  "pc waveform gradient: twiddle table allocation failed");
+      ProfScope ps("wgrad8192_kernel", s);
+      wgrad8192_kernel<<<dim3((unsigned)grid), dim3(512), 0, s>>>(p);
+    } else {
       ProfScope ps("pc_wgrad_kernel", s);
       pc_wgrad_kernel<M><<<dim3((unsigned)grid), dim3(T::NT), 0, s>>>(a);
     }

@@ -248,34 +248,6 @@ struct Pc4096Args {
   const float4 *base2;  // [256][2] float4: pass Ls = 256
 };
 
-__device__ __forceinline__ void tw_from_base(float2 (&v)[16], float4 b0, float4 b1, bool inv) {
-  float2 w1 = make_float2(b0.x, b0.y), w2 = make_float2(b0.z, b0.w);
-  float2 w4 = make_float2(b1.x, b1.y), w8 = make_float2(b1.z, b1.w);
-  if (inv) {
-    w1.y = -w1.y;
-    w2.y = -w2.y;
-    w4.y = -w4.y;
-    w8.y = -w8.y;
-  }
-  float2 w[16];
-  w[1] = w1;
-  w[2] = w2;
-  w[4] = w4;
-  w[8] = w8;
-  w[3] = cmul(w2, w1);
-  w[5] = cmul(w4, w1);
-  w[6] = cmul(w4, w2);
-  w[7] = cmul(w[6], w1);
-  w[9] = cmul(w8, w1);
-  w[10] = cmul(w8, w2);
-  w[11] = cmul(w[10], w1);
-  w[12] = cmul(w8, w4);
-  w[13] = cmul(w[12], w1);
-  w[14] = cmul(w[12], w2);
-  w[15] = cmul(w[14], w1);
-#pragma unroll
-  for (int m = 1; m < 16; ++m) v[m] = cmul(v[m], w[m]);
-}
 
 // One M = 4096 overlap-save block (256 threads); lds holds 4096 padded
 // elements.  Pre: the workgroup's previous use of lds is finished.
@@ -304,12 +276,20 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
   const float4 *b1p = base1 + (j & 15) * 2;
   const float4 *b2p = base2 + j * 2;
 
-  // forward pass 0 (Ls = 1) straight from HBM (zero padding and wrap implicit)
+  // forward pass 0 (Ls = 1) straight from HBM (zero padding and wrap
+  // implicit); a block inside the row (the common case) loads without
+  // per-element checks
+  if (lo + 4096 <= seff32 && lo + 4096 <= n32) {
+    const float2 *xb = xr + lo + j;
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    int gi = lo + j + m * NT;
-    if (gi >= n32) gi -= n32;
-    v[m] = gi < seff32 ? xr[gi] : make_float2(0.f, 0.f);
+    for (int m = 0; m < 16; ++m) v[m] = xb[m * NT];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      int gi = lo + j + m * NT;
+      if (gi >= n32) gi -= n32;
+      v[m] = gi < seff32 ? xr[gi] : make_float2(0.f, 0.f);
+    }
   }
   dft16<false>(v);
   float4 t0 = b1p[0], t1 = b1p[1];
@@ -334,11 +314,14 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
   xchg(16);
   tw_from_base(v, t0, t1, true);   // inverse pass 2 (Ls = 256)
   dft16<true>(v);
+  // lag j + 256q is kept when below lim: whole rows q < lim/256 are a
+  // uniform test, only row lim/256 compares per lane
   const int lim = (ob.nout - lo) < ob.V ? (ob.nout - lo) : ob.V;
+  const int qn = lim >> 8, rem = lim & 255;
+  float2 *yb = yr + lo + j;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
-    const int i = j + q * 256;
-    if (i < lim) yr[lo + i] = v[q];  // the spectrum carries the 1/M
+    if (q < qn || (q == qn && j < rem)) yb[q * 256] = v[q];  // the spectrum carries the 1/M
   }
 }
 
@@ -401,15 +384,25 @@ __device__ __forceinline__ void ols_block8192(const OlsBlock ob, const float4 *b
   const float4 *pB = bB + (j & 31) * 2;
   const float4 *pC = bC + j * 2;
 
-  // forward radix-2 pass folded into the load
+  // forward radix-2 pass folded into the load (no per-element checks for a
+  // block inside the row)
   {
     const int jh = j >> 1;
     const bool odd = (j & 1) != 0;
     float2 a[16], b[16];
+    if (lo + 8192 <= seff32 && lo + 8192 <= n32) {
+      const float2 *xb = xr + lo + jh;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      a[m] = ldx(jh + 256 * m);
-      b[m] = ldx(jh + 256 * m + 4096);
+      for (int m = 0; m < 16; ++m) {
+        a[m] = xb[256 * m];
+        b[m] = xb[256 * m + 4096];
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        a[m] = ldx(jh + 256 * m);
+        b[m] = ldx(jh + 256 * m + 4096);
+      }
     }
 #pragma unroll
     for (int m = 0; m < 16; ++m) v[m] = odd ? csub(a[m], b[m]) : cadd(a[m], b[m]);
@@ -455,10 +448,11 @@ __device__ __forceinline__ void ols_block8192(const OlsBlock ob, const float4 *b
   tw_from_base(v, t0, t1, true);  // inverse pass Ls = 512: lags j + 512q
   dft16<true>(v);
   const int lim = (ob.nout - lo) < ob.V ? (ob.nout - lo) : ob.V;
+  const int qn = lim >> 9, rem = lim & 511;
+  float2 *yb = yr + lo + j;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
-    const int i = j + q * 512;
-    if (i < lim) yr[lo + i] = v[q];  // the spectrum carries the 1/M
+    if (q < qn || (q == qn && j < rem)) yb[q * 512] = v[q];  // the spectrum carries the 1/M
   }
 }
 
