@@ -613,10 +613,12 @@ def bench_pipeline(args, cfg):
               "pc_ols_kernel": 8 * C * P * S + 8 * C * P * R,
               "dopcfar_kernel": 8 * C * P * R + 4 * C * nd * R,
               "doppler_kernel": 8 * C * P * R + 4 * C * nd * R,
-              "cfar_kernel": 4 * C * nd * R + 4 * C * nd * R}
+              "cfar_kernel": 4 * C * nd * R + 4 * C * nd * R,
+              "cfar_lds_kernel": 4 * C * nd * R + 4 * C * nd * R}
     if fused:
         kbytes.pop("doppler_kernel")
         kbytes.pop("cfar_kernel")   # the name also matches dopcfar_kernel
+        kbytes.pop("cfar_lds_kernel")
     kern = {}
     if prof:
         for k in ["wfspec_kernel"] + list(kbytes):

@@ -241,6 +241,152 @@ __global__ __launch_bounds__(CF_NT) void cfar_kernel(CfarArgs a) {
   }
 }
 
+// LDS-staged form for an f32 power map with unit column stride and 16-byte
+// aligned rows (the pulse-axis pass's output).  A 256-thread workgroup
+// stages a tile of CL_TY + 2HY rows x 256 + 2*HALO columns with coalesced
+// 16-byte loads (zero outside the map), then each wave walks one band of
+// CL_RB output rows with a lane per 4 adjacent columns, reading its window
+// from LDS.  The register-only form above keeps two rows of halo reads and a
+// ring of centre values per lane, which at a 21-bin window fills 256 VGPRs
+// (one wave per SIMD); here the live set is one LDS row and the ring of
+// partial sums.
+constexpr int CL_NT = 256;
+constexpr int CL_TX = 256;            // output columns per tile (a quad per lane)
+constexpr int CL_RB = 16;             // output rows per wave
+constexpr int CL_TY = 4 * CL_RB;      // output rows per tile
+
+template <int HY, int HX, int EY, int EX, bool EXACT>
+__global__ __launch_bounds__(CL_NT) void cfar_lds_kernel(CfarArgs a) {
+  constexpr int HALO = (HX + 3) & ~3;
+  constexpr int PS = CL_TX + 2 * HALO;  // tile row (floats), a multiple of 4
+  constexpr int ROWS = CL_TY + 2 * HY;
+  constexpr int QPR = PS / 4;
+  constexpr int NQ = 1 + 2 * HALO / 4;
+  constexpr int NV = 4 * NQ;
+  constexpr int NCELL = (2 * HY + 1) * (2 * HX + 1);
+  constexpr int RING = 2 * HY + 1;
+  constexpr int NLY = CL_RB + 2 * HY;  // tile rows a band walks
+  __shared__ __align__(16) float tile[ROWS * PS];
+  __shared__ float thr_tab[NCELL + 1];
+
+  const int c = blockIdx.z, y0 = blockIdx.y * CL_TY, x0 = blockIdx.x * CL_TX;
+  const int Nx = a.Nx, Ny = a.Ny;
+  for (int n = threadIdx.x; n <= NCELL; n += CL_NT) {
+    const float nf = (float)n;
+    const float alpha = nf * (powf(a.pfa, -1.0f / nf) - 1.0f);
+    thr_tab[n] = EXACT ? alpha : alpha / nf;
+  }
+  const float *in_c = reinterpret_cast<const float *>(a.in) + (int64_t)c * a.is_c;
+  const int isy = (int)a.is_y;
+  for (int qi = threadIdx.x; qi < ROWS * QPR; qi += CL_NT) {
+    const int r = qi / QPR, qc = qi - r * QPR;
+    const int gy = y0 - HY + r, gx = x0 - HALO + 4 * qc;
+    float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (gy >= 0 && gy < Ny) {
+      const float *rp = in_c + gy * isy;
+      if (gx >= 0 && gx + 3 < Nx) {
+        z = *reinterpret_cast<const float4 *>(rp + gx);
+      } else {
+        z.x = (gx >= 0 && gx < Nx) ? rp[gx] : 0.f;
+        z.y = (gx + 1 >= 0 && gx + 1 < Nx) ? rp[gx + 1] : 0.f;
+        z.z = (gx + 2 >= 0 && gx + 2 < Nx) ? rp[gx + 2] : 0.f;
+        z.w = (gx + 3 >= 0 && gx + 3 < Nx) ? rp[gx + 3] : 0.f;
+      }
+    }
+    *reinterpret_cast<float4 *>(tile + r * PS + 4 * qc) = z;
+  }
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int gx0 = x0 + 4 * lane;
+  const int k0 = wv * CL_RB;  // band's first output row, tile-relative
+  if (gx0 >= Nx || y0 + k0 >= Ny) return;  // no barrier follows
+  int co[4], chh[4];
+  float thr_int[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int gx = gx0 + j;
+    co[j] = min(gx + HX, Nx - 1) - max(gx - HX, 0) + 1;
+    chh[j] = min(gx + EX, Nx - 1) - max(gx - EX, 0) + 1;
+    thr_int[j] = thr_tab[min(max((2 * HY + 1) * co[j] - (2 * EY + 1) * chh[j], 0), NCELL)];
+  }
+  const bool vec_out = a.vec_dets && gx0 + 3 < Nx;
+  const int dsy = (int)a.ds_y, dsx = (int)a.ds_x;
+  int *const dets_c = a.dets + (int64_t)c * a.ds_c;
+  const float *col0 = tile + 4 * lane;  // u[i] of a row is column gx0 - HALO + i
+
+  float full[RING][4], lr[RING][4];
+#pragma unroll 1
+  for (int ly0 = 0; ly0 < NLY; ly0 += RING)
+#pragma unroll
+  for (int sl = 0; sl < RING; ++sl) {
+    const int ly = ly0 + sl;
+    if (ly >= NLY) break;
+    const float *row = col0 + (k0 + ly) * PS;
+    float u[NV];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const float4 z = *reinterpret_cast<const float4 *>(row + 4 * q);
+      u[4 * q] = z.x;
+      u[4 * q + 1] = z.y;
+      u[4 * q + 2] = z.z;
+      u[4 * q + 3] = z.w;
+    }
+    float L[4], Mh[4], Rt[4];
+    win4<HX - EX>(u + HALO - HX, L);
+    win4<2 * EX + 1>(u + HALO - EX, Mh);
+    win4<HX - EX>(u + HALO + EX + 1, Rt);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      lr[sl][j] = L[j] + Rt[j];
+      full[sl][j] = lr[sl][j] + Mh[j];
+    }
+    if (ly < 2 * HY) continue;
+    const int gy = y0 + k0 + ly - 2 * HY;  // row of the cells under test
+    if (gy >= Ny) break;
+    float thr[4];
+    int nrm[4];
+    const bool edge = gy < HY || gy + HY >= Ny;
+    const int ro = min(gy + HY, Ny - 1) - max(gy - HY, 0) + 1;
+    const int rh = min(gy + EY, Ny - 1) - max(gy - EY, 0) + 1;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      nrm[j] = ro * co[j] - rh * chh[j];
+      thr[j] = edge ? thr_tab[min(max(nrm[j], 0), NCELL)] : thr_int[j];
+    }
+    const float4 cut = *reinterpret_cast<const float4 *>(col0 + (k0 + ly - HY) * PS + HALO);
+    const float cv[4] = {cut.x, cut.y, cut.z, cut.w};
+    int det[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float sm = 0.f;
+#pragma unroll
+      for (int dy = -HY; dy <= HY; ++dy) {
+        const int kk = (sl - HY + dy + 2 * RING) % RING;  // ring slot of row ly - HY + dy
+        const float t = (dy >= -EY && dy <= EY) ? lr[kk][j] : full[kk][j];
+        sm = dy == -HY ? t : sm + t;
+      }
+      if constexpr (EXACT) {
+        const float b = sm / (float)nrm[j];
+        det[j] = cv[j] > thr[j] * b ? 1 : 0;
+        if (gx0 + j < Nx)
+          a.ba[(int64_t)c * a.bs_c + (int64_t)gy * a.bs_y + (int64_t)(gx0 + j) * a.bs_x] = b;
+      } else {
+        det[j] = cv[j] > thr[j] * sm ? 1 : 0;
+      }
+    }
+    int *drow = dets_c + gy * dsy;
+    if (vec_out) {
+      *reinterpret_cast<int4 *>(drow + gx0) = make_int4(det[0], det[1], det[2], det[3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (gx0 + j < Nx) drow[(gx0 + j) * dsx] = det[j];
+    }
+  }
+}
+
 template <int HY, int HX, int EY, int EX, bool CIN, bool EXACT, bool WPOW>
 static void launch_cfar3(const CfarArgs &a, dim3 grid, hipStream_t s) {
   cfar_kernel<HY, HX, EY, EX, CIN, EXACT, WPOW><<<grid, dim3(CF_NT), 0, s>>>(a);
@@ -250,6 +396,16 @@ template <int HY, int HX, int EY, int EX>
 static mxr_status launch_cfar(const CfarArgs &a, hipStream_t s) {
   static_assert(HX <= 16 && HY <= 10, "window");
   dim3 grid((a.Nx + CF_TX - 1) / CF_TX, (a.Ny + CF_TY - 1) / CF_TY, a.C);
+  if constexpr (HY <= 4) {
+    if (!a.in_c64 && a.vec_in && a.pwo == nullptr) {
+      dim3 g2((a.Nx + CL_TX - 1) / CL_TX, (a.Ny + CL_TY - 1) / CL_TY, a.C);
+      ProfScope ps("cfar_lds_kernel", s);
+      if (a.ba) cfar_lds_kernel<HY, HX, EY, EX, true><<<g2, dim3(CL_NT), 0, s>>>(a);
+      else cfar_lds_kernel<HY, HX, EY, EX, false><<<g2, dim3(CL_NT), 0, s>>>(a);
+      MXR_HIP_TRY(hipGetLastError());
+      return MXR_SUCCESS;
+    }
+  }
   {
     ProfScope ps("cfar_kernel", s);
     const bool ex = a.ba != nullptr, wp = a.pwo != nullptr;

@@ -208,6 +208,40 @@ def test_cfar_complex_input_and_edges(mx, orc):
     assert bad == 0
 
 
+@pytest.mark.parametrize("C,Ny,Nx,win,guard,aligned", [
+    (2, 512, 3001, (7, 21), (3, 5), True),    # the large cube's window, ragged last tile
+    (3, 40, 777, (5, 13), (3, 5), True),      # fewer rows than one tile band
+    (1, 130, 1024, (5, 13), (3, 3), True),    # MatX mask
+    (2, 70, 513, (7, 21), (3, 5), False),     # unaligned rows: the register-only kernel
+])
+def test_cfar_lds_tiles(mx, orc, C, Ny, Nx, win, guard, aligned):
+    """The LDS-staged CFAR (f32 map, 16-byte aligned rows) and the
+    register-only one (unaligned rows) against the FP64 oracle, with the
+    background averages and detections near strong cells and map edges."""
+    rng = np.random.default_rng(Ny + Nx)
+    pw = rng.exponential(1.0, (C, Ny, Nx)).astype(np.float32)
+    for _ in range(60):
+        pw[rng.integers(C), rng.integers(Ny), rng.integers(Nx)] *= rng.uniform(5, 1e5)
+    pw[0, 0, 0] *= 1e4
+    pw[-1, -1, -1] *= 1e4
+    mask = np.ones(win)
+    cy, cx, gy, gx = win[0] // 2, win[1] // 2, guard[0] // 2, guard[1] // 2
+    mask[cy - gy:cy + gy + 1, cx - gx:cx + gx + 1] = 0
+    ref = orc.cfar(pw.astype(np.float64), mask)
+    ld = (Nx + 63) // 64 * 64 if aligned else Nx + 1
+    buf = torch.zeros((C, Ny, ld), dtype=torch.float32, device="cuda")
+    buf[:, :, :Nx] = dev(pw, torch.float32)
+    view = buf[:, :, :Nx]
+    dets, ba, _ = mx.cfar(view, win=win, guard=guard, want_ba=True)
+    assert rel(host(ba), ref["ba"]) < RTOL
+    bad, _ = dets_match(host(dets), ref)
+    assert bad == 0
+    d2, _, _ = mx.cfar(view, win=win, guard=guard)   # detection-only form (tabulated alpha/norm)
+    bad, _ = dets_match(host(d2), ref)
+    assert bad == 0
+    assert host(dets).sum() > 0
+
+
 # ------------------------------------------------------------------ conv
 def test_conv_kats_gpu(mx):
     import json
