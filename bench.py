@@ -135,6 +135,56 @@ def cpu_baseline_leg(x0, w, pow0, dets0, nd, budget_s, C, mask):
     return cpu, parity
 
 
+def cpu_standin_leg(x0, w, nd, win, guard, budget_s, C):
+    """MatX's host-executor dataflow (HostExecutor + FFTW, which cannot be
+    built here) restated stage by stage with torch on the CPU: FP32, MKL
+    FFTs, all host threads.  Padded N-point FFT, multiply by the conjugate
+    waveform spectrum, inverse FFT, three-pulse canceller (valid), window,
+    Doppler FFT, |.|^2, window sums and cell counts as 2-D convolutions with
+    the CFAR mask, average and threshold.  Channel 0 of the cube, scaled to
+    cubes/s."""
+    import torch
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    P, S = x0.shape
+    L = len(w)
+    R = S - L + 1
+    N = 1
+    while N < S:
+        N *= 2
+    X = torch.from_numpy(x0)
+    wt = torch.from_numpy(w)
+    hl = torch.hamming_window(L, periodic=False, dtype=torch.float32)
+    hp = torch.hamming_window(P - 2, periodic=False, dtype=torch.float32)[:, None]
+    mask = torch.tensor(cfar_mask(win, guard), dtype=torch.float32)[None, None]
+    pad = (win[0] // 2, win[1] // 2)
+    nrm = torch.nn.functional.conv2d(torch.ones((1, 1, nd, R)), mask, padding=pad)[0, 0]
+    alpha = nrm * (1e-5 ** (-1.0 / nrm) - 1.0)
+
+    def cube():
+        w1 = wt * hl
+        Wc = torch.conj(torch.fft.fft(w1 / torch.sqrt((w1.abs() ** 2).sum()), N))
+        y = torch.fft.ifft(torch.fft.fft(X, N, dim=-1) * Wc, dim=-1)[:, :R]
+        t = (y[:-2] - 2 * y[1:-1] + y[2:]) * hp
+        D = torch.fft.fft(t, nd, dim=0)
+        pw = D.real ** 2 + D.imag ** 2
+        s = torch.nn.functional.conv2d(pw[None, None], mask, padding=pad)[0, 0]
+        return pw > alpha * (s / nrm)
+
+    cube()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        cube()
+        reps += 1
+        if time.perf_counter() - t0 >= budget_s or reps >= 200:
+            break
+    el = time.perf_counter() - t0
+    return {"value": round(reps / C / el, 3), "unit": "cubes/s", "cores": cores,
+            "kind": "port (stand-in for MatX HostExecutor + FFTW)",
+            "sample": f"channel 0 of the cube (1/{C}), torch CPU FP32 with MKL FFTs, "
+                      f"{reps} runs in {el:.1f} s"}
+
+
 def load_traffic(kernel, workload="radar"):
     tag = {"radar": "radar16x128x9000", "large": "radar64x512x32768"}[workload]
     p = os.path.join(ROOT, "profiles", f"{kernel}_{tag}_traffic.json")
@@ -651,6 +701,10 @@ def bench_pipeline(args, cfg):
         cpu, parity = cpu_baseline_leg(x[0].cpu().numpy(), w.cpu().numpy(),
                                        pipe.pw[0].cpu().numpy(), dets[0].cpu().numpy(), nd,
                                        args.cpu_seconds, C, cfar_mask(win, guard))
+    standin = None
+    if args.cpu_seconds > 0 and args.workload == "radar":
+        standin = cpu_standin_leg(x[0].cpu().numpy(), w.cpu().numpy(), nd, win, guard,
+                                  args.cpu_seconds, C)
     fwd_bwd = None
     if args.workload == "large":
         del pipe
@@ -685,6 +739,7 @@ def bench_pipeline(args, cfg):
                    "detections": det_cells, "targets_detected": f"{hit}/{len(targets)}",
                    "kernel_rooflines": {k: v for k, v in rooflines.items() if k != dominant},
                    "parity": parity,
+                   **({"cpu_matx_host_standin": standin} if standin else {}),
                    **({"fwd_bwd": fwd_bwd} if fwd_bwd else {})},
         "roofline": roofline,
         "cpu_baseline": cpu,
