@@ -31,10 +31,9 @@ struct DopCfg {
 
 // Coalesced load of a pulse tile: rows p < P of TR range bins from r0 into
 // lds[p * TR + t] (zero past R), 16 loads in flight per lane.
-template <int ND, int TR = DopCfg<ND>::TR>
+template <int ND, int TR = DopCfg<ND>::TR, int NT = DopCfg<ND>::NT>
 __device__ __forceinline__ void load_pulse_tile(const float2 *yc, int64_t ys_p, int64_t ys_r,
                                                 int P, int R, int r0, float2 *lds) {
-  constexpr int NT = DopCfg<ND>::NT;
   constexpr int RPI = NT / TR;  // pulses covered by one sweep of the workgroup
   constexpr int BATCH = 16;
   const int t = threadIdx.x % TR, p0 = threadIdx.x / TR;
@@ -59,10 +58,9 @@ __device__ __forceinline__ void load_pulse_tile(const float2 *yc, int64_t ys_p, 
 // One tile: TR range bins from r0 of channel c.  lds holds rows_buf * TR
 // elements, win ND floats.  Pre: the workgroup's previous use of both is
 // finished.
-template <int ND, int TR = DopCfg<ND>::TR>
+template <int ND, int TR = DopCfg<ND>::TR, int NT = DopCfg<ND>::NT>
 __device__ __forceinline__ void doppler_tile(const DopArgs &a, int c, int r0, int rows_buf,
                                              float2 *lds, float *win) {
-  constexpr int NT = DopCfg<ND>::NT;
   using LY = Layout<ND, TR, true>;
   using SQ = Seq<ND, false>;
   constexpr int K = SQ::n;
@@ -72,7 +70,7 @@ __device__ __forceinline__ void doppler_tile(const DopArgs &a, int c, int r0, in
       h = a.taps ? a.taps[(int64_t)i * a.taps_s] : (i < a.win_rows ? hamming_f32(i, a.win_rows) : 1.0f);
     win[i] = h;
   }
-  load_pulse_tile<ND, TR>(a.y + (int64_t)c * a.ys_c, a.ys_p, a.ys_r, a.P, a.R, r0, lds);
+  load_pulse_tile<ND, TR, NT>(a.y + (int64_t)c * a.ys_c, a.ys_p, a.ys_r, a.P, a.R, r0, lds);
   wg_barrier();
 
   auto tpc = [&](int t, int i) -> float2 {
@@ -130,11 +128,14 @@ __global__ __launch_bounds__(DopCfg<ND>::NT) void doppler_kernel(DopArgs a, int 
 template <int ND>
 struct DopBatchCfg {
   static constexpr int TR = ND <= 256 ? 32 : DopCfg<ND>::TR;
+  // 512 threads once the first radix-16 pass has 512 butterflies (256 x 32)
+  static constexpr int NT = ND >= 256 ? 512 : 256;
 };
 
 template <int ND, bool PERSIST>
-__global__ __launch_bounds__(DopCfg<ND>::NT) void doppler_batch_kernel(const DopJob *jobs, const int *item_job,
-                                                            int64_t items) {
+__global__ __launch_bounds__(DopBatchCfg<ND>::NT) void doppler_batch_kernel(const DopJob *jobs,
+                                                                          const int *item_job,
+                                                                          int64_t items) {
   constexpr int TR = DopBatchCfg<ND>::TR;
   __shared__ float2 lds[ND * TR];
   __shared__ float win[ND];
@@ -142,7 +143,8 @@ __global__ __launch_bounds__(DopCfg<ND>::NT) void doppler_batch_kernel(const Dop
     const int ji = item_job ? __builtin_amdgcn_readfirstlane(item_job[it]) : 0;
     const DopJob jb = uniform_copy(&jobs[ji]);
     const int64_t local = it - jb.item0;
-    doppler_tile<ND, TR>(jb.a, (int)(local / jb.tiles), (int)(local % jb.tiles) * TR, ND, lds, win);
+    doppler_tile<ND, TR, DopBatchCfg<ND>::NT>(jb.a, (int)(local / jb.tiles), (int)(local % jb.tiles) * TR,
+                                             ND, lds, win);
   };
   if constexpr (!PERSIST) {
     item(blockIdx.x);
@@ -164,8 +166,8 @@ mxr_status launch_doppler_batch(int nd, const DopJob *jobs, const int *item_job,
   switch (nd) {
 #define MXR_DB(N)                                                                          \
   case N:                                                                                  \
-    if (persist) doppler_batch_kernel<N, true><<<dim3(grid), dim3(DopCfg<N>::NT), 0, s>>>(jobs, item_job, items); \
-    else doppler_batch_kernel<N, false><<<dim3(grid), dim3(DopCfg<N>::NT), 0, s>>>(jobs, item_job, items); \
+    if (persist) doppler_batch_kernel<N, true><<<dim3(grid), dim3(DopBatchCfg<N>::NT), 0, s>>>(jobs, item_job, items); \
+    else doppler_batch_kernel<N, false><<<dim3(grid), dim3(DopBatchCfg<N>::NT), 0, s>>>(jobs, item_job, items); \
     break;
     MXR_DB(2)
     MXR_DB(4)
