@@ -90,8 +90,12 @@ class RadarPower(torch.autograd.Function):
     def forward(ctx, w, x, plan, nd: int, tpc: str, win_rows):
         w = w.contiguous()
         plan.set_waveform(w)
-        y = plan(x)
-        pw, _ = ops.doppler(y, nd, tpc=tpc, win_rows=win_rows)
+        # compressed cube and power map on row-padded storage (aligned strips
+        # for the pulse-axis kernels, forward and adjoint)
+        y = plan(x, out=ops.empty_rows(x.shape[:-1] + (plan.nout,), torch.complex64, x.device))
+        C, _, R = y.shape
+        pw, _ = ops.doppler(y, nd, tpc=tpc, win_rows=win_rows,
+                            pw=ops.empty_rows((C, nd, R), torch.float32, y.device))
         ctx.save_for_backward(w, x, y)
         ctx.plan, ctx.nd, ctx.tpc, ctx.win_rows = plan, nd, tpc, win_rows
         return pw
@@ -99,7 +103,8 @@ class RadarPower(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gpow):
         w, x, y = ctx.saved_tensors
-        gy = ops.doppler_adjoint(gpow, y, ctx.nd, tpc=ctx.tpc, win_rows=ctx.win_rows)
+        gy = ops.doppler_adjoint(gpow, y, ctx.nd, tpc=ctx.tpc, win_rows=ctx.win_rows,
+                                 gy=ops.empty_rows(y.shape, y.dtype, y.device))
         gw = ctx.plan.waveform_grad(gy, x, w)
         return gw, None, None, None, None, None
 
@@ -109,14 +114,16 @@ class WeightedPowerSum(torch.autograd.Function):
     def forward(ctx, pw, wmap):
         ctx.save_for_backward(wmap)
         ctx.shape = pw.shape
-        C = pw.shape[0]
-        # one matrix-vector product over the map (no [C, nd, R] temporary)
-        return torch.mv(pw.reshape(C, -1), wmap.reshape(-1)).sum()
+        # channel sum first ([nd, R] temporary): one pass over the map in
+        # place through its strides (row-padded maps included)
+        return (pw.sum(0) * wmap).sum()
 
     @staticmethod
     def backward(ctx, g):
         (wmap,) = ctx.saved_tensors
-        return (g * wmap).expand(ctx.shape), None
+        gm = ops.empty_rows(wmap.shape, wmap.dtype, wmap.device)
+        torch.mul(wmap, g, out=gm)
+        return gm.expand(ctx.shape), None
 
 
 def radar_power(w, x, plan, nd: int, tpc: str = "valid", win_rows=None):

@@ -22,6 +22,18 @@ def _mode(m):
     return _MODES[m] if isinstance(m, str) else int(m)
 
 
+def empty_rows(shape, dtype, device, align: int = 64):
+    """Uninitialised tensor of `shape` whose rows (last dimension) start every
+    roundup(shape[-1], align) elements: a view into a padded allocation, so
+    that each row begins on a 512-byte (c64) or 256-byte (f32) boundary.  The
+    pulse-axis kernels read and write 16-range-bin column strips; on rows of
+    odd length every strip straddles two cache lines."""
+    shape = tuple(int(s) for s in shape)
+    n = shape[-1]
+    ld = (n + align - 1) // align * align
+    return torch.empty(shape[:-1] + (ld,), dtype=dtype, device=device)[..., :n]
+
+
 def _fft(x, n, axis, norm, out, inverse, stream):
     ax = axis % x.dim()
     n = int(n) if n else x.shape[ax]
