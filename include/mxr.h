@@ -89,7 +89,10 @@ mxr_status mxr_profile_query(const char *name, int64_t *count, double *total_ms)
  * n = transform size; 0 means in->shape[axis].  The input is zero-padded or
  * truncated to n, and out->shape[axis] must equal n; every other dimension of
  * in and out must match (batch dims).  Any strides; in == out allowed.
- * Power-of-two n from 1 to 16384.
+ * Power-of-two n from 1 to 16384 (Stockham radix-16/8/4/2 passes in LDS);
+ * any other n from 3 to 8192 through Bluestein's chirp-z identity (one
+ * power-of-two convolution of length >= 2n - 1 in LDS; cuFFT's arbitrary
+ * sizes, which MatX inherits, are mixed-radix).
  * Replaces fft_impl / ifft_impl (include/matx/transforms/fft/fft_cuda.h:680-774,
  * size rules include/matx/operators/fft.h:111-199 and fft_common.h:39-140,
  * normalisation fft_cuda.h:126-188). */

@@ -10,6 +10,11 @@
 // pulse axis of a permuted view) the gather/scatter walk that dim fastest so
 // HBM accesses stay coalesced.
 #include <cmath>
+#include <complex>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
 
 #include "fft_engine.hpp"
 #include "mxr_cache.hpp"
@@ -129,6 +134,211 @@ static mxr_status dispatch_generic(int64_t n, const FftArgs &a, bool inv, hipStr
   return MXR_NOT_SUPPORTED;
 }
 
+// ------------------------------------------------- other sizes: Bluestein
+// An n-point DFT of any n as a circular convolution of power-of-two length
+// M >= 2n - 1 (Bluestein's chirp-z identity nk = (n^2 + k^2 - (k - n)^2)/2):
+//   X[k] = c[k] * sum_j (x[j] c[j]) h[k - j],  c[j] = exp(-i pi j^2 / n),
+//   h[m] = conj(c[|m|]),
+// with the convolution done as IFFT_M(FFT_M(x.c) . H), H = FFT_M(h_ext)/M
+// and h_ext[m] = h[m] (m < n), h_ext[M - m] = h[m] (0 < m < n), zero between.
+// The inverse DFT is conj(DFT(conj(x))): same tables, conjugated in and out.
+// One workgroup holds TB whole M-point sequences in LDS; gather, forward
+// passes, spectrum multiply, inverse passes and scatter all stay there.
+struct CztArgs {
+  FftArgs f;
+  int n;               // transform size
+  const float2 *chirp; // c[0..n)
+  const float2 *spec;  // H[0..M), scaled by 1/M
+};
+
+template <int M, bool INV>
+__global__ __launch_bounds__(GenericCfg<M>::NT) void czt_kernel(CztArgs z) {
+  constexpr int TB = GenericCfg<M>::TB;
+  constexpr int NT = GenericCfg<M>::NT;
+  using LY = Layout<M, TB, false>;
+  using SQ = Seq<M, false>;
+  __shared__ float2 lds[LY::LDS_ELEMS];
+  __shared__ int64_t base_in[TB], base_out[TB];
+  const FftArgs &a = z.f;
+  const int n = z.n;
+
+  const int64_t t0 = (int64_t)blockIdx.x * TB;
+  for (int t = threadIdx.x; t < TB; t += NT) {
+    int64_t oi = 0, oo = 0;
+    if (t0 + t < a.nbatch) batch_offsets(a.bi, t0 + t, &oi, &oo);
+    base_in[t] = oi;
+    base_out[t] = oo;
+  }
+  __syncthreads();
+
+  // gather x.c into the first n slots (x conjugated for the inverse), zeros
+  // to M
+  for (int e = threadIdx.x; e < TB * M; e += NT) {
+    int t, i;
+    if (a.batch_fast) {
+      t = e % TB;
+      i = e / TB;
+    } else {
+      t = e / M;
+      i = e % M;
+    }
+    float2 v = make_float2(0.f, 0.f);
+    if (t0 + t < a.nbatch && i < a.in_len) {
+      float2 x = a.in[base_in[t] + (int64_t)i * a.istride];
+      if (INV) x.y = -x.y;
+      v = cmul(x, z.chirp[i]);
+    }
+    lds[LY::addr(t, i)] = v;
+  }
+  __syncthreads();
+  lds_passes<LY, SQ, M, NT, TB, false, 0, SQ::n>(lds, a.tw);
+  for (int e = threadIdx.x; e < TB * M; e += NT) {
+    const int t = e / M, k = e % M;
+    lds[LY::addr(t, k)] = cmul(lds[LY::addr(t, k)], z.spec[k]);
+  }
+  __syncthreads();
+  lds_passes<LY, SQ, M, NT, TB, true, 0, SQ::n>(lds, a.tw);
+
+  // scatter c.y for k < n with the norm scale (conjugated for the inverse)
+  for (int e = threadIdx.x; e < TB * n; e += NT) {
+    int t, k;
+    if (a.batch_fast) {
+      t = e % TB;
+      k = e / TB;
+    } else {
+      t = e / n;
+      k = e % n;
+    }
+    if (t0 + t >= a.nbatch) continue;
+    float2 y = cscale(cmul(lds[LY::addr(t, k)], z.chirp[k]), a.scale);
+    if (INV) y.y = -y.y;
+    a.out[base_out[t] + (int64_t)k * a.ostride] = y;
+  }
+}
+
+namespace {
+// In-place radix-2 DFT in double, exp(-2 pi i jk / m), m a power of two.
+void dft_pow2_double(std::vector<std::complex<double>> &v) {
+  const size_t m = v.size();
+  for (size_t i = 1, j = 0; i < m; ++i) {
+    size_t bit = m >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(v[i], v[j]);
+  }
+  const double two_pi = 6.283185307179586476925286766559;
+  for (size_t len = 2; len <= m; len <<= 1) {
+    for (size_t i = 0; i < m; i += len) {
+      for (size_t k = 0; k < len / 2; ++k) {
+        const double ang = -two_pi * (double)k / (double)len;
+        const std::complex<double> w(std::cos(ang), std::sin(ang));
+        const std::complex<double> u = v[i + k], t = w * v[i + k + len / 2];
+        v[i + k] = u + t;
+        v[i + k + len / 2] = u - t;
+      }
+    }
+  }
+}
+
+std::mutex g_czt_mu;
+std::map<std::pair<int, int>, float2 *> g_czt;  // (device, n) -> [chirp n | spec M]
+
+void clear_czt() {
+  std::lock_guard<std::mutex> g(g_czt_mu);
+  for (auto &kv : g_czt) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(kv.first.first);
+    (void)hipFree(kv.second);
+    (void)hipSetDevice(cur);
+  }
+  g_czt.clear();
+}
+CacheClearRegistrar g_reg_czt(clear_czt);
+}  // namespace
+
+// Chirp and spectrum tables of an n-point transform (M-point convolution),
+// built in double and rounded once; the chirp angle pi (j^2 mod 2n) / n is
+// reduced in integers first.  Device pointer owned by the library.
+static const float2 *czt_tables(int n, int M) {
+  const auto key = std::make_pair(device_id(), n);
+  std::lock_guard<std::mutex> g(g_czt_mu);
+  auto it = g_czt.find(key);
+  if (it != g_czt.end()) return it->second;
+  const double pi = 3.14159265358979323846264338327950288;
+  std::vector<std::complex<double>> c((size_t)n), h((size_t)M, 0.0);
+  for (int64_t j = 0; j < n; ++j) {
+    const int64_t e = (j * j) % (2 * (int64_t)n);
+    const double ang = -pi * (double)e / (double)n;
+    c[(size_t)j] = std::complex<double>(std::cos(ang), std::sin(ang));
+  }
+  h[0] = std::conj(c[0]);
+  for (int j = 1; j < n; ++j) h[(size_t)j] = h[(size_t)(M - j)] = std::conj(c[(size_t)j]);
+  dft_pow2_double(h);
+  std::vector<float2> host((size_t)(n + M));
+  for (int j = 0; j < n; ++j)
+    host[(size_t)j] = make_float2((float)c[(size_t)j].real(), (float)c[(size_t)j].imag());
+  for (int k = 0; k < M; ++k)
+    host[(size_t)(n + k)] = make_float2((float)(h[(size_t)k].real() / M), (float)(h[(size_t)k].imag() / M));
+  float2 *d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * host.size()) != hipSuccess) return nullptr;
+  // a synchronous copy: tables are built once per (device, size), before use
+  if (hipMemcpy(d, host.data(), sizeof(float2) * host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  g_czt[key] = d;
+  return d;
+}
+
+template <int M>
+static mxr_status launch_czt(const CztArgs &z, bool inv, hipStream_t s) {
+  using C = GenericCfg<M>;
+  const int64_t grid = (z.f.nbatch + C::TB - 1) / C::TB;
+  if (grid == 0) return MXR_SUCCESS;
+  MXR_REQUIRE(grid <= 0x7fffffff, MXR_INVALID_SIZE, "fft: batch too large");
+  ProfScope ps("czt_kernel", s);
+  if (inv) czt_kernel<M, true><<<dim3((unsigned)grid), dim3(C::NT), 0, s>>>(z);
+  else czt_kernel<M, false><<<dim3((unsigned)grid), dim3(C::NT), 0, s>>>(z);
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
+// Largest size the Bluestein path takes (its convolution fits one 16384-point
+// workgroup transform).
+constexpr int64_t CZT_MAX_N = 8192;
+
+static mxr_status dispatch_czt(int64_t n, FftArgs a, bool inv, hipStream_t s) {
+  int M = 1;
+  while (M < 2 * n - 1) M <<= 1;
+  CztArgs z{};
+  z.f = a;
+  z.n = (int)n;
+  z.f.tw = twiddles_device(M, false);
+  const float2 *tab = czt_tables((int)n, M);
+  MXR_REQUIRE(z.f.tw != nullptr && tab != nullptr, MXR_OUT_OF_MEMORY,
+              "fft: table allocation failed for size %lld", (long long)n);
+  z.chirp = tab;
+  z.spec = tab + n;
+  switch (M) {
+    case 8: return launch_czt<8>(z, inv, s);
+    case 16: return launch_czt<16>(z, inv, s);
+    case 32: return launch_czt<32>(z, inv, s);
+    case 64: return launch_czt<64>(z, inv, s);
+    case 128: return launch_czt<128>(z, inv, s);
+    case 256: return launch_czt<256>(z, inv, s);
+    case 512: return launch_czt<512>(z, inv, s);
+    case 1024: return launch_czt<1024>(z, inv, s);
+    case 2048: return launch_czt<2048>(z, inv, s);
+    case 4096: return launch_czt<4096>(z, inv, s);
+    case 8192: return launch_czt<8192>(z, inv, s);
+    case 16384: return launch_czt<16384>(z, inv, s);
+    default: break;
+  }
+  set_error(MXR_NOT_SUPPORTED, "fft: no Bluestein kernel for size %lld", (long long)n);
+  return MXR_NOT_SUPPORTED;
+}
+
 mxr_status fft_c2c_impl(const mxr_tensor *out, const mxr_tensor *in, int64_t n, int dir, int norm,
                         int axis, hipStream_t stream) {
   mxr_status st;
@@ -155,8 +365,10 @@ mxr_status fft_c2c_impl(const mxr_tensor *out, const mxr_tensor *in, int64_t n, 
                 "fft: batch dim %d differs (%lld vs %lld)", d, (long long)in->shape[d],
                 (long long)out->shape[d]);
   }
-  MXR_REQUIRE(n >= 1 && n <= 16384 && (n & (n - 1)) == 0, MXR_NOT_SUPPORTED,
-              "fft: size %lld is not a power of two in [1, 16384]", (long long)n);
+  const bool pow2 = n >= 1 && (n & (n - 1)) == 0;
+  MXR_REQUIRE((pow2 && n <= 16384) || (n >= 1 && n <= CZT_MAX_N), MXR_NOT_SUPPORTED,
+              "fft: size %lld unsupported (powers of two up to 16384, other sizes up to %lld)",
+              (long long)n, (long long)CZT_MAX_N);
   if (in->data != out->data && views_overlap(*in, *out)) {
     set_error(MXR_ALIAS_ERROR, "fft: input and output overlap without being the same view");
     return MXR_ALIAS_ERROR;
@@ -195,6 +407,7 @@ mxr_status fft_c2c_impl(const mxr_tensor *out, const mxr_tensor *in, int64_t n, 
     MXR_HIP_TRY(hipGetLastError());
     return MXR_SUCCESS;
   }
+  if (!pow2) return dispatch_czt(n, a, inv, stream);
   a.tw = twiddles_device((int)n, false);
   MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "fft: twiddle table allocation failed");
   return dispatch_generic(n, a, inv, stream);

@@ -86,6 +86,40 @@ def test_fft_strided_axis_and_inplace(mx, orc):
     assert rel(host(xd), orc.fft(x, axis=1)) < RTOL
 
 
+@pytest.mark.parametrize("n", [3, 5, 6, 7, 12, 100, 127, 243, 1000, 1023, 3000, 4097, 5000, 8191])
+def test_fft_bluestein_sizes(mx, orc, n):
+    """Sizes that are not powers of two (chirp-z through a power-of-two
+    convolution of length >= 2n - 1) against the FP64 O(n^2) DFT."""
+    rng = np.random.default_rng(n)
+    b = max(1, 4096 // n)
+    x = crandn(rng, b, n)
+    X = host(mx.fft(dev(x)))
+    assert rel(X, orc.fft(x)) < RTOL
+    xi = host(mx.ifft(dev(X)))
+    assert rel(xi, x) < RTOL
+
+
+def test_fft_bluestein_pad_norms_strided_inplace(mx, orc):
+    rng = np.random.default_rng(3)
+    x = crandn(rng, 3, 100)
+    for n in (90, 120, 1000):  # truncate, pad
+        for norm in ("backward", "forward", "ortho"):
+            assert rel(host(mx.fft(dev(x), n, norm=norm)), orc.fft(x, n, norm=norm)) < RTOL
+            assert rel(host(mx.ifft(dev(x), n, norm=norm)), orc.ifft(x, n, norm=norm)) < RTOL
+    y = crandn(rng, 2, 150, 64)
+    yd = dev(y)
+    assert rel(host(mx.fft(yd, axis=1)), orc.fft(y, axis=1)) < RTOL
+    v = yd.permute(0, 2, 1)
+    mx.fft(v, out=v)
+    assert rel(host(yd), orc.fft(y, axis=1)) < RTOL
+
+
+def test_fft_unsupported_size_raises(mx):
+    x = torch.zeros((2, 8193), dtype=torch.complex64, device="cuda")
+    with pytest.raises(Exception, match="unsupported"):
+        mx.fft(x)
+
+
 # --------------------------------------------------------- fft_conv / PC
 def lfm(L, bw, rng=None):
     n = np.arange(L)

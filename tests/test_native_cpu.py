@@ -72,9 +72,13 @@ def test_fft_size_checks(L):
     t3 = fake(L, [9], L.C64, addr=0x90000)
     r, _ = status(L, lib.mxr_fft_c2c, ctypes.byref(t3), ctypes.byref(tc), 0, 0, 0, 0, None)
     assert r == 4
-    # non-power-of-two size is reported as unsupported
-    t5 = fake(L, [12], L.C64, addr=0x90000)
-    r, _ = status(L, lib.mxr_fft_c2c, ctypes.byref(t5), ctypes.byref(tc), 12, 0, 0, 0, None)
+    # sizes past both paths (powers of two to 16384, others to 8192) are
+    # reported as unsupported before any device work
+    t5 = fake(L, [8193], L.C64, addr=0x90000)
+    r, msg = status(L, lib.mxr_fft_c2c, ctypes.byref(t5), ctypes.byref(tc), 8193, 0, 0, 0, None)
+    assert r == 1 and "unsupported" in msg
+    t6 = fake(L, [32768], L.C64, addr=0x90000)
+    r, _ = status(L, lib.mxr_fft_c2c, ctypes.byref(t6), ctypes.byref(tc), 32768, 0, 0, 0, None)
     assert r == 1
     # wrong dtype
     tf = fake(L, [16], L.F32, addr=0x90000)
