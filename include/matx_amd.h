@@ -44,6 +44,7 @@ constexpr index_t matxEnd = std::numeric_limits<index_t>::max() - 1;
 constexpr index_t matxDropDim = std::numeric_limits<index_t>::max() - 2;
 
 enum matxConvCorrMode_t { MATX_C_MODE_FULL = 0, MATX_C_MODE_SAME = 1, MATX_C_MODE_VALID = 2 };
+enum matxConvCorrMethod_t { MATX_C_METHOD_DIRECT = 0, MATX_C_METHOD_FFT = 1 };
 enum class FFTNorm { BACKWARD = 0, FORWARD = 1, ORTHO = 2 };
 enum matxMemorySpace_t { MATX_MANAGED_MEMORY, MATX_DEVICE_MEMORY, MATX_HOST_MALLOC_MEMORY };
 
@@ -434,6 +435,8 @@ struct FFTConvOp {
 struct Conv1DOp {
   Expr in, f;
   int mode;
+  int method = MATX_C_METHOD_DIRECT;
+  int corr = 0;  // correlate: conv1d with reverse(conj(f)) (FFT kernel)
 };
 struct Conv2DOp {
   Expr in, f;
@@ -467,8 +470,17 @@ FFTConvOp fft_conv(const X &x, const W &w, index_t n, bool conj_w = true) {
   return FFTConvOp{detail::to_expr(x), detail::to_expr(w), n, conj_w ? 1 : 0};
 }
 template <typename X, typename F>
-Conv1DOp conv1d(const X &x, const F &f, matxConvCorrMode_t mode) {
-  return Conv1DOp{detail::to_expr(x), detail::to_expr(f), (int)mode};
+Conv1DOp conv1d(const X &x, const F &f, matxConvCorrMode_t mode,
+                matxConvCorrMethod_t method = MATX_C_METHOD_DIRECT) {
+  return Conv1DOp{detail::to_expr(x), detail::to_expr(f), (int)mode, (int)method, 0};
+}
+// corr(x, f) = conv1d(x, reverse(conj(f))) (include/matx/transforms/corr.h:60-67);
+// both methods run mxr_conv1d_fft, whose spectrum kernel reverses the filter.
+template <typename X, typename F>
+Conv1DOp corr(const X &x, const F &f, matxConvCorrMode_t mode,
+              matxConvCorrMethod_t method = MATX_C_METHOD_DIRECT) {
+  (void)method;
+  return Conv1DOp{detail::to_expr(x), detail::to_expr(f), (int)mode, MATX_C_METHOD_FFT, 1};
 }
 template <typename X, typename F>
 Conv2DOp conv2d(const X &x, const F &f, matxConvCorrMode_t mode) {
@@ -592,7 +604,10 @@ struct SetOp {
     } else if constexpr (std::is_same_v<Op, Conv1DOp>) {
       auto x = operand(op.in, {}, MXR_C64, s);
       auto f = operand(op.f, {}, MXR_F32, s);
-      check(mxr_conv1d(&out, &x.t, &f.t, op.mode, s));
+      if (op.method == MATX_C_METHOD_FFT || op.corr)
+        check(mxr_conv1d_fft(&out, &x.t, &f.t, op.mode, op.corr, s));
+      else
+        check(mxr_conv1d(&out, &x.t, &f.t, op.mode, s));
     } else if constexpr (std::is_same_v<Op, Conv2DOp>) {
       auto x = operand(op.in, {}, MXR_F32, s);
       auto f = operand(op.f, {}, MXR_F32, s);

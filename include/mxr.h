@@ -379,6 +379,19 @@ mxr_status mxr_sum(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stre
 mxr_status mxr_conv1d(const mxr_tensor *out, const mxr_tensor *in,
                       const mxr_tensor *filt, int mode, hipStream_t stream);
 
+/* 1-D convolution (corr = 0) or correlation (corr = 1: the filter
+ * reverse(conj(filt)), corr_impl include/matx/transforms/corr.h:60-67) by
+ * the FFT method, matxFFTConv1DInternal include/matx/transforms/conv.h:51-172:
+ * windows FULL [0, n+L-1), SAME [s, s+n) with s = L odd ? (L-1)/2 : L/2-1,
+ * VALID [L-1, n).  in/out c64 or f32 (out c64 iff in or filt is), filt [L]
+ * broadcast over the leading dims, any strides.  n + L - 1 <= 16384: one
+ * fused kernel per row (power-of-two transform >= n + L - 1 in LDS).
+ * Longer rows: c64, unit stride along the axis and L <= 8193 (overlap-save
+ * correlation blocks for the outputs at FULL index >= L-1, the fused kernel
+ * for the ones before). */
+mxr_status mxr_conv1d_fft(const mxr_tensor *out, const mxr_tensor *in,
+                          const mxr_tensor *filt, int mode, int corr, hipStream_t stream);
+
 /* Direct 2-D convolution over the last two axes, batched over leading dims
  * (include/matx/transforms/conv.h:547-595; index rule :340-380). */
 mxr_status mxr_conv2d(const mxr_tensor *out, const mxr_tensor *in,

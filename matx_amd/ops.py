@@ -285,8 +285,25 @@ def psl_adjoint(gpsl, pw, targets, ml: int = 2, q: int = 8, stream=None):
     return gp
 
 
-def conv1d(x, f, mode="full", out=None, stream=None):
-    """Direct conv1d along the last axis (include/matx/operators/conv.h:244-266)."""
+def conv1d(x, f, mode="full", method="direct", out=None, stream=None):
+    """conv1d along the last axis (include/matx/operators/conv.h:244-266):
+    method "direct" (mxr_conv1d) or "fft" (mxr_conv1d_fft, MatX's
+    MATX_C_METHOD_FFT, include/matx/transforms/conv.h:51-172)."""
+    return _conv1d(x, f, mode, method, False, out, stream)
+
+
+def corr(x, f, mode="full", method="direct", out=None, stream=None):
+    """corr(x, f) = conv1d(x, reverse(conj(f))) (include/matx/transforms/
+    corr.h:60-67).  The direct method reverses the (short) filter first; the
+    FFT method transforms the reversed filter in its spectrum kernel."""
+    if method == "direct":
+        fr = torch.flip(f, [-1])
+        fr = fr.conj().resolve_conj() if fr.is_complex() else fr.contiguous()
+        return _conv1d(x, fr, mode, method, False, out, stream)
+    return _conv1d(x, f, mode, method, True, out, stream)
+
+
+def _conv1d(x, f, mode, method, is_corr, out, stream):
     m = _mode(m=mode)
     n, k = x.shape[-1], f.shape[-1]
     olen = {L.FULL: n + k - 1, L.SAME: n, L.VALID: n - k + 1}[m]
@@ -294,8 +311,14 @@ def conv1d(x, f, mode="full", out=None, stream=None):
     if out is None:
         out = torch.empty(x.shape[:-1] + (olen,), device=x.device,
                           dtype=torch.complex64 if cplx else torch.float32)
-    L.check(L.lib().mxr_conv1d(L.ref(L.desc(out)), L.ref(L.desc(x)), L.ref(L.desc(f)), m,
-                               L.stream_handle(stream)))
+    if method == "direct":
+        L.check(L.lib().mxr_conv1d(L.ref(L.desc(out)), L.ref(L.desc(x)), L.ref(L.desc(f)), m,
+                                   L.stream_handle(stream)))
+    elif method == "fft":
+        L.check(L.lib().mxr_conv1d_fft(L.ref(L.desc(out)), L.ref(L.desc(x)), L.ref(L.desc(f)), m,
+                                       1 if is_corr else 0, L.stream_handle(stream)))
+    else:
+        raise ValueError(f"conv1d: method must be 'direct' or 'fft', got {method!r}")
     return out
 
 
