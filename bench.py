@@ -50,12 +50,15 @@ def parse():
                          "on MI355X: the co-resident kernels starve each other of LDS)")
     ap.add_argument("--no-profile", action="store_true",
                     help="time without per-kernel events (roofline omitted)")
-    ap.add_argument("--workload", default="radar", choices=["radar", "fftconv", "large", "grad", "ragged"],
+    ap.add_argument("--workload", default="radar",
+                    choices=["radar", "fftconv", "large", "grad", "ragged", "conv"],
                     help="radar: the 16x128x9000 pipeline (default); fftconv: fft_conv on "
                          "4x32x2048 with a 256-sample chirp at N=4096 (us per call); large: "
                          "the 64x512x32768 pipeline (8.6 GB cube); grad: waveform/window "
                          "design loss forward + backward (grad steps/s); ragged: 256 uneven "
-                         "pulse-compression + Doppler jobs, batched vs per-job launches")
+                         "pulse-compression + Doppler jobs, batched vs per-job launches; "
+                         "conv: conv1d by the FFT method on 16 x 1048576 c64 with 209 taps "
+                         "(MatX's large ConvCorr size) and 1000-point Bluestein FFTs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 512 MiB device copies (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -194,6 +197,47 @@ def load_traffic(kernel, workload="radar"):
     return None
 
 
+def profiled_kernels(fn, calls, names):
+    """Runs fn() `calls` times with per-launch HIP events (recorded by the
+    library on the launch stream) and returns {kernel: (launches per call,
+    average us per launch)} for the names that ran."""
+    import torch
+    from matx_amd import _lib
+    lib = _lib.lib()
+    torch.cuda.synchronize()
+    _lib.check(lib.mxr_profile_reset())
+    _lib.check(lib.mxr_profile_enable(1))
+    for _ in range(calls):
+        fn()
+    torch.cuda.synchronize()
+    _lib.check(lib.mxr_profile_enable(0))
+    out = {}
+    for name in names:
+        cnt, tot = _lib.profile_query(name)
+        if cnt:
+            out[name] = (cnt / calls, 1000.0 * tot / cnt)
+    return out
+
+
+def kernel_roofline(kern, bytes_per_launch):
+    """Roofline object of the kernel with the largest time per call among
+    those with a byte model (HBM-bound)."""
+    best = None
+    for name, (per_call, avg_us) in kern.items():
+        if name not in bytes_per_launch:
+            continue
+        if best is None or per_call * avg_us > best[1] * best[2]:
+            best = (name, per_call, avg_us)
+    if best is None:
+        return None
+    name, _, avg_us = best
+    b = bytes_per_launch[name]
+    ach = b / (avg_us * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "algorithmic_bytes_per_launch": b, "avg_us": round(avg_us, 2)}
+
+
 def bench_fftconv(args):
     """fft_conv workload: x {4, 32, 2048} c64 (noise sigma = 1 plus three
     delayed copies of the waveform at 10 dB, delays uniform in [0, 1792)),
@@ -271,6 +315,10 @@ def bench_fftconv(args):
     rel = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     B = 8 * C * P * S + 8 * C * P * N
     best = min(us_plain, us_graph)
+    # the filter kernel reads x and writes the full circular output
+    kern = profiled_kernels(lambda: matx_amd.fft_conv(x, w, N, out=out), 50,
+                            ["pc4096_kernel", "pc_ols_kernel", "wfspec_kernel"])
+    roof = kernel_roofline(kern, {"pc4096_kernel": B, "pc_ols_kernel": B})
     print(json.dumps({
         "metric": "fft_conv us per call (4x32x2048 c64, L=256, N=4096)",
         "value": round(best, 3), "unit": "us/call", "n_gpus": 1, "steps": steps,
@@ -284,11 +332,101 @@ def bench_fftconv(args):
                    "calls_per_s": round(1e6 / best, 1),
                    "algorithmic_bytes_per_call": B,
                    "hbm_gbs_at_best": round(B / (best * 1e-6) / 1e9, 1),
+                   "per_kernel_us": {k: round(v[1], 2) for k, v in kern.items()},
                    "rel_l2_vs_fp64": rel},
+        "roofline": roof,
         "cpu_baseline": {"value": round(cpu_s * 1e6, 1), "unit": "us/call", "cores":
                          int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0)))),
                          "kind": "port",
                          "sample": f"FP64 oracle fft_conv on the same input, {cpu_reps} calls"},
+    }))
+
+
+def bench_conv(args):
+    """FFT-method conv1d at the size of MatX's CorrelationConvolutionLargeTest
+    (test/00_transform/ConvCorr.cu:61-63: 8*122880 + 2*32768 samples, 209
+    taps, FULL), batched over 16 rows of unit complex noise (sigma = 1), and
+    8192 Bluestein FFTs of 1000 points.  Timed as back-to-back calls with the
+    inputs resident; parity of row 0 (conv) and of 64 rows (FFT) against the
+    FP64 oracle."""
+    import numpy as np
+    import torch
+    import matx_amd
+    dev = "cuda:0"
+    rows, N, L = 16, 8 * 122880 + 2 * 32768, 209
+    g = torch.Generator(device="cpu").manual_seed(args.seed)
+
+    def crandn(*shape):
+        return (torch.complex(torch.randn(shape, generator=g), torch.randn(shape, generator=g))
+                / math.sqrt(2)).to(torch.complex64)
+
+    x, f = crandn(rows, N).to(dev), crandn(L).to(dev)
+    out = torch.empty((rows, N + L - 1), dtype=torch.complex64, device=dev)
+    rows2, n2 = 8192, 1000
+    x2 = crandn(rows2, n2).to(dev)
+    out2 = torch.empty_like(x2)
+    conv = lambda: matx_amd.conv1d(x, f, "full", method="fft", out=out)  # noqa: E731
+    ftf = lambda: matx_amd.fft(x2, out=out2)  # noqa: E731
+
+    def timed(fn, k):
+        for _ in range(max(args.warmup, 3)):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(k):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return 1000.0 * e0.elapsed_time(e1) / k
+
+    k = max(args.steps, 20)
+    us_conv = timed(conv, k)
+    us_fft = timed(ftf, k)
+    # algorithmic bytes: read the signal, write the output (the filter is
+    # negligible); the overlap-save kernel writes outputs L-1 .. N+L-2, the
+    # fused head kernel the first L-1
+    B_conv = 8 * rows * N + 8 * rows * (N + L - 1)
+    B_ols = 8 * rows * N + 8 * rows * N
+    B_fft = 2 * 8 * rows2 * n2
+    kern = profiled_kernels(conv, 10, ["pc4096_kernel", "pc8192_kernel", "pc_ols_kernel",
+                                       "wfspec_kernel", "fconv_kernel", "fconv_spec_kernel"])
+    roof = kernel_roofline(kern, {"pc4096_kernel": B_ols, "pc8192_kernel": B_ols,
+                                  "pc_ols_kernel": B_ols})
+    kfft = profiled_kernels(ftf, 10, ["czt_kernel"])
+    roof_fft = kernel_roofline(kfft, {"czt_kernel": B_fft})
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as orc
+    orc.build()
+    xh, fh = x[0].cpu().numpy(), f.cpu().numpy()
+    t0 = time.perf_counter()
+    ref = orc.conv1d(xh, fh, "full")
+    cpu_row_s = time.perf_counter() - t0
+    rel_conv = float(np.linalg.norm(out[0].cpu().numpy() - ref) / np.linalg.norm(ref))
+    x2h = x2[:64].cpu().numpy()
+    ref2 = orc.fft(x2h)
+    rel_fft = float(np.linalg.norm(out2[:64].cpu().numpy() - ref2) / np.linalg.norm(ref2))
+    print(json.dumps({
+        "metric": "conv1d FFT method us per call (16 x 1048576 c64, 209 taps, FULL)",
+        "value": round(us_conv, 2), "unit": "us/call", "n_gpus": 1, "steps": k,
+        "warmup": max(args.warmup, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64", "data": "synthetic (unit complex noise, seeded)",
+        "config": {"workload": "conv1d(x, f, FULL, MATX_C_METHOD_FFT), x {16, 1048576} c64, "
+                               "f {209} c64 (MatX ConvCorr large-test size, 16 rows)",
+                   "algorithmic_bytes_per_call": B_conv,
+                   "hbm_gbs": round(B_conv / (us_conv * 1e-6) / 1e9, 1),
+                   "per_kernel_us": {kk: round(v[1], 2) for kk, v in kern.items()},
+                   "rel_l2_vs_fp64_row0": rel_conv,
+                   "bluestein_fft": {"workload": "fft of 8192 rows x 1000 points c64 "
+                                                 "(2048-point chirp-z convolution)",
+                                     "us_per_call": round(us_fft, 2),
+                                     "hbm_gbs": round(B_fft / (us_fft * 1e-6) / 1e9, 1),
+                                     "roofline": roof_fft,
+                                     "rel_l2_vs_fp64_64rows": rel_fft}},
+        "roofline": roof,
+        "cpu_baseline": {"value": round(cpu_row_s * rows * 1e6, 1), "unit": "us/call",
+                         "cores": 1, "kind": "port",
+                         "sample": "FP64 oracle direct convolution of row 0, scaled by 16 rows"},
     }))
 
 
@@ -559,6 +697,9 @@ def main():
         return
     if args.workload == "ragged":
         bench_ragged(args)
+        return
+    if args.workload == "conv":
+        bench_conv(args)
         return
     bench_pipeline(args, WORKLOADS[args.workload])
 
