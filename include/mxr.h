@@ -99,6 +99,21 @@ mxr_status mxr_profile_query(const char *name, int64_t *count, double *total_ms)
 mxr_status mxr_fft_c2c(const mxr_tensor *out, const mxr_tensor *in, int64_t n,
                        int dir, int norm, int axis, hipStream_t stream);
 
+/* Real-to-complex FFT along `axis` (rfft, include/matx/operators/fft.h:580-626):
+ * in f32, out c64 with n/2 + 1 bins along the axis; n = 0 means
+ * in->shape[axis]; the input is zero-padded or truncated to n.  Sizes, norms
+ * and strides as mxr_fft_c2c (forward direction). */
+mxr_status mxr_fft_r2c(const mxr_tensor *out, const mxr_tensor *in, int64_t n,
+                       int norm, int axis, hipStream_t stream);
+
+/* Complex-to-real inverse FFT along `axis` (irfft, include/matx/operators/
+ * fft.h:686-740): bins 0..n/2 of `in` (zero-padded or truncated) extended
+ * as X[n-k] = conj(X[k]); out f32 with n samples along the axis; n = 0
+ * means 2 * (in->shape[axis] - 1).  Sizes, norms and strides as
+ * mxr_fft_c2c (inverse direction). */
+mxr_status mxr_fft_c2r(const mxr_tensor *out, const mxr_tensor *in, int64_t n,
+                       int norm, int axis, hipStream_t stream);
+
 /* fft_conv: out = ifft(fft(x, n) .* conj(fft(w, n)))  (conj_w = 1, circular
  * cross-correlation, the BASELINE operator) or ifft(fft(x,n) .* fft(w,n))
  * (conj_w = 0, circular convolution), along the last axis of x.

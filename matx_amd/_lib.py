@@ -22,7 +22,7 @@ PC_WINDOW, PC_NORMALIZE, PC_WRITEBACK = 1, 2, 4
 
 # Every entry point include/mxr.h declares.
 EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_enable",
-           "mxr_profile_reset", "mxr_profile_query", "mxr_fft_c2c",
+           "mxr_profile_reset", "mxr_profile_query", "mxr_fft_c2c", "mxr_fft_r2c", "mxr_fft_c2r",
            "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar", "mxr_doppler_cfar",
            "mxr_conv1d", "mxr_conv1d_fft", "mxr_conv2d", "mxr_ew_eval", "mxr_sum", "mxr_pc_plan_create",
            "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy",
@@ -98,6 +98,8 @@ def lib():
     L.mxr_profile_query.restype = I
     sigs = {
         "mxr_fft_c2c": [T, T, I64, I, I, I, V],
+        "mxr_fft_r2c": [T, T, I64, I, I, V],
+        "mxr_fft_c2r": [T, T, I64, I, I, V],
         "mxr_fft_conv": [T, T, T, I64, I, V],
         "mxr_pulse_compress": [T, T, T, I64, I, V],
         "mxr_doppler": [T, T, T, I, I64, I64, V],

@@ -21,17 +21,46 @@
 
 namespace mxr {
 
+// Real-data forms (rfft / irfft, include/matx/operators/fft.h:580-740): R2C
+// reads f32 samples (imaginary part zero) and stores bins 0..n/2; C2R reads
+// bins 0..n/2, extends them as X[n-k] = conj(X[k]) and stores the real part
+// (the imaginary parts of bins 0 and n/2 drop out, as in cuFFT and numpy).
+enum FftKind { FFT_C2C = 0, FFT_R2C = 1, FFT_C2R = 2 };
+
 struct FftArgs {
-  const float2 *in;
-  float2 *out;
+  const float2 *in;  // f32 samples for R2C
+  float2 *out;       // f32 samples for C2R
   int64_t nbatch;
-  int64_t in_len;  // elements of the input axis used (min(extent, n))
+  int kind;
+  int64_t n;
+  int64_t out_len; // outputs stored per transform (n, or n/2+1 for R2C)
+  int64_t in_len;  // elements of the input axis used (min(extent, n); n/2+1 for C2R)
   int64_t istride, ostride;
   BatchIdx bi;     // batch dims: sa = input strides, sb = output strides
   int batch_fast;  // gather/scatter with lanes over the batch index
   float scale;
   const float2 *tw;
 };
+
+__device__ __forceinline__ float2 fft_gather(const FftArgs &a, int64_t base, int64_t i) {
+  if (a.kind == FFT_C2R) {
+    const bool mirror = i > a.n / 2;
+    const int64_t h = mirror ? a.n - i : i;
+    if (h >= a.in_len) return make_float2(0.f, 0.f);
+    float2 v = a.in[base + h * a.istride];
+    if (mirror) v.y = -v.y;
+    return v;
+  }
+  if (i >= a.in_len) return make_float2(0.f, 0.f);
+  if (a.kind == FFT_R2C)
+    return make_float2(reinterpret_cast<const float *>(a.in)[base + i * a.istride], 0.f);
+  return a.in[base + i * a.istride];
+}
+
+__device__ __forceinline__ void fft_scatter(const FftArgs &a, int64_t base, int64_t k, float2 v) {
+  if (a.kind == FFT_C2R) reinterpret_cast<float *>(a.out)[base + k * a.ostride] = v.x;
+  else a.out[base + k * a.ostride] = v;
+}
 
 template <int N>
 struct GenericCfg {
@@ -68,7 +97,7 @@ __global__ __launch_bounds__(GenericCfg<N>::NT) void fft_generic_kernel(FftArgs 
       i = e % N;
     }
     float2 v = make_float2(0.f, 0.f);
-    if (t0 + t < a.nbatch && i < a.in_len) v = a.in[base_in[t] + (int64_t)i * a.istride];
+    if (t0 + t < a.nbatch) v = fft_gather(a, base_in[t], i);
     lds[LY::addr(t, i)] = v;
   }
   __syncthreads();
@@ -85,7 +114,7 @@ __global__ __launch_bounds__(GenericCfg<N>::NT) void fft_generic_kernel(FftArgs 
       t = e / N;
       i = e % N;
     }
-    if (t0 + t < a.nbatch) a.out[base_out[t] + (int64_t)i * a.ostride] = cscale(lds[LY::addr(t, i)], a.scale);
+    if (t0 + t < a.nbatch && i < a.out_len) fft_scatter(a, base_out[t], i, cscale(lds[LY::addr(t, i)], a.scale));
   }
 }
 
@@ -95,8 +124,7 @@ __global__ void fft_n1_kernel(FftArgs a) {
   if (b >= a.nbatch) return;
   int64_t oi, oo;
   batch_offsets(a.bi, b, &oi, &oo);
-  float2 v = a.in_len > 0 ? a.in[oi] : make_float2(0.f, 0.f);
-  a.out[oo] = cscale(v, a.scale);
+  fft_scatter(a, oo, 0, cscale(fft_gather(a, oi, 0), a.scale));
 }
 
 template <int N>
@@ -171,8 +199,8 @@ __global__ __launch_bounds__(GenericCfg<M>::NT) void czt_kernel(CztArgs z) {
   }
   __syncthreads();
 
-  // gather x.c into the first n slots (x conjugated for the inverse), zeros
-  // to M
+  // gather x.c into the first n slots (x conjugated for the inverse; C2R
+  // input extended to n first), zeros to M
   for (int e = threadIdx.x; e < TB * M; e += NT) {
     int t, i;
     if (a.batch_fast) {
@@ -183,8 +211,8 @@ __global__ __launch_bounds__(GenericCfg<M>::NT) void czt_kernel(CztArgs z) {
       i = e % M;
     }
     float2 v = make_float2(0.f, 0.f);
-    if (t0 + t < a.nbatch && i < a.in_len) {
-      float2 x = a.in[base_in[t] + (int64_t)i * a.istride];
+    if (t0 + t < a.nbatch && i < n) {
+      float2 x = fft_gather(a, base_in[t], i);
       if (INV) x.y = -x.y;
       v = cmul(x, z.chirp[i]);
     }
@@ -199,20 +227,22 @@ __global__ __launch_bounds__(GenericCfg<M>::NT) void czt_kernel(CztArgs z) {
   __syncthreads();
   lds_passes<LY, SQ, M, NT, TB, true, 0, SQ::n>(lds, a.tw);
 
-  // scatter c.y for k < n with the norm scale (conjugated for the inverse)
-  for (int e = threadIdx.x; e < TB * n; e += NT) {
+  // scatter c.y for k < out_len (<= n) with the norm scale (conjugated for
+  // the inverse)
+  const int nout = (int)a.out_len;
+  for (int e = threadIdx.x; e < TB * nout; e += NT) {
     int t, k;
     if (a.batch_fast) {
       t = e % TB;
       k = e / TB;
     } else {
-      t = e / n;
-      k = e % n;
+      t = e / nout;
+      k = e % nout;
     }
     if (t0 + t >= a.nbatch) continue;
     float2 y = cscale(cmul(lds[LY::addr(t, k)], z.chirp[k]), a.scale);
     if (INV) y.y = -y.y;
-    a.out[base_out[t] + (int64_t)k * a.ostride] = y;
+    fft_scatter(a, base_out[t], k, y);
   }
 }
 
@@ -339,45 +369,58 @@ static mxr_status dispatch_czt(int64_t n, FftArgs a, bool inv, hipStream_t s) {
   return MXR_NOT_SUPPORTED;
 }
 
-mxr_status fft_c2c_impl(const mxr_tensor *out, const mxr_tensor *in, int64_t n, int dir, int norm,
-                        int axis, hipStream_t stream) {
+static mxr_status fft_impl(const char *fn, const mxr_tensor *out, const mxr_tensor *in, int64_t n,
+                           int dir, int norm, int axis, int kind, hipStream_t stream) {
   mxr_status st;
-  if ((st = check_tensor(in, "fft input", MXR_C64, 1, MXR_MAX_RANK))) return st;
-  if ((st = check_tensor(out, "fft output", MXR_C64, 1, MXR_MAX_RANK))) return st;
-  MXR_REQUIRE(in->rank == out->rank, MXR_INVALID_DIM, "fft: input rank %d != output rank %d",
+  const int in_t = kind == FFT_R2C ? MXR_F32 : MXR_C64;
+  const int out_t = kind == FFT_C2R ? MXR_F32 : MXR_C64;
+  if ((st = check_tensor(in, "fft input", in_t, 1, MXR_MAX_RANK))) return st;
+  if ((st = check_tensor(out, "fft output", out_t, 1, MXR_MAX_RANK))) return st;
+  MXR_REQUIRE(in->rank == out->rank, MXR_INVALID_DIM, "%s: input rank %d != output rank %d", fn,
               in->rank, out->rank);
   const int rank = in->rank;
   if (axis < 0) axis += rank;
-  MXR_REQUIRE(axis >= 0 && axis < rank, MXR_INVALID_DIM, "fft: axis out of range");
+  MXR_REQUIRE(axis >= 0 && axis < rank, MXR_INVALID_DIM, "%s: axis out of range", fn);
   MXR_REQUIRE(dir == MXR_FFT_FORWARD || dir == MXR_FFT_INVERSE, MXR_INVALID_PARAMETER,
-              "fft: bad direction %d", dir);
+              "%s: bad direction %d", fn, dir);
   MXR_REQUIRE(norm >= MXR_NORM_BACKWARD && norm <= MXR_NORM_ORTHO, MXR_INVALID_PARAMETER,
-              "fft: bad norm %d", norm);
-  if (n == 0) n = in->shape[axis];
+              "%s: bad norm %d", fn, norm);
+  // transform size: C2C and R2C default to the input length, C2R to
+  // 2 (N - 1) for N input bins (include/matx/operators/fft.h:111-199)
+  if (n == 0) n = kind == FFT_C2R ? 2 * (in->shape[axis] - 1) : in->shape[axis];
   // MatX never pads implicitly: the output extent along the axis is the
-  // transform size (include/matx/operators/fft.h:111-199, FFT.cu:661-719).
-  MXR_REQUIRE(out->shape[axis] == n, MXR_INVALID_SIZE,
-              "fft: output size %lld along axis %d does not match transform size %lld",
-              (long long)out->shape[axis], axis, (long long)n);
+  // transform size, or n/2 + 1 bins for R2C (fft.h:111-199, FFT.cu:661-719).
+  const int64_t out_len = kind == FFT_R2C ? n / 2 + 1 : n;
+  MXR_REQUIRE(out->shape[axis] == out_len, MXR_INVALID_SIZE,
+              "%s: output size %lld along axis %d does not match the %lld outputs of a "
+              "%lld-point transform", fn, (long long)out->shape[axis], axis, (long long)out_len,
+              (long long)n);
   for (int d = 0; d < rank; ++d) {
     if (d == axis) continue;
     MXR_REQUIRE(in->shape[d] == out->shape[d], MXR_INVALID_SIZE,
-                "fft: batch dim %d differs (%lld vs %lld)", d, (long long)in->shape[d],
+                "%s: batch dim %d differs (%lld vs %lld)", fn, d, (long long)in->shape[d],
                 (long long)out->shape[d]);
   }
   const bool pow2 = n >= 1 && (n & (n - 1)) == 0;
   MXR_REQUIRE((pow2 && n <= 16384) || (n >= 1 && n <= CZT_MAX_N), MXR_NOT_SUPPORTED,
-              "fft: size %lld unsupported (powers of two up to 16384, other sizes up to %lld)",
+              "%s: size %lld unsupported (powers of two up to 16384, other sizes up to %lld)", fn,
               (long long)n, (long long)CZT_MAX_N);
-  if (in->data != out->data && views_overlap(*in, *out)) {
-    set_error(MXR_ALIAS_ERROR, "fft: input and output overlap without being the same view");
-    return MXR_ALIAS_ERROR;
+  if (views_overlap(*in, *out)) {
+    // a transform may overwrite its own input view (every element is read
+    // before any is written), not a different view of the same memory
+    const bool same = kind == FFT_C2C && in->data == out->data;
+    MXR_REQUIRE(same, MXR_ALIAS_ERROR, "%s: input and output overlap%s", fn,
+                kind == FFT_C2C ? " without being the same view" : "");
   }
 
   FftArgs a{};
   a.in = (const float2 *)in->data;
   a.out = (float2 *)out->data;
-  a.in_len = in->shape[axis] < n ? in->shape[axis] : n;
+  a.kind = kind;
+  a.n = n;
+  a.out_len = out_len;
+  const int64_t used = kind == FFT_C2R ? n / 2 + 1 : n;
+  a.in_len = in->shape[axis] < used ? in->shape[axis] : used;
   a.istride = in->stride[axis];
   a.ostride = out->stride[axis];
   a.bi.nd = 0;
@@ -409,11 +452,30 @@ mxr_status fft_c2c_impl(const mxr_tensor *out, const mxr_tensor *in, int64_t n, 
   }
   if (!pow2) return dispatch_czt(n, a, inv, stream);
   a.tw = twiddles_device((int)n, false);
-  MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "fft: twiddle table allocation failed");
+  MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
   return dispatch_generic(n, a, inv, stream);
 }
 
+mxr_status fft_c2c_impl(const mxr_tensor *out, const mxr_tensor *in, int64_t n, int dir, int norm,
+                        int axis, hipStream_t stream) {
+  return fft_impl("fft", out, in, n, dir, norm, axis, FFT_C2C, stream);
+}
+
 }  // namespace mxr
+
+extern "C" mxr_status mxr_fft_r2c(const mxr_tensor *out, const mxr_tensor *in, int64_t n, int norm,
+                                  int axis, hipStream_t stream) {
+  return mxr::guarded("mxr_fft_r2c", [&] {
+    return mxr::fft_impl("rfft", out, in, n, MXR_FFT_FORWARD, norm, axis, mxr::FFT_R2C, stream);
+  });
+}
+
+extern "C" mxr_status mxr_fft_c2r(const mxr_tensor *out, const mxr_tensor *in, int64_t n, int norm,
+                                  int axis, hipStream_t stream) {
+  return mxr::guarded("mxr_fft_c2r", [&] {
+    return mxr::fft_impl("irfft", out, in, n, MXR_FFT_INVERSE, norm, axis, mxr::FFT_C2R, stream);
+  });
+}
 
 extern "C" mxr_status mxr_fft_c2c(const mxr_tensor *out, const mxr_tensor *in, int64_t n, int dir,
                                   int norm, int axis, hipStream_t stream) {

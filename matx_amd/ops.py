@@ -58,6 +58,65 @@ def ifft(x, n: int = 0, axis: int = -1, norm: str = "backward", out=None, stream
     return _fft(x, n, axis, norm, out, True, stream)
 
 
+def rfft(x, n: int = 0, axis: int = -1, norm: str = "backward", out=None, stream=None):
+    """Real-to-complex ``rfft(x, n)`` (include/matx/operators/fft.h:580-626):
+    f32 in, n/2 + 1 complex bins out along ``axis`` (mxr_fft_r2c)."""
+    ax = axis % x.dim()
+    n = int(n) if n else x.shape[ax]
+    if out is None:
+        shape = list(x.shape)
+        shape[ax] = n // 2 + 1
+        out = torch.empty(shape, dtype=torch.complex64, device=x.device)
+    L.check(L.lib().mxr_fft_r2c(L.ref(L.desc(out)), L.ref(L.desc(x)), n, _NORMS[norm], ax,
+                                L.stream_handle(stream)))
+    return out
+
+
+def irfft(x, n: int = 0, axis: int = -1, norm: str = "backward", out=None, stream=None):
+    """Complex-to-real ``irfft(x, n)`` (include/matx/operators/fft.h:686-740):
+    bins 0..n/2 in, n real samples out (n defaults to 2 (N - 1)) (mxr_fft_c2r)."""
+    ax = axis % x.dim()
+    n = int(n) if n else 2 * (x.shape[ax] - 1)
+    if out is None:
+        shape = list(x.shape)
+        shape[ax] = n
+        out = torch.empty(shape, dtype=torch.float32, device=x.device)
+    L.check(L.lib().mxr_fft_c2r(L.ref(L.desc(out)), L.ref(L.desc(x)), n, _NORMS[norm], ax,
+                                L.stream_handle(stream)))
+    return out
+
+
+def fft2(x, norm: str = "backward", out=None, stream=None):
+    """2-D FFT over the last two axes (fft2, include/matx/operators/fft.h:
+    743-1130): the transform along the rows, then along the columns of the
+    result, in place (the column pass reads its strided axis lane-parallel
+    over the contiguous row index)."""
+    out = _fft(x, 0, -1, norm, out, False, stream)
+    return _fft(out, 0, -2, norm, out, False, stream)
+
+
+def ifft2(x, norm: str = "backward", out=None, stream=None):
+    """2-D inverse FFT over the last two axes (ifft2)."""
+    out = _fft(x, 0, -1, norm, out, True, stream)
+    return _fft(out, 0, -2, norm, out, True, stream)
+
+
+def rfft2(x, norm: str = "backward", out=None, stream=None):
+    """2-D real-to-complex FFT (rfft2, include/matx/operators/fft.h:1236-1272):
+    rfft along the last axis (n/2 + 1 bins), then the complex transform
+    along the second-to-last."""
+    out = rfft(x, 0, -1, norm, out, stream)
+    return _fft(out, 0, -2, norm, out, False, stream)
+
+
+def irfft2(x, n: int = 0, norm: str = "backward", out=None, stream=None):
+    """2-D complex-to-real inverse FFT (irfft2, include/matx/operators/
+    fft.h:1275-1310): the complex inverse along the second-to-last axis into
+    a temporary, then irfft along the last (n samples, default 2 (N - 1))."""
+    t = _fft(x, 0, -2, norm, None, True, stream)
+    return irfft(t, n, -1, norm, out, stream)
+
+
 def fft_conv(x, w, n: int, conj: bool = True, nout: int | None = None, out=None, stream=None):
     """out = ifft(fft(x, n) * conj(fft(w, n))) along the last axis (lags
     0..nout-1, default all n); conj=False gives circular convolution."""

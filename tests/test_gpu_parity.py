@@ -120,6 +120,60 @@ def test_fft_unsupported_size_raises(mx):
         mx.fft(x)
 
 
+@pytest.mark.parametrize("n", [1, 2, 8, 100, 1000, 1024, 4096, 4097, 5000, 16384])
+def test_rfft_irfft_sizes(mx, n):
+    """R2C / C2R (mxr_fft_r2c / mxr_fft_c2r, powers of two and Bluestein
+    sizes) against numpy's FP64 rfft / irfft (numpy's FFT pins the oracle's
+    in tests/test_oracle.py)."""
+    rng = np.random.default_rng(n + 7)
+    b = max(1, 8192 // n)
+    x = rng.standard_normal((b, n)).astype(np.float32)
+    X = host(mx.rfft(dev(x, torch.float32)))
+    assert X.shape == (b, n // 2 + 1)
+    assert rel(X, np.fft.rfft(x.astype(np.float64))) < RTOL
+    xr = host(mx.irfft(dev(X), n))
+    assert xr.dtype == np.float32 and xr.shape == (b, n)
+    assert rel(xr, x) < RTOL
+    # bins that are not a Hermitian half (imaginary DC / Nyquist parts):
+    # dropped, as numpy and cuFFT drop them
+    Z = crandn(rng, b, n // 2 + 1)
+    assert rel(host(mx.irfft(dev(Z), n)), np.fft.irfft(Z.astype(np.complex128), n)) < RTOL
+
+
+def test_rfft_irfft_pad_norms_axis(mx):
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((3, 100, 6)).astype(np.float32)
+    xd = dev(x, torch.float32)
+    for n in (64, 100, 128, 150):          # truncate, exact, pad (pow2 and not)
+        for norm in ("backward", "forward", "ortho"):
+            X = host(mx.rfft(xd, n, axis=1, norm=norm))
+            ref = np.fft.rfft(x.astype(np.float64), n, axis=1, norm=norm)
+            assert rel(X, ref) < RTOL, (n, norm)
+            back = host(mx.irfft(dev(X), n, axis=1, norm=norm))
+            assert rel(back, np.fft.irfft(ref, n, axis=1, norm=norm)) < RTOL, (n, norm)
+    # irfft with fewer bins than n/2 + 1 (zero-padded) and with more (truncated)
+    Z = crandn(rng, 4, 40)
+    for n in (200, 40, 33):
+        assert rel(host(mx.irfft(dev(Z), n)), np.fft.irfft(Z.astype(np.complex128), n)) < RTOL, n
+    # default sizes: rfft keeps n, irfft makes 2 (N - 1)
+    assert host(mx.irfft(dev(Z))).shape == (4, 78)
+
+
+def test_fft2_family(mx, orc):
+    rng = np.random.default_rng(12)
+    x = crandn(rng, 2, 96, 128)
+    X = host(mx.fft2(dev(x)))
+    ref = orc.fft(orc.fft(x, axis=-1), axis=-2)
+    assert rel(X, ref) < RTOL
+    assert rel(host(mx.ifft2(dev(X))), x) < RTOL
+    r = rng.standard_normal((2, 60, 100)).astype(np.float32)
+    R = host(mx.rfft2(dev(r, torch.float32)))
+    assert rel(R, np.fft.rfft2(r.astype(np.float64))) < RTOL
+    assert rel(host(mx.irfft2(dev(R), 100)), r) < RTOL
+    for norm in ("forward", "ortho"):
+        assert rel(host(mx.fft2(dev(x), norm=norm)), np.fft.fft2(x.astype(np.complex128), norm=norm)) < RTOL
+
+
 # --------------------------------------------------------- fft_conv / PC
 def lfm(L, bw, rng=None):
     n = np.arange(L)
