@@ -421,10 +421,15 @@ struct FFTSpec {
   int dir = MXR_FFT_FORWARD;
   int norm = (int)FFTNorm::BACKWARD;
   int axis = -1;
+  int kind = 0;  // 0 complex-to-complex, 1 real-to-complex, 2 complex-to-real
 };
 }  // namespace detail
 
 struct FFTOp {
+  detail::FFTSpec s;
+};
+// 2-D transform over the last two axes (fft2 / ifft2 / rfft2 / irfft2)
+struct FFT2Op {
   detail::FFTSpec s;
 };
 struct FFTConvOp {
@@ -463,6 +468,34 @@ FFTOp fft(const tensor_t<T, R> &a, std::array<int, 1> axis, index_t n = 0, FFTNo
 template <typename T, int R>
 FFTOp ifft(const tensor_t<T, R> &a, std::array<int, 1> axis, index_t n = 0, FFTNorm norm = FFTNorm::BACKWARD) {
   return FFTOp{{detail::to_expr(a), {}, n, MXR_FFT_INVERSE, (int)norm, axis[0]}};
+}
+// rfft / irfft (include/matx/operators/fft.h:580-740): real input, n/2 + 1
+// bins out; bins in, n real samples out (n = 0: 2 (N - 1))
+template <typename T, int R>
+FFTOp rfft(const tensor_t<T, R> &a, index_t n = 0, FFTNorm norm = FFTNorm::BACKWARD) {
+  return FFTOp{{detail::to_expr(a), {}, n, MXR_FFT_FORWARD, (int)norm, R - 1, 1}};
+}
+template <typename T, int R>
+FFTOp irfft(const tensor_t<T, R> &a, index_t n = 0, FFTNorm norm = FFTNorm::BACKWARD) {
+  return FFTOp{{detail::to_expr(a), {}, n, MXR_FFT_INVERSE, (int)norm, R - 1, 2}};
+}
+// fft2 / ifft2 / rfft2 / irfft2 over the last two axes (include/matx/operators/fft.h:743-1310)
+template <typename T, int R>
+FFT2Op fft2(const tensor_t<T, R> &a, FFTNorm norm = FFTNorm::BACKWARD) {
+  return FFT2Op{{detail::to_expr(a), {}, 0, MXR_FFT_FORWARD, (int)norm, R - 1, 0}};
+}
+template <typename T, int R>
+FFT2Op ifft2(const tensor_t<T, R> &a, FFTNorm norm = FFTNorm::BACKWARD) {
+  return FFT2Op{{detail::to_expr(a), {}, 0, MXR_FFT_INVERSE, (int)norm, R - 1, 0}};
+}
+template <typename T, int R>
+FFT2Op rfft2(const tensor_t<T, R> &a, FFTNorm norm = FFTNorm::BACKWARD) {
+  return FFT2Op{{detail::to_expr(a), {}, 0, MXR_FFT_FORWARD, (int)norm, R - 1, 1}};
+}
+// out's last extent gives the real length n
+template <typename T, int R>
+FFT2Op irfft2(const tensor_t<T, R> &a, FFTNorm norm = FFTNorm::BACKWARD) {
+  return FFT2Op{{detail::to_expr(a), {}, 0, MXR_FFT_INVERSE, (int)norm, R - 1, 2}};
 }
 // out = ifft(fft(x, n) * conj(fft(w, n)))  (circular cross-correlation)
 template <typename X, typename W>
@@ -594,9 +627,37 @@ struct SetOp {
     if constexpr (std::is_same_v<Op, Expr>) {
       eval_into(out, op, s);
     } else if constexpr (std::is_same_v<Op, FFTOp>) {
-      auto in = operand(op.s.in, op.s.in_shape, MXR_C64, s);
+      auto in = operand(op.s.in, op.s.in_shape, op.s.kind == 1 ? MXR_F32 : MXR_C64, s);
       const int axis = op.s.axis < 0 ? out.rank - 1 : op.s.axis;
-      check(mxr_fft_c2c(&out, &in.t, op.s.n, op.s.dir, op.s.norm, axis, s));
+      if (op.s.kind == 1) check(mxr_fft_r2c(&out, &in.t, op.s.n, op.s.norm, axis, s));
+      else if (op.s.kind == 2) check(mxr_fft_c2r(&out, &in.t, op.s.n, op.s.norm, axis, s));
+      else check(mxr_fft_c2c(&out, &in.t, op.s.n, op.s.dir, op.s.norm, axis, s));
+    } else if constexpr (std::is_same_v<Op, FFT2Op>) {
+      auto in = operand(op.s.in, op.s.in_shape, op.s.kind == 1 ? MXR_F32 : MXR_C64, s);
+      const int r = out.rank;
+      if (op.s.kind == 0) {  // rows, then columns in place
+        check(mxr_fft_c2c(&out, &in.t, 0, op.s.dir, op.s.norm, r - 1, s));
+        check(mxr_fft_c2c(&out, &out, 0, op.s.dir, op.s.norm, r - 2, s));
+      } else if (op.s.kind == 1) {
+        check(mxr_fft_r2c(&out, &in.t, 0, op.s.norm, r - 1, s));
+        check(mxr_fft_c2c(&out, &out, 0, MXR_FFT_FORWARD, op.s.norm, r - 2, s));
+      } else {  // columns into a stream-ordered temporary, then the real rows
+        mxr_tensor t{};
+        t.rank = in.t.rank;
+        t.dtype = MXR_C64;
+        index_t total = 1;
+        for (int d = t.rank - 1; d >= 0; --d) {
+          t.shape[d] = in.t.shape[d];
+          t.stride[d] = total;
+          total *= t.shape[d];
+        }
+        void *p = nullptr;
+        hip_check(hipMallocAsync(&p, 8 * (size_t)total, s), "hipMallocAsync");
+        std::shared_ptr<void> hold(p, [s](void *q) { (void)hipFreeAsync(q, s); });
+        t.data = p;
+        check(mxr_fft_c2c(&t, &in.t, 0, MXR_FFT_INVERSE, op.s.norm, r - 2, s));
+        check(mxr_fft_c2r(&out, &t, out.shape[r - 1], op.s.norm, r - 1, s));
+      }
     } else if constexpr (std::is_same_v<Op, FFTConvOp>) {
       auto x = operand(op.x, {}, MXR_C64, s);
       auto w = operand(op.w, {}, MXR_C64, s);
@@ -622,7 +683,7 @@ struct SetOp {
 template <typename T, int RANK>
 template <typename Op>
 auto tensor_t<T, RANK>::operator=(const Op &op) const {
-  if constexpr (std::is_same_v<Op, FFTOp> || std::is_same_v<Op, FFTConvOp> ||
+  if constexpr (std::is_same_v<Op, FFTOp> || std::is_same_v<Op, FFT2Op> || std::is_same_v<Op, FFTConvOp> ||
                 std::is_same_v<Op, Conv1DOp> || std::is_same_v<Op, Conv2DOp> ||
                 std::is_same_v<Op, SumOp>) {
     return SetOp<tensor_t, Op>{*this, op};

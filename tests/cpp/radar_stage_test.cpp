@@ -73,6 +73,48 @@ int main(int argc, char **argv) {
       threw = true;
     }
     printf(threw ? "size-check ok\n" : "size-check MISSING\n");
+
+    // transform operators on the first 4 x 100 input samples (real parts for
+    // the real-input forms), dumped for the FP64 comparison
+    auto xr = make_tensor<float, 2>({4, 100});
+    auto xc = make_tensor<cfloat, 2>({4, 100});
+    auto fh = make_tensor<cfloat, 1>({9});
+    {
+      std::vector<float> hr(400);
+      std::vector<cfloat> hc(400);
+      for (int i = 0; i < 400; ++i) {
+        hr[i] = x[i].real();
+        hc[i] = x[i];
+      }
+      std::vector<cfloat> hf(x.begin() + 400, x.begin() + 409);
+      detail::hip_check(hipMemcpy(xr.Data(), hr.data(), sizeof(float) * 400, hipMemcpyDefault), "copy");
+      detail::hip_check(hipMemcpy(xc.Data(), hc.data(), sizeof(cfloat) * 400, hipMemcpyDefault), "copy");
+      detail::hip_check(hipMemcpy(fh.Data(), hf.data(), sizeof(cfloat) * 9, hipMemcpyDefault), "copy");
+    }
+    auto rf = make_tensor<cfloat, 2>({4, 51});
+    auto ir = make_tensor<float, 2>({4, 100});
+    auto f2 = make_tensor<cfloat, 2>({4, 100});
+    auto if2 = make_tensor<cfloat, 2>({4, 100});
+    auto rf2 = make_tensor<cfloat, 2>({4, 51});
+    auto ir2 = make_tensor<float, 2>({4, 100});
+    auto cv = make_tensor<cfloat, 2>({4, 108});
+    auto cr = make_tensor<cfloat, 2>({4, 100});
+    (rf = rfft(xr)).run(ex);
+    (ir = irfft(rf, 100)).run(ex);
+    (f2 = fft2(xc)).run(ex);
+    (if2 = ifft2(f2)).run(ex);
+    (rf2 = rfft2(xr)).run(ex);
+    (ir2 = irfft2(rf2)).run(ex);
+    (cv = conv1d(xc, fh, MATX_C_MODE_FULL, MATX_C_METHOD_FFT)).run(ex);
+    (cr = corr(xc, fh, MATX_C_MODE_SAME, MATX_C_METHOD_FFT)).run(ex);
+    dump(dir + "/op_rfft.bin", rf, ex);
+    dump(dir + "/op_irfft.bin", ir, ex);
+    dump(dir + "/op_fft2.bin", f2, ex);
+    dump(dir + "/op_ifft2.bin", if2, ex);
+    dump(dir + "/op_rfft2.bin", rf2, ex);
+    dump(dir + "/op_irfft2.bin", ir2, ex);
+    dump(dir + "/op_conv.bin", cv, ex);
+    dump(dir + "/op_corr.bin", cr, ex);
   } catch (const matxException &e) {
     fprintf(stderr, "matx error %d: %s\n", e.code(), e.what());
     return 1;

@@ -48,3 +48,20 @@ def test_cpp_radar_stages(orc, matx_inputs, golden, tmp_path):
         near = np.abs(st["margin"][0]) < 1e-5
         assert ((dets[c] != st["dets"][0]) & ~near).sum() == 0
         assert np.array_equal(np.argwhere(dets[c] == 1), golden["matx_det_idx"])
+    # transform operators (rfft/irfft, fft2/ifft2, rfft2/irfft2, conv1d by the
+    # FFT method, corr) on the first 4 x 100 samples, against numpy FP64
+    xs = x.reshape(-1)[:400].astype(np.complex128).reshape(4, 100)
+    xr, fh = xs.real.astype(np.float32).astype(np.float64), x.reshape(-1)[400:409].astype(np.complex128)
+
+    def op(name, dt, shape):
+        return np.fromfile(tmp_path / f"op_{name}.bin", dt).reshape(shape)
+    assert rel(op("rfft", np.complex64, (4, 51)), np.fft.rfft(xr)) < 1e-5
+    assert rel(op("irfft", np.float32, (4, 100)), xr) < 1e-5
+    assert rel(op("fft2", np.complex64, (4, 100)), np.fft.fft2(xs)) < 1e-5
+    assert rel(op("ifft2", np.complex64, (4, 100)), xs) < 1e-5
+    assert rel(op("rfft2", np.complex64, (4, 51)), np.fft.rfft2(xr)) < 1e-5
+    assert rel(op("irfft2", np.float32, (4, 100)), xr) < 1e-5
+    conv = np.array([np.convolve(r, fh) for r in xs])
+    assert rel(op("conv", np.complex64, (4, 108)), conv) < 1e-5
+    corr = np.array([np.convolve(r, np.conj(fh[::-1]))[4:104] for r in xs])  # SAME start (9-1)/2
+    assert rel(op("corr", np.complex64, (4, 100)), corr) < 1e-5
