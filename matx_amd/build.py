@@ -78,7 +78,9 @@ def build(verbose: bool = True, jobs: int | None = None, incremental: bool = Fal
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(lambda f: _compile(f, incremental), srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    rocm_lib = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs + [
+        "-L" + rocm_lib, "-lhiprtc", "-Wl,-rpath," + rocm_lib]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)

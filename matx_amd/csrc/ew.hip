@@ -3,14 +3,21 @@
 // MatX fuses an elementwise expression tree into one templated kernel per
 // (lhs = expr).run(exec) at compile time (include/matx/executors/kernel.h).
 // Behind a C ABI the expression cannot be a C++ type, so the front end lowers
-// it to a short postfix program and one precompiled kernel interprets it per
-// element.  The program is the same for every lane, so its decode and
+// it to a short postfix program.  By default the program is turned into HIP
+// source and compiled once per expression shape (the expression compiler
+// below); otherwise one precompiled kernel interprets it per element.  The program is the same for every lane, so its decode and
 // branches run on the scalar unit; only the arithmetic is per lane.  Each
 // lane handles EW_EPT consecutive elements of the innermost dimension, so the
 // index decomposition is paid once per run of elements.
+#include <hip/hiprtc.h>
+
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
 
 #include "mxr_cache.hpp"
 #include "mxr_internal.hpp"
@@ -201,6 +208,277 @@ static mxr_status ew_validate_program(const mxr_ew_instr *prog, int nprog, int n
   return MXR_SUCCESS;
 }
 
+// ------------------------------------------------------ expression compiler
+// The default path generates one HIP kernel per expression shape (program,
+// operand dtypes, rank, which constants are real) as straight-line source,
+// compiles it with hipRTC for the current device and caches the module: the
+// MatX analogue is the compile-time fused kernel of kernel.h and the NVRTC
+// string generator of executors/jit_cuda.h:325-420.  Values known to be real
+// at generation time (real inputs, real constants, comparisons, |.|^2, ...)
+// stay single floats, so a real expression does real arithmetic only.
+// Constants, pointers, shape and strides are kernel arguments, so a new
+// constant value or view reuses the cached kernel.  The interpreter above
+// runs when the compiler is unavailable or MXR_EW_JIT=0.
+struct EwjArgs {
+  void *out;
+  const void *in[EW_MAXIN];
+  long long shape[MXR_MAX_RANK];
+  long long ost[MXR_MAX_RANK];
+  long long ist[EW_MAXIN][MXR_MAX_RANK];
+  long long total_runs, runs_per_row;
+  float cst[2 * EW_MAXCONST];
+};
+
+// Device source shared by every generated kernel: the argument block (the
+// same layout as EwjArgs) and the complex helpers of the interpreter.
+static const char *kEwjPrelude = R"(
+struct EwjArgs {
+  void *out;
+  const void *in[EWJ_MAXIN];
+  long long shape[EWJ_RANKMAX];
+  long long ost[EWJ_RANKMAX];
+  long long ist[EWJ_MAXIN][EWJ_RANKMAX];
+  long long total_runs, runs_per_row;
+  float cst[2 * EWJ_MAXCONST];
+};
+__device__ __forceinline__ float2 mk(float x, float y) { float2 r; r.x = x; r.y = y; return r; }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return mk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cdiv(float2 a, float2 b) {
+  const float d = b.x * b.x + b.y * b.y;
+  return mk((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ float ham(long long i, int n) {
+  if (n <= 1) return 1.0f;
+  return 0.54f - 0.46f * cosf(6.28318530717958647692f * (float)i / (float)(n - 1));
+}
+)";
+
+namespace {
+struct EwjVal {
+  std::string name;
+  bool real;
+};
+
+std::string ewj_c(const EwjVal &v) { return v.real ? "mk(" + v.name + ", 0.f)" : v.name; }
+std::string ewj_re(const EwjVal &v) { return v.real ? v.name : v.name + ".x"; }
+
+// Straight-line source of one program; `creal` marks real constants.
+std::string ewj_source(const mxr_ew_instr *prog, int nprog, const int *idt, int nin, int odt,
+                       int rank, unsigned creal) {
+  std::ostringstream k;
+  k << "#define EWJ_MAXIN " << EW_MAXIN << "\n#define EWJ_RANKMAX " << MXR_MAX_RANK
+    << "\n#define EWJ_MAXCONST " << EW_MAXCONST << "\n"
+    << kEwjPrelude
+    << "extern \"C\" __global__ __launch_bounds__(256) void mxr_ew_jit(EwjArgs a) {\n"
+    << "  const long long run = (long long)blockIdx.x * 256 + threadIdx.x;\n"
+    << "  if (run >= a.total_runs) return;\n"
+    << "  long long idx[" << rank << "];\n"
+    << "  long long rem = run / a.runs_per_row;\n"
+    << "  idx[" << rank - 1 << "] = (run % a.runs_per_row) * " << EW_EPT << ";\n";
+  for (int d = rank - 2; d >= 0; --d)
+    k << "  idx[" << d << "] = rem % a.shape[" << d << "]; rem /= a.shape[" << d << "];\n";
+  k << "  long long oo = 0;\n";
+  for (int i = 0; i < nin; ++i) k << "  long long io" << i << " = 0;\n";
+  for (int d = 0; d < rank; ++d) {
+    k << "  oo += idx[" << d << "] * a.ost[" << d << "];\n";
+    for (int i = 0; i < nin; ++i)
+      k << "  io" << i << " += idx[" << d << "] * a.ist[" << i << "][" << d << "];\n";
+  }
+  k << "  const long long last = idx[" << rank - 1 << "];\n"
+    << "  const long long left = a.shape[" << rank - 1 << "] - last;\n"
+    << "  const int cnt = left < " << EW_EPT << " ? (int)left : " << EW_EPT << ";\n"
+    << "#pragma unroll\n"
+    << "  for (int e = 0; e < " << EW_EPT << "; ++e) {\n"
+    << "    if (e >= cnt) break;\n";
+  std::vector<EwjVal> st;
+  int nv = 0;
+  auto def = [&](bool real, const std::string &expr) {
+    EwjVal v{"v" + std::to_string(nv++), real};
+    k << "    const " << (real ? "float " : "float2 ") << v.name << " = " << expr << ";\n";
+    st.push_back(v);
+  };
+  auto pos = [&](int d) {
+    return d == rank - 1 ? std::string("(last + e)") : "idx[" + std::to_string(d) + "]";
+  };
+  for (int pc = 0; pc < nprog; ++pc) {
+    const mxr_ew_instr &ins = prog[pc];
+    switch (ins.op) {
+      case MXR_EW_LOAD: {
+        const int i = ins.a;
+        const std::string off = "io" + std::to_string(i) + " + e * a.ist[" + std::to_string(i) + "][" +
+                                std::to_string(rank - 1) + "]";
+        const std::string p = "a.in[" + std::to_string(i) + "]";
+        if (idt[i] == MXR_C64) def(false, "((const float2 *)" + p + ")[" + off + "]");
+        else if (idt[i] == MXR_F32) def(true, "((const float *)" + p + ")[" + off + "]");
+        else def(true, "(float)((const int *)" + p + ")[" + off + "]");
+        break;
+      }
+      case MXR_EW_CONST: {
+        const std::string c = std::to_string(2 * ins.a);
+        if (creal >> ins.a & 1u) def(true, "a.cst[" + c + "]");
+        else def(false, "mk(a.cst[" + c + "], a.cst[" + c + " + 1])");
+        break;
+      }
+      case MXR_EW_HAMMING: def(true, "ham(" + pos(ins.a) + ", " + std::to_string(ins.b) + ")"); break;
+      case MXR_EW_INDEX: def(true, "(float)" + pos(ins.a)); break;
+      case MXR_EW_SELECT: {
+        const EwjVal c = st.back(); st.pop_back();
+        const EwjVal y = st.back(); st.pop_back();
+        const EwjVal x = st.back(); st.pop_back();
+        if (x.real && y.real) def(true, "(" + ewj_re(c) + " != 0.f) ? " + x.name + " : " + y.name);
+        else def(false, "(" + ewj_re(c) + " != 0.f) ? " + ewj_c(x) + " : " + ewj_c(y));
+        break;
+      }
+      default: {
+        if (ins.op >= MXR_EW_ADD && ins.op <= MXR_EW_MIN) {
+          const EwjVal y = st.back(); st.pop_back();
+          const EwjVal x = st.back(); st.pop_back();
+          const bool rr = x.real && y.real;
+          const std::string X = ewj_c(x), Y = ewj_c(y), xr = ewj_re(x), yr = ewj_re(y);
+          switch (ins.op) {
+            case MXR_EW_ADD: rr ? def(true, x.name + " + " + y.name) : def(false, "cadd(" + X + ", " + Y + ")"); break;
+            case MXR_EW_SUB: rr ? def(true, x.name + " - " + y.name) : def(false, "csub(" + X + ", " + Y + ")"); break;
+            case MXR_EW_MUL:
+              if (rr) def(true, x.name + " * " + y.name);
+              else if (x.real) def(false, "mk(" + x.name + " * " + y.name + ".x, " + x.name + " * " + y.name + ".y)");
+              else if (y.real) def(false, "mk(" + x.name + ".x * " + y.name + ", " + x.name + ".y * " + y.name + ")");
+              else def(false, "cmul(" + x.name + ", " + y.name + ")");
+              break;
+            case MXR_EW_DIV:
+              if (rr) def(true, x.name + " / " + y.name);
+              else if (y.real) def(false, "mk(" + X + ".x / " + y.name + ", " + X + ".y / " + y.name + ")");
+              else def(false, "(" + y.name + ".y == 0.f) ? mk(" + X + ".x / " + y.name + ".x, " + X + ".y / " +
+                                   y.name + ".x) : cdiv(" + X + ", " + y.name + ")");
+              break;
+            case MXR_EW_POW: def(true, "powf(" + xr + ", " + yr + ")"); break;
+            case MXR_EW_GT: def(true, "(" + xr + " > " + yr + ") ? 1.f : 0.f"); break;
+            case MXR_EW_LT: def(true, "(" + xr + " < " + yr + ") ? 1.f : 0.f"); break;
+            case MXR_EW_GE: def(true, "(" + xr + " >= " + yr + ") ? 1.f : 0.f"); break;
+            case MXR_EW_LE: def(true, "(" + xr + " <= " + yr + ") ? 1.f : 0.f"); break;
+            case MXR_EW_EQ:
+              if (rr) def(true, "(" + x.name + " == " + y.name + ") ? 1.f : 0.f");
+              else def(true, "(" + X + ".x == " + Y + ".x && " + X + ".y == " + Y + ".y) ? 1.f : 0.f");
+              break;
+            case MXR_EW_MAX: def(true, "fmaxf(" + xr + ", " + yr + ")"); break;
+            default: def(true, "fminf(" + xr + ", " + yr + ")"); break;
+          }
+        } else {
+          const EwjVal x = st.back(); st.pop_back();
+          const std::string X = ewj_c(x), xr = ewj_re(x), n = x.name;
+          switch (ins.op) {
+            case MXR_EW_CONJ: x.real ? def(true, n) : def(false, "mk(" + n + ".x, -" + n + ".y)"); break;
+            case MXR_EW_ABS2: x.real ? def(true, n + " * " + n) : def(true, n + ".x * " + n + ".x + " + n + ".y * " + n + ".y"); break;
+            case MXR_EW_ABS:
+              x.real ? def(true, "sqrtf(" + n + " * " + n + ")")
+                     : def(true, "sqrtf(" + n + ".x * " + n + ".x + " + n + ".y * " + n + ".y)");
+              break;
+            case MXR_EW_SQRT: def(true, "sqrtf(" + xr + ")"); break;
+            case MXR_EW_NEG: x.real ? def(true, "-" + n) : def(false, "mk(-" + n + ".x, -" + n + ".y)"); break;
+            case MXR_EW_REAL: def(true, xr); break;
+            case MXR_EW_IMAG: x.real ? def(true, "0.f") : def(true, n + ".y"); break;
+            case MXR_EW_EXP:
+              if (x.real) def(true, "expf(" + n + ")");
+              else def(false, "mk(expf(" + n + ".x) * cosf(" + n + ".y), expf(" + n + ".x) * sinf(" + n + ".y))");
+              break;
+            case MXR_EW_LOG:
+              def(false, "mk(logf(sqrtf(" + X + ".x * " + X + ".x + " + X + ".y * " + X + ".y)), atan2f(" + X +
+                             ".y, " + X + ".x))");
+              break;
+            case MXR_EW_COS: def(true, "cosf(" + xr + ")"); break;
+            default: def(true, "sinf(" + xr + ")"); break;
+          }
+        }
+      }
+    }
+  }
+  const EwjVal v = st.back();
+  const std::string o = "oo + e * a.ost[" + std::to_string(rank - 1) + "]";
+  if (odt == MXR_C64) k << "    ((float2 *)a.out)[" << o << "] = " << ewj_c(v) << ";\n";
+  else if (odt == MXR_F32) k << "    ((float *)a.out)[" << o << "] = " << ewj_re(v) << ";\n";
+  else k << "    ((int *)a.out)[" << o << "] = (int)" << ewj_re(v) << ";\n";
+  k << "  }\n}\n";
+  return k.str();
+}
+
+struct EwjKernel {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+  bool failed = false;
+};
+std::mutex g_ewj_mu;
+std::map<std::pair<int, std::string>, EwjKernel> g_ewj;
+
+bool ewj_enabled() {
+  static const bool on = [] {
+    const char *e = getenv("MXR_EW_JIT");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// Compiles `src` for the current device with the library's code-generation
+// flags (packed FP32 instructions off).  Empty on failure.
+std::vector<char> ewj_compile(const std::string &src) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device_id()) != hipSuccess) return {};
+  std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
+  hiprtcProgram p;
+  if (hiprtcCreateProgram(&p, src.c_str(), "mxr_ew_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+    return {};
+  const char *opts[] = {arch.c_str(), "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  std::vector<char> code;
+  if (hiprtcCompileProgram(p, 6, opts) == HIPRTC_SUCCESS) {
+    size_t n = 0;
+    if (hiprtcGetCodeSize(p, &n) == HIPRTC_SUCCESS && n > 0) {
+      code.resize(n);
+      if (hiprtcGetCode(p, code.data()) != HIPRTC_SUCCESS) code.clear();
+    }
+  }
+  (void)hiprtcDestroyProgram(&p);
+  return code;
+}
+}  // namespace
+
+// The generated kernel of this expression shape, compiled on first use;
+// nullptr when the compiler is off or failed (the interpreter runs then).
+static hipFunction_t ewj_kernel(const mxr_ew_instr *prog, int nprog, const int *idt, int nin,
+                                int odt, int rank, unsigned creal) {
+  if (!ewj_enabled()) return nullptr;
+  std::ostringstream key;
+  key << rank << '|' << odt << '|' << creal << '|';
+  for (int i = 0; i < nin; ++i) key << idt[i] << ',';
+  key << '|';
+  for (int i = 0; i < nprog; ++i) key << prog[i].op << ':' << prog[i].a << ':' << prog[i].b << ';';
+  std::lock_guard<std::mutex> g(g_ewj_mu);
+  EwjKernel &kk = g_ewj[{device_id(), key.str()}];
+  if (kk.fn || kk.failed) return kk.fn;
+  const std::vector<char> code = ewj_compile(ewj_source(prog, nprog, idt, nin, odt, rank, creal));
+  if (code.empty() || hipModuleLoadData(&kk.mod, code.data()) != hipSuccess ||
+      hipModuleGetFunction(&kk.fn, kk.mod, "mxr_ew_jit") != hipSuccess) {
+    kk.fn = nullptr;
+    kk.failed = true;
+  }
+  return kk.fn;
+}
+
+static void clear_ewj() {
+  std::lock_guard<std::mutex> g(g_ewj_mu);
+  for (auto &kv : g_ewj) {
+    if (!kv.second.mod) continue;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(kv.first.first);
+    (void)hipModuleUnload(kv.second.mod);
+    (void)hipSetDevice(cur);
+  }
+  g_ewj.clear();
+}
+static CacheClearRegistrar g_reg_ewj(clear_ewj);
+
 mxr_status ew_impl(const mxr_tensor *out, const mxr_tensor *const *ins, int nin,
                    const mxr_ew_instr *prog, int nprog, const float *consts, int nconst,
                    hipStream_t stream) {
@@ -237,6 +515,32 @@ mxr_status ew_impl(const mxr_tensor *out, const mxr_tensor *const *ins, int nin,
   if (numel(*out) == 0) return MXR_SUCCESS;
   a.runs_per_row = (n_last + EW_EPT - 1) / EW_EPT;
   a.total_runs = (numel(*out) / n_last) * a.runs_per_row;
+  const int64_t blocks = (a.total_runs + 255) / 256;
+  MXR_REQUIRE(blocks <= 0x7fffffff, MXR_INVALID_SIZE, "ew: too large");
+  int idt[EW_MAXIN] = {0};
+  for (int k = 0; k < nin; ++k) idt[k] = ins[k]->dtype;
+  unsigned creal = 0;
+  for (int i = 0; i < nconst; ++i)
+    if (consts[2 * i + 1] == 0.f) creal |= 1u << i;
+  if (hipFunction_t fn = ewj_kernel(prog, nprog, idt, nin, out->dtype, out->rank, creal)) {
+    EwjArgs j{};
+    j.out = out->data;
+    for (int k = 0; k < nin; ++k) {
+      j.in[k] = ins[k]->data;
+      for (int d = 0; d < out->rank; ++d) j.ist[k][d] = ins[k]->stride[d];
+    }
+    for (int d = 0; d < out->rank; ++d) {
+      j.shape[d] = out->shape[d];
+      j.ost[d] = out->stride[d];
+    }
+    j.total_runs = a.total_runs;
+    j.runs_per_row = a.runs_per_row;
+    for (int i = 0; i < 2 * nconst; ++i) j.cst[i] = consts[i];
+    void *params[] = {&j};
+    ProfScope ps("ew_jit_kernel", stream);
+    MXR_HIP_TRY(hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, 256, 1, 1, 0, stream, params, nullptr));
+    return MXR_SUCCESS;
+  }
   // the program and views travel through a per-stream device ring of
   // argument slots (an in-order stream never reuses a slot before the launch
   // that read it has run, as long as fewer than `slots` launches are queued)
@@ -252,8 +556,6 @@ mxr_status ew_impl(const mxr_tensor *out, const mxr_tensor *const *ins, int nin,
     ring.next = (ring.next + 1) % ring.slots;
   }
   MXR_HIP_TRY(hipMemcpyAsync(slot, &a, sizeof(EwArgs), hipMemcpyHostToDevice, stream));
-  const int64_t blocks = (a.total_runs + 255) / 256;
-  MXR_REQUIRE(blocks <= 0x7fffffff, MXR_INVALID_SIZE, "ew: too large");
   {
     ProfScope ps("ew_kernel", stream);
     ew_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(slot);
