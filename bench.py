@@ -51,14 +51,16 @@ def parse():
     ap.add_argument("--no-profile", action="store_true",
                     help="time without per-kernel events (roofline omitted)")
     ap.add_argument("--workload", default="radar",
-                    choices=["radar", "fftconv", "large", "grad", "ragged", "conv"],
+                    choices=["radar", "fftconv", "large", "grad", "ragged", "conv", "ew"],
                     help="radar: the 16x128x9000 pipeline (default); fftconv: fft_conv on "
                          "4x32x2048 with a 256-sample chirp at N=4096 (us per call); large: "
                          "the 64x512x32768 pipeline (8.6 GB cube); grad: waveform/window "
                          "design loss forward + backward (grad steps/s); ragged: 256 uneven "
                          "pulse-compression + Doppler jobs, batched vs per-job launches; "
                          "conv: conv1d by the FFT method on 16 x 1048576 c64 with 209 taps "
-                         "(MatX's large ConvCorr size) and 1000-point Bluestein FFTs")
+                         "(MatX's large ConvCorr size) and 1000-point Bluestein FFTs; ew: fused "
+                         "elementwise expressions at the radar cube's size (calcDets threshold, "
+                         "Doppler window), generated kernels or, with MXR_EW_JIT=0, the interpreter")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 512 MiB device copies (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -430,6 +432,98 @@ def bench_conv(args):
     }))
 
 
+def bench_ew(args):
+    """Fused elementwise expressions (mxr_ew_eval) at the 16 x 128 x 8001
+    radar cube's size: the calcDets threshold of
+    examples/simple_radar_pipeline.h:86-98, dets = |x|^2 > n (pfa^(-1/n) - 1) ba
+    (x c64, n and ba f32, dets i32), and the in-place Doppler window
+    x *= hamming<1>(126) (:353-355).  Generated kernels by default; the
+    interpreter kernel when MXR_EW_JIT=0."""
+    import numpy as np
+    import torch
+    from matx_amd import expr as E
+    dev = "cuda:0"
+    C, P, R = 16, 128, 8001
+    g = torch.Generator(device="cpu").manual_seed(args.seed)
+    x = torch.complex(torch.randn((C, P, R), generator=g), torch.randn((C, P, R), generator=g)).to(dev)
+    nrm = torch.randint(20, 57, (C, P, R), generator=g).to(torch.float32).to(dev)
+    ba = (torch.rand((C, P, R), generator=g) * 2).to(dev)
+    dets = torch.empty((C, P, R), dtype=torch.int32, device=dev)
+    xw = x.clone()[:, :126, :]  # the window rewrites its operand: a copy
+    thr = lambda: E.run(dets, E.abs2(x) > E.wrap(nrm) * (E.pow(1e-5, -1.0 / E.wrap(nrm)) - 1.0) * ba)  # noqa: E731
+    win = lambda: E.run(xw, E.wrap(xw) * E.hamming(1, 126))  # noqa: E731
+    t0 = time.perf_counter()
+    thr()
+    torch.cuda.synchronize()
+    first_ms = (time.perf_counter() - t0) * 1000.0
+    k = max(args.steps, 50)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(k):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return 1000.0 * e0.elapsed_time(e1) / k
+
+    us_thr_eager, us_win = timed(thr), timed(win)
+    # value: the threshold replayed from a hipGraph of 20 captured calls (the
+    # eager figure above includes the Python front end's per-call lowering)
+    G = 20
+    gs = torch.cuda.Stream()
+    gs.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=gs):
+        for _ in range(G):
+            thr()
+    torch.cuda.synchronize()
+    graph.replay()
+    torch.cuda.synchronize()
+    reps = max(1, k // G)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    us_thr = 1000.0 * e0.elapsed_time(e1) / (reps * G)
+    del graph
+    B_thr = (8 + 4 + 4 + 4) * C * P * R
+    B_win = 2 * 8 * C * 126 * R
+    jit = os.environ.get("MXR_EW_JIT", "1") != "0"
+    name = "ew_jit_kernel" if jit else "ew_kernel"
+    kern = profiled_kernels(thr, 10, [name])
+    roof = kernel_roofline(kern, {name: B_thr})
+    # check one channel of the threshold against FP64
+    xs = x[0].cpu().numpy().astype(np.complex128)
+    ns, bs = (t[0].cpu().numpy().astype(np.float64) for t in (nrm, ba))
+    ref = (np.abs(xs) ** 2 > ns * (np.power(1e-5, -1.0 / ns) - 1.0) * bs)
+    got = dets[0].cpu().numpy().astype(bool)
+    pw, th = np.abs(xs) ** 2, ns * (np.power(1e-5, -1.0 / ns) - 1.0) * bs
+    near = np.abs(pw - th) <= 1e-5 * th
+    mismatch = int(((got != ref) & ~near).sum())
+    print(json.dumps({
+        "metric": "fused elementwise expression GB/s (calcDets threshold, 16x128x8001)",
+        "value": round(B_thr / (us_thr * 1e-6) / 1e9, 1), "unit": "GB/s", "n_gpus": 1, "steps": k,
+        "warmup": 3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64/f32", "data": "synthetic (complex noise, integer norms, uniform averages, seeded)",
+        "config": {"workload": "dets = |x|^2 > n (pfa^(-1/n) - 1) ba over 16 x 128 x 8001 cells; "
+                               "x *= hamming<1>(126) in place over 16 x 126 x 8001",
+                   "path": "generated kernels (hipRTC)" if jit else "interpreter kernel",
+                   "us_threshold": round(us_thr, 2), "us_threshold_eager": round(us_thr_eager, 2),
+                   "us_window_eager": round(us_win, 2),
+                   "window_gbs": round(B_win / (us_win * 1e-6) / 1e9, 1),
+                   "first_call_ms_incl_compile": round(first_ms, 1),
+                   "mismatches_vs_fp64_channel0": mismatch},
+        "roofline": roof,
+        "cpu_baseline": None,
+    }))
+
+
 def bench_grad(args):
     """Waveform/window design: 64 phase-only 1024-sample waveforms + a
     128-tap Doppler window (Hamming start); loss = mean integrated sidelobe
@@ -700,6 +794,9 @@ def main():
         return
     if args.workload == "conv":
         bench_conv(args)
+        return
+    if args.workload == "ew":
+        bench_ew(args)
         return
     bench_pipeline(args, WORKLOADS[args.workload])
 

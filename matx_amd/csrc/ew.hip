@@ -45,6 +45,7 @@ struct EwArgs {
   mxr_ew_instr prog[EW_MAXPROG];
   float2 cst[EW_MAXCONST];
 };
+static_assert(sizeof(EwArgs) <= 4096, "the argument block must fit the kernel arguments");
 
 __device__ __forceinline__ float2 ew_load(const EwIn &in, int64_t off) {
   switch (in.dtype) {
@@ -60,8 +61,10 @@ __device__ __forceinline__ float2 cdiv(float2 a, float2 b) {
   return make_float2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
 }
 
-__global__ __launch_bounds__(256) void ew_kernel(const EwArgs *__restrict__ ap) {
-  const EwArgs &a = *ap;
+// The argument block (about 1.5 KB) travels by value in the kernel
+// arguments, so a launch holds no pointer to host memory (and can be
+// captured into a graph).
+__global__ __launch_bounds__(256) void ew_kernel(const EwArgs a) {
   const int64_t run = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (run >= a.total_runs) return;
   const int r = a.rank;
@@ -161,17 +164,6 @@ __global__ __launch_bounds__(256) void ew_kernel(const EwArgs *__restrict__ ap) 
   }
 }
 
-// Kernel arguments live in a small per-stream device ring so a launch never
-// depends on host memory staying alive.
-namespace {
-struct ArgRing {
-  EwArgs *dev = nullptr;
-  int slots = 0, next = 0;
-};
-std::mutex g_ring_mu;
-std::map<std::pair<int, uintptr_t>, ArgRing> g_rings;
-}  // namespace
-
 static mxr_status ew_validate_program(const mxr_ew_instr *prog, int nprog, int nin, int nconst,
                                       int rank) {
   MXR_REQUIRE(nprog >= 1 && nprog <= EW_MAXPROG, MXR_INVALID_PARAMETER,
@@ -225,9 +217,12 @@ struct EwjArgs {
   long long shape[MXR_MAX_RANK];
   long long ost[MXR_MAX_RANK];
   long long ist[EW_MAXIN][MXR_MAX_RANK];
-  long long total_runs, runs_per_row;
+  long long cblocks;  // workgroups per row (EWJ_COLS columns each)
+  long long pad;
   float cst[2 * EW_MAXCONST];
 };
+// Columns of one row per generated-kernel workgroup: 256 lanes x 4.
+constexpr int EWJ_COLS = 1024;
 
 // Device source shared by every generated kernel: the argument block (the
 // same layout as EwjArgs) and the complex helpers of the interpreter.
@@ -238,7 +233,8 @@ struct EwjArgs {
   long long shape[EWJ_RANKMAX];
   long long ost[EWJ_RANKMAX];
   long long ist[EWJ_MAXIN][EWJ_RANKMAX];
-  long long total_runs, runs_per_row;
+  long long cblocks;
+  long long pad;
   float cst[2 * EWJ_MAXCONST];
 };
 __device__ __forceinline__ float2 mk(float x, float y) { float2 r; r.x = x; r.y = y; return r; }
@@ -273,27 +269,29 @@ std::string ewj_source(const mxr_ew_instr *prog, int nprog, const int *idt, int 
   k << "#define EWJ_MAXIN " << EW_MAXIN << "\n#define EWJ_RANKMAX " << MXR_MAX_RANK
     << "\n#define EWJ_MAXCONST " << EW_MAXCONST << "\n"
     << kEwjPrelude
+    // a workgroup owns EWJ_COLS columns of one row: the row's coordinates and
+    // base offsets are workgroup-uniform (scalar unit), and consecutive lanes
+    // take consecutive columns (coalesced)
     << "extern \"C\" __global__ __launch_bounds__(256) void mxr_ew_jit(EwjArgs a) {\n"
-    << "  const long long run = (long long)blockIdx.x * 256 + threadIdx.x;\n"
-    << "  if (run >= a.total_runs) return;\n"
+    << "  const long long row = (long long)blockIdx.x / a.cblocks;\n"
+    << "  const long long col0 = ((long long)blockIdx.x - row * a.cblocks) * " << EWJ_COLS << ";\n"
     << "  long long idx[" << rank << "];\n"
-    << "  long long rem = run / a.runs_per_row;\n"
-    << "  idx[" << rank - 1 << "] = (run % a.runs_per_row) * " << EW_EPT << ";\n";
+    << "  long long rem = row;\n"
+    << "  idx[" << rank - 1 << "] = 0;\n";
   for (int d = rank - 2; d >= 0; --d)
     k << "  idx[" << d << "] = rem % a.shape[" << d << "]; rem /= a.shape[" << d << "];\n";
   k << "  long long oo = 0;\n";
   for (int i = 0; i < nin; ++i) k << "  long long io" << i << " = 0;\n";
-  for (int d = 0; d < rank; ++d) {
+  for (int d = 0; d < rank - 1; ++d) {
     k << "  oo += idx[" << d << "] * a.ost[" << d << "];\n";
     for (int i = 0; i < nin; ++i)
       k << "  io" << i << " += idx[" << d << "] * a.ist[" << i << "][" << d << "];\n";
   }
-  k << "  const long long last = idx[" << rank - 1 << "];\n"
-    << "  const long long left = a.shape[" << rank - 1 << "] - last;\n"
-    << "  const int cnt = left < " << EW_EPT << " ? (int)left : " << EW_EPT << ";\n"
+  k << "  const long long ncol = a.shape[" << rank - 1 << "];\n"
     << "#pragma unroll\n"
-    << "  for (int e = 0; e < " << EW_EPT << "; ++e) {\n"
-    << "    if (e >= cnt) break;\n";
+    << "  for (int e = 0; e < " << EWJ_COLS / 256 << "; ++e) {\n"
+    << "    const long long c = col0 + threadIdx.x + e * 256;\n"
+    << "    if (c >= ncol) break;\n";
   std::vector<EwjVal> st;
   int nv = 0;
   auto def = [&](bool real, const std::string &expr) {
@@ -302,14 +300,14 @@ std::string ewj_source(const mxr_ew_instr *prog, int nprog, const int *idt, int 
     st.push_back(v);
   };
   auto pos = [&](int d) {
-    return d == rank - 1 ? std::string("(last + e)") : "idx[" + std::to_string(d) + "]";
+    return d == rank - 1 ? std::string("c") : "idx[" + std::to_string(d) + "]";
   };
   for (int pc = 0; pc < nprog; ++pc) {
     const mxr_ew_instr &ins = prog[pc];
     switch (ins.op) {
       case MXR_EW_LOAD: {
         const int i = ins.a;
-        const std::string off = "io" + std::to_string(i) + " + e * a.ist[" + std::to_string(i) + "][" +
+        const std::string off = "io" + std::to_string(i) + " + c * a.ist[" + std::to_string(i) + "][" +
                                 std::to_string(rank - 1) + "]";
         const std::string p = "a.in[" + std::to_string(i) + "]";
         if (idt[i] == MXR_C64) def(false, "((const float2 *)" + p + ")[" + off + "]");
@@ -396,7 +394,7 @@ std::string ewj_source(const mxr_ew_instr *prog, int nprog, const int *idt, int 
     }
   }
   const EwjVal v = st.back();
-  const std::string o = "oo + e * a.ost[" + std::to_string(rank - 1) + "]";
+  const std::string o = "oo + c * a.ost[" + std::to_string(rank - 1) + "]";
   if (odt == MXR_C64) k << "    ((float2 *)a.out)[" << o << "] = " << ewj_c(v) << ";\n";
   else if (odt == MXR_F32) k << "    ((float *)a.out)[" << o << "] = " << ewj_re(v) << ";\n";
   else k << "    ((int *)a.out)[" << o << "] = (int)" << ewj_re(v) << ";\n";
@@ -533,32 +531,18 @@ mxr_status ew_impl(const mxr_tensor *out, const mxr_tensor *const *ins, int nin,
       j.shape[d] = out->shape[d];
       j.ost[d] = out->stride[d];
     }
-    j.total_runs = a.total_runs;
-    j.runs_per_row = a.runs_per_row;
+    j.cblocks = (n_last + EWJ_COLS - 1) / EWJ_COLS;
+    const int64_t jgrid = (numel(*out) / n_last) * j.cblocks;
+    MXR_REQUIRE(jgrid <= 0x7fffffff, MXR_INVALID_SIZE, "ew: too large");
     for (int i = 0; i < 2 * nconst; ++i) j.cst[i] = consts[i];
     void *params[] = {&j};
     ProfScope ps("ew_jit_kernel", stream);
-    MXR_HIP_TRY(hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, 256, 1, 1, 0, stream, params, nullptr));
+    MXR_HIP_TRY(hipModuleLaunchKernel(fn, (unsigned)jgrid, 1, 1, 256, 1, 1, 0, stream, params, nullptr));
     return MXR_SUCCESS;
   }
-  // the program and views travel through a per-stream device ring of
-  // argument slots (an in-order stream never reuses a slot before the launch
-  // that read it has run, as long as fewer than `slots` launches are queued)
-  EwArgs *slot = nullptr;
-  {
-    std::lock_guard<std::mutex> g(g_ring_mu);
-    ArgRing &ring = g_rings[{device_id(), (uintptr_t)stream}];
-    if (!ring.dev) {
-      ring.slots = 1024;
-      MXR_HIP_TRY(hipMalloc(&ring.dev, sizeof(EwArgs) * ring.slots));
-    }
-    slot = ring.dev + ring.next;
-    ring.next = (ring.next + 1) % ring.slots;
-  }
-  MXR_HIP_TRY(hipMemcpyAsync(slot, &a, sizeof(EwArgs), hipMemcpyHostToDevice, stream));
   {
     ProfScope ps("ew_kernel", stream);
-    ew_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(slot);
+    ew_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(a);
   }
   MXR_HIP_TRY(hipGetLastError());
   return MXR_SUCCESS;
@@ -624,17 +608,6 @@ std::mutex g_sum_mu;
 std::map<std::pair<int, uintptr_t>, SumWs> g_sum;
 
 void clear_ew() {
-  {
-    std::lock_guard<std::mutex> g(g_ring_mu);
-    for (auto &kv : g_rings) {
-      int cur = 0;
-      (void)hipGetDevice(&cur);
-      (void)hipSetDevice(kv.first.first);
-      (void)hipFree(kv.second.dev);
-      (void)hipSetDevice(cur);
-    }
-    g_rings.clear();
-  }
   std::lock_guard<std::mutex> g(g_sum_mu);
   for (auto &kv : g_sum) {
     int cur = 0;

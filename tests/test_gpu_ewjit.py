@@ -82,8 +82,13 @@ def cases():
          torch.float32),
         ("cconst", lambda E, a, b, r: E.wrap(r) * (0.5 - 2j) + a, lambda a, b, r: r * (0.5 - 2j) + a,
          torch.complex64),
-        ("calcdets", lambda E, a, b, r: E.abs2(a) > E.wrap(r) * (E.pow(1e-5, -1.0 / E.wrap(r)) - 1.0),
-         lambda a, b, r: np.abs(a) ** 2 > r * (np.power(1e-5, -1.0 / r.astype(np.float64)) - 1.0), torch.int32),
+        # calcDets (simple_radar_pipeline.h:86-98) with norms n = 8 r + 20 and
+        # averages r / 20: thresholds in the range of |a|^2, so both outcomes
+        # occur; cells within 1e-5 of the threshold are not compared
+        ("calcdets", lambda E, a, b, r: E.abs2(a) > (E.wrap(r) * 8.0 + 20.0) *
+         (E.pow(1e-5, -1.0 / (E.wrap(r) * 8.0 + 20.0)) - 1.0) * (E.wrap(r) / 20.0),
+         lambda a, b, r: (np.abs(a) ** 2, (8 * r + 20) * (np.power(1e-5, -1.0 / (8 * r + 20)) - 1) * r / 20),
+         torch.int32),
     ]
 
 
@@ -123,7 +128,12 @@ def test_every_opcode_vs_fp64(mx):
     for name, _, ref, dt in cases():
         want = ref(a.astype(np.complex128) if np.iscomplexobj(a) else a, b.astype(np.complex128),
                    r.astype(np.float64))
-        if dt == torch.int32:
+        if isinstance(want, tuple):  # (value, threshold): value > threshold
+            v, t = want
+            near = np.abs(v - t) <= 1e-5 * np.abs(t)
+            assert ((got[name] != (v > t)) & ~near).sum() == 0, name
+            assert 0 < got[name].sum() < got[name].size, name
+        elif dt == torch.int32:
             assert np.array_equal(got[name], np.asarray(want).astype(np.int32)), name
         else:
             assert rel(got[name], want) < RTOL, name
