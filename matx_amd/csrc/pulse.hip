@@ -253,8 +253,11 @@ struct Pc4096Args {
 // elements.  Pre: the workgroup's previous use of lds is finished.
 // (The base rows are deliberately not __restrict__: re-fetching them for the
 // inverse passes is cheaper than keeping them live across the passes.)
+// spec(i) gives spectrum bin i (the block's 1/M included); thread j reads
+// bins j + 256 q only.
+template <class Spec>
 __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *base1,
-                                              const float4 *base2, float2 *lds) {
+                                              const float4 *base2, float2 *lds, Spec spec) {
   constexpr int NT = 256;
   using LY = Layout<4096, 1, false>;
   const int j = threadIdx.x;
@@ -302,7 +305,7 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
   tw_from_base(v, t0, t1, false);  // forward pass 2 (Ls = 256)
   dft16<false>(v);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], ob.wc[j + q * 256]);
+  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], spec(j + q * 256));
   dft16<true>(v);                   // inverse pass 0 (Ls = 1)
   t0 = b1p[0];
   t1 = b1p[1];
@@ -325,6 +328,12 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
   }
 }
 
+// The block's spectrum from its buffer (ob.wc).
+__device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *base1,
+                                              const float4 *base2, float2 *lds) {
+  ols_block4096(ob, base1, base2, lds, [&](int i) { return ob.wc[i]; });
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void pc4096_kernel(Pc4096Args pa) {
   using LY = Layout<4096, 1, false>;
@@ -337,6 +346,73 @@ void pc4096_kernel(Pc4096Args pa) {
   const OlsBlock ob{a.x + ox, a.y + oy, (int)((g % a.nblk) * a.V), (int)a.n, (int)a.seff,
                     (int)a.nout, (int)a.V, a.wc};
   ols_block4096(ob, pa.base1, pa.base2, lds);
+}
+
+// Small grids (fft_conv on a few hundred pulses, where one workgroup per
+// pulse leaves most CUs idle): every workgroup first transforms the
+// zero-padded waveform itself with the same passes, instead of waiting for a
+// one-workgroup spectrum launch.  The last forward pass leaves bins j + 256 q
+// in thread j, exactly the bins its multiply step reads, so the spectrum goes
+// to a slice of LDS that only the same thread reads back (no barrier).  No
+// window, normalisation or write-back (the fft_conv form).
+struct Pc4096WfArgs {
+  Pc4096Args p;
+  const float2 *w;
+  int64_t wstride;
+  int L, conj;
+  float out_scale;
+};
+
+__global__ __launch_bounds__(256) void pc4096_wf_kernel(Pc4096WfArgs q) {
+  using LY = Layout<4096, 1, false>;
+  __shared__ float2 lds[LY::LDS_ELEMS];
+  __shared__ float2 wsp[4096];
+  constexpr int NT = 256;
+  const int j = threadIdx.x;
+  {
+    float2 v[16];
+    auto xchg = [&](int ls_out) {
+      const int k = j & (ls_out - 1), base = (j - k) * 16 + k;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) lds[LY::addr(0, base + m * ls_out)] = v[m];
+      wg_barrier();
+#pragma unroll
+      for (int m = 0; m < 16; ++m) v[m] = lds[LY::addr(0, j + m * NT)];
+      wg_barrier();
+    };
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int i = j + m * NT;
+      v[m] = i < q.L ? q.w[(int64_t)i * q.wstride] : make_float2(0.f, 0.f);
+    }
+    const float4 *b1p = q.p.base1 + (j & 15) * 2;
+    const float4 *b2p = q.p.base2 + j * 2;
+    dft16<false>(v);
+    float4 t0 = b1p[0], t1 = b1p[1];
+    xchg(1);
+    tw_from_base(v, t0, t1, false);
+    dft16<false>(v);
+    t0 = b2p[0];
+    t1 = b2p[1];
+    xchg(16);
+    tw_from_base(v, t0, t1, false);
+    dft16<false>(v);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      float2 x = cscale(v[m], q.out_scale);
+      if (q.conj) x.y = -x.y;
+      wsp[j + m * NT] = x;
+    }
+  }
+  // the exchanges above end with a barrier: lds is free for the block
+  const PcArgs &a = q.p.a;
+  const int64_t g = xcd_remap(blockIdx.x, a.groups);
+  const int64_t row = g / a.nblk;
+  int64_t ox, oy;
+  batch_offsets(a.bi, row, &ox, &oy);
+  const OlsBlock ob{a.x + ox, a.y + oy, (int)((g % a.nblk) * a.V), (int)a.n, (int)a.seff,
+                    (int)a.nout, (int)a.V, nullptr};
+  ols_block4096(ob, q.p.base1, q.p.base2, lds, [&](int i) { return wsp[i]; });
 }
 
 
@@ -849,8 +925,25 @@ CacheClearRegistrar g_reg_wc(clear_wc);
 
 // Either part may be skipped: wa == nullptr keeps the spectrum already in
 // the buffer (a plan's), pa == nullptr only (re)computes the spectrum.
+// Grids up to this many workgroups take pc4096_wf_kernel when the spectrum
+// is computed in the same call (two resident workgroups per CU).
+constexpr int64_t PC_WF_FUSE_MAX_GROUPS = 512;
+
 template <int M>
 mxr_status launch_pc(const WfArgs *wa, const PcArgs *pa, hipStream_t s) {
+  if constexpr (M == 4096) {
+    if (wa && pa && pa->groups > 0 && pa->bpw == 1 && pa->groups <= PC_WF_FUSE_MAX_GROUPS &&
+        !wa->window && !wa->normalize && !wa->writeback && wa->L <= 4096) {
+      Pc4096WfArgs q{{*pa, pow_base_device(4096, 16), pow_base_device(4096, 256)}, wa->w, wa->wstride,
+                     (int)wa->L, wa->conj, wa->out_scale};
+      MXR_REQUIRE(q.p.base1 && q.p.base2, MXR_OUT_OF_MEMORY,
+                  "pulse compression: twiddle table allocation failed");
+      ProfScope ps("pc4096_wf_kernel", s);
+      pc4096_wf_kernel<<<dim3((unsigned)pa->groups), dim3(256), 0, s>>>(q);
+      MXR_HIP_TRY(hipGetLastError());
+      return MXR_SUCCESS;
+    }
+  }
   if (wa) {
     ProfScope ps("wfspec_kernel", s);
     wfspec_kernel<M><<<dim3(1), dim3(PcCfg<M>::NT), 0, s>>>(*wa);
