@@ -22,6 +22,7 @@
 #include <array>
 #include <complex>
 #include <cstdint>
+#include <cctype>
 #include <cstring>
 #include <initializer_list>
 #include <limits>
@@ -411,6 +412,137 @@ Expr hamming(const std::array<index_t, Rank> &shape) {
 // ones(shape) (include/matx/generators/ones.h)
 template <typename T = float, int Rank = 1>
 Expr ones(const std::array<index_t, Rank> &) { return detail::to_expr(1.0f); }
+
+// ------------------------------------------------------------ interop
+// DLPack import (make_tensor(DLManagedTensor*), include/matx/core/
+// make_tensor.h:1136-1190): the tensor takes ownership and runs the
+// producer's deleter when its last view goes.  mxr_dl_managed has the layout
+// of DLPack's DLManagedTensor.
+struct mxr_dl_managed {
+  mxr_dl_tensor dl_tensor;
+  void *manager_ctx;
+  void (*deleter)(mxr_dl_managed *self);
+};
+template <typename T, int RANK>
+tensor_t<T, RANK> make_tensor_from_dlpack(mxr_dl_managed *m) {
+  mxr_tensor d{};
+  detail::check(mxr_tensor_from_dlpack(&d, &m->dl_tensor));
+  if (d.rank != RANK || d.dtype != detail::dtype_of<T>::value)
+    throw matxException(MXR_INVALID_TYPE, "make_tensor_from_dlpack: rank or element type differs");
+  std::array<index_t, RANK> sh{}, st{};
+  for (int i = 0; i < RANK; ++i) {
+    sh[i] = d.shape[i];
+    st[i] = d.stride[i];
+  }
+  std::shared_ptr<void> own(m, [](void *p) {
+    auto *mm = static_cast<mxr_dl_managed *>(p);
+    if (mm->deleter) mm->deleter(mm);
+  });
+  return tensor_t<T, RANK>(own, static_cast<T *>(d.data), sh, st, MATX_DEVICE_MEMORY);
+}
+
+namespace io {
+namespace npy_detail {
+template <typename T>
+inline const char *descr() {
+  if constexpr (std::is_same_v<T, cfloat>) return "<c8";
+  else if constexpr (std::is_same_v<T, float>) return "<f4";
+  else if constexpr (std::is_same_v<T, int>) return "<i4";
+  else static_assert(sizeof(T) == 0, "npy: complex<float>, float or int elements");
+}
+template <typename T, int RANK>
+void require_contiguous(const tensor_t<T, RANK> &t, const char *fn) {
+  if (!t.IsContiguous()) throw matxException(MXR_INVALID_PARAMETER, std::string(fn) + ": view is not contiguous");
+}
+}  // namespace npy_detail
+
+// write_npy / read_npy (include/matx/file_io/file_io.h:321-362), with the
+// .npy format written and parsed here (version 1.0 written; 1.0, 2.0 and 3.0
+// read; little-endian complex<float>, float or int; C order).  read_npy fills
+// an existing contiguous view whose shape must match the file's.
+template <typename T, int RANK>
+void write_npy(const tensor_t<T, RANK> &t, const std::string &fname) {
+  npy_detail::require_contiguous(t, "write_npy");
+  std::string shape = "(";
+  for (int d = 0; d < RANK; ++d) shape += std::to_string(t.Size(d)) + (RANK == 1 ? "," : (d + 1 < RANK ? ", " : ""));
+  shape += ")";
+  std::string hdr = std::string("{'descr': '") + npy_detail::descr<T>() + "', 'fortran_order': False, 'shape': " +
+                    shape + ", }";
+  const size_t total = 10 + hdr.size() + 1;
+  hdr.append((64 - total % 64) % 64, ' ');
+  hdr.push_back('\n');
+  std::vector<T> h((size_t)t.TotalSize());
+  detail::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  if (!h.empty())
+    detail::hip_check(hipMemcpy(h.data(), t.Data(), sizeof(T) * h.size(), hipMemcpyDefault), "copy");
+  FILE *f = fopen(fname.c_str(), "wb");
+  if (!f) throw matxException(MXR_INVALID_PARAMETER, "write_npy: cannot open " + fname);
+  const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+  const uint16_t hl = (uint16_t)hdr.size();
+  const bool ok = fwrite(magic, 1, 8, f) == 8 && fwrite(&hl, 2, 1, f) == 1 &&
+                  fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size() &&
+                  (h.empty() || fwrite(h.data(), sizeof(T), h.size(), f) == h.size());
+  fclose(f);
+  if (!ok) throw matxException(MXR_INVALID_PARAMETER, "write_npy: short write to " + fname);
+}
+
+template <typename T, int RANK>
+void read_npy(tensor_t<T, RANK> &t, const std::string &fname) {
+  npy_detail::require_contiguous(t, "read_npy");
+  FILE *f = fopen(fname.c_str(), "rb");
+  if (!f) throw matxException(MXR_INVALID_PARAMETER, "read_npy: failed to read [" + fname + "], does not exist");
+  auto fail = [&](const std::string &why) {
+    fclose(f);
+    throw matxException(MXR_INVALID_PARAMETER, "read_npy: " + fname + ": " + why);
+  };
+  unsigned char magic[8];
+  if (fread(magic, 1, 8, f) != 8 || magic[0] != 0x93 || std::memcmp(magic + 1, "NUMPY", 5) != 0)
+    fail("not an .npy file");
+  uint32_t hl = 0;
+  if (magic[6] == 1) {
+    uint16_t h16 = 0;
+    if (fread(&h16, 2, 1, f) != 1) fail("truncated header");
+    hl = h16;
+  } else if (magic[6] == 2 || magic[6] == 3) {
+    if (fread(&hl, 4, 1, f) != 1) fail("truncated header");
+  } else {
+    fail("unknown format version");
+  }
+  std::string hdr(hl, '\0');
+  if (fread(&hdr[0], 1, hl, f) != hl) fail("truncated header");
+  auto field = [&](const char *key) -> std::string {
+    const size_t k = hdr.find(key);
+    if (k == std::string::npos) fail(std::string("no ") + key);
+    size_t v = hdr.find(':', k) + 1;
+    while (v < hdr.size() && hdr[v] == ' ') ++v;
+    return hdr.substr(v);
+  };
+  const std::string ds = field("'descr'");
+  const std::string want = std::string("'") + npy_detail::descr<T>() + "'";
+  if (ds.compare(0, want.size(), want) != 0) fail("element type is not " + want);
+  if (field("'fortran_order'").compare(0, 5, "False") != 0) fail("Fortran order is not supported");
+  const std::string sh = field("'shape'");
+  std::vector<index_t> dims;
+  for (size_t i = 1; i < sh.size() && sh[i] != ')';) {
+    if (std::isdigit((unsigned char)sh[i])) {
+      size_t e = i;
+      while (e < sh.size() && std::isdigit((unsigned char)sh[e])) ++e;
+      dims.push_back((index_t)std::stoll(sh.substr(i, e - i)));
+      i = e;
+    } else {
+      ++i;
+    }
+  }
+  if ((int)dims.size() != RANK) fail("rank differs from the tensor's");
+  for (int d = 0; d < RANK; ++d)
+    if (dims[d] != t.Size(d)) fail("shape differs from the tensor's");
+  std::vector<T> h((size_t)t.TotalSize());
+  if (!h.empty() && fread(h.data(), sizeof(T), h.size(), f) != h.size()) fail("truncated data");
+  fclose(f);
+  if (!h.empty())
+    detail::hip_check(hipMemcpy(t.Data(), h.data(), sizeof(T) * h.size(), hipMemcpyDefault), "copy");
+}
+}  // namespace io
 
 // --------------------------------------------------------- transform ops
 namespace detail {

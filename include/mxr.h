@@ -84,6 +84,27 @@ mxr_status mxr_profile_enable(int on);
 mxr_status mxr_profile_reset(void);
 mxr_status mxr_profile_query(const char *name, int64_t *count, double *total_ms);
 
+/* DLPack interop (MatX: tensor_t::ToDlPack / make_tensor(DLManagedTensor*),
+ * include/matx/core/tensor.h:1471-1555, make_tensor.h:1136-1190).
+ * mxr_dl_tensor has the memory layout of DLPack's DLTensor (data, device
+ * {type, id}, ndim, dtype {code, bits, lanes}, shape, strides, byte_offset),
+ * so a producer's DLTensor* (or the first member of its DLManagedTensor) can
+ * be passed as is.  mxr_tensor_from_dlpack fills a descriptor of the same
+ * memory, without copying: ROCm device memory (device type 10) only; complex64
+ * (code 5, 64 bits), float32 (2, 32) or int32 (0, 32), one lane; NULL strides
+ * mean compact row-major; rank 0..8.  Ownership stays with the producer. */
+typedef struct mxr_dl_tensor {
+  void *data;
+  int32_t device_type, device_id;
+  int32_t ndim;
+  uint8_t dtype_code, dtype_bits;
+  uint16_t dtype_lanes;
+  int64_t *shape;
+  int64_t *strides;
+  uint64_t byte_offset;
+} mxr_dl_tensor;
+mxr_status mxr_tensor_from_dlpack(mxr_tensor *out, const mxr_dl_tensor *src);
+
 /* ------------------------------------------------------------------ FFT */
 /* Batched 1-D complex-to-complex FFT along `axis`.
  * n = transform size; 0 means in->shape[axis].  The input is zero-padded or

@@ -22,7 +22,7 @@ PC_WINDOW, PC_NORMALIZE, PC_WRITEBACK = 1, 2, 4
 
 # Every entry point include/mxr.h declares.
 EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_enable",
-           "mxr_profile_reset", "mxr_profile_query", "mxr_fft_c2c", "mxr_fft_r2c", "mxr_fft_c2r",
+           "mxr_profile_reset", "mxr_profile_query", "mxr_tensor_from_dlpack", "mxr_fft_c2c", "mxr_fft_r2c", "mxr_fft_c2r",
            "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar", "mxr_doppler_cfar",
            "mxr_conv1d", "mxr_conv1d_fft", "mxr_conv2d", "mxr_ew_eval", "mxr_sum", "mxr_pc_plan_create",
            "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy",
@@ -49,6 +49,15 @@ class mxr_tensor(ctypes.Structure):
                 ("rank", ctypes.c_int32),
                 ("shape", ctypes.c_int64 * MXR_MAX_RANK),
                 ("stride", ctypes.c_int64 * MXR_MAX_RANK)]
+
+
+class mxr_dl_tensor(ctypes.Structure):
+    """DLPack's DLTensor layout (mxr.h mxr_dl_tensor)."""
+    _fields_ = [("data", ctypes.c_void_p), ("device_type", ctypes.c_int32),
+                ("device_id", ctypes.c_int32), ("ndim", ctypes.c_int32),
+                ("dtype_code", ctypes.c_uint8), ("dtype_bits", ctypes.c_uint8),
+                ("dtype_lanes", ctypes.c_uint16), ("shape", ctypes.POINTER(ctypes.c_int64)),
+                ("strides", ctypes.POINTER(ctypes.c_int64)), ("byte_offset", ctypes.c_uint64)]
 
 
 class mxr_cfar_params(ctypes.Structure):
@@ -97,6 +106,7 @@ def lib():
                                     ctypes.POINTER(ctypes.c_double)]
     L.mxr_profile_query.restype = I
     sigs = {
+        "mxr_tensor_from_dlpack": [T, V],
         "mxr_fft_c2c": [T, T, I64, I, I, I, V],
         "mxr_fft_r2c": [T, T, I64, I, I, V],
         "mxr_fft_c2r": [T, T, I64, I, I, V],
@@ -174,6 +184,20 @@ def profile_query(name: str):
     t = ctypes.c_double()
     check(lib().mxr_profile_query(name.encode(), ctypes.byref(n), ctypes.byref(t)))
     return n.value, t.value
+
+
+def desc_from_dlpack(obj):
+    """(mxr_tensor, capsule) for a DLPack producer's memory, no copy: obj has
+    __dlpack__ (a torch, CuPy ... array) or is a "dltensor" capsule.  The
+    descriptor is valid while the capsule (and its producer) are alive."""
+    cap = obj.__dlpack__() if hasattr(obj, "__dlpack__") else obj
+    get = ctypes.pythonapi.PyCapsule_GetPointer
+    get.restype = ctypes.c_void_p
+    get.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    ptr = get(cap, b"dltensor")
+    d = mxr_tensor()
+    check(lib().mxr_tensor_from_dlpack(ctypes.byref(d), ptr))
+    return d, cap
 
 
 def stream_handle(stream=None):

@@ -338,4 +338,45 @@ mxr_status mxr_clear_caches(void) {
   });
 }
 
+mxr_status mxr_tensor_from_dlpack(mxr_tensor *out, const mxr_dl_tensor *src) {
+  return mxr::guarded("mxr_tensor_from_dlpack", [&]() -> mxr_status {
+    MXR_REQUIRE(out != nullptr && src != nullptr, MXR_INVALID_PARAMETER,
+                "tensor_from_dlpack: null argument");
+    constexpr int32_t kDLROCM = 10;
+    MXR_REQUIRE(src->device_type == kDLROCM, MXR_NOT_SUPPORTED,
+                "tensor_from_dlpack: device type %d is not ROCm device memory (10)",
+                (int)src->device_type);
+    MXR_REQUIRE(src->ndim >= 0 && src->ndim <= MXR_MAX_RANK, MXR_INVALID_DIM,
+                "tensor_from_dlpack: rank %d not in [0, %d]", (int)src->ndim, MXR_MAX_RANK);
+    MXR_REQUIRE(src->dtype_lanes == 1, MXR_INVALID_TYPE, "tensor_from_dlpack: %d lanes",
+                (int)src->dtype_lanes);
+    int dt = 0;
+    if (src->dtype_code == 5 && src->dtype_bits == 64) dt = MXR_C64;
+    else if (src->dtype_code == 2 && src->dtype_bits == 32) dt = MXR_F32;
+    else if (src->dtype_code == 0 && src->dtype_bits == 32) dt = MXR_I32;
+    MXR_REQUIRE(dt != 0, MXR_INVALID_TYPE,
+                "tensor_from_dlpack: dtype (code %d, %d bits) is not complex64, float32 or int32",
+                (int)src->dtype_code, (int)src->dtype_bits);
+    MXR_REQUIRE(src->ndim == 0 || src->shape != nullptr, MXR_INVALID_PARAMETER,
+                "tensor_from_dlpack: null shape");
+    const size_t esz = dt == MXR_C64 ? 8 : 4;
+    MXR_REQUIRE(src->byte_offset % esz == 0, MXR_INVALID_PARAMETER,
+                "tensor_from_dlpack: byte offset %llu is not a multiple of the element size",
+                (unsigned long long)src->byte_offset);
+    mxr_tensor t{};
+    t.data = static_cast<char *>(src->data) + src->byte_offset;
+    t.dtype = dt;
+    t.rank = src->ndim;
+    int64_t compact = 1;
+    for (int d = src->ndim - 1; d >= 0; --d) {
+      MXR_REQUIRE(src->shape[d] >= 0, MXR_INVALID_SIZE, "tensor_from_dlpack: negative extent");
+      t.shape[d] = src->shape[d];
+      t.stride[d] = src->strides ? src->strides[d] : compact;
+      compact *= src->shape[d];
+    }
+    *out = t;
+    return MXR_SUCCESS;
+  });
+}
+
 }  // extern "C"

@@ -115,6 +115,21 @@ int main(int argc, char **argv) {
     dump(dir + "/op_irfft2.bin", ir2, ex);
     dump(dir + "/op_conv.bin", cv, ex);
     dump(dir + "/op_corr.bin", cr, ex);
+    // .npy round trips: write two results; read the file the Python side wrote
+    // and dump it back
+    io::write_npy(rf, dir + "/op_rfft.npy");
+    io::write_npy(ir, dir + "/op_irfft.npy");
+    auto npin = make_tensor<cfloat, 2>({3, 7});
+    io::read_npy(npin, dir + "/npy_in.npy");
+    dump(dir + "/npy_in.bin", npin, ex);
+    bool npy_threw = false;
+    try {
+      auto wrong = make_tensor<cfloat, 2>({7, 3});
+      io::read_npy(wrong, dir + "/npy_in.npy");
+    } catch (const matxException &) {
+      npy_threw = true;
+    }
+    printf(npy_threw ? "npy-shape-check ok\n" : "npy-shape-check MISSING\n");
   } catch (const matxException &e) {
     fprintf(stderr, "matx error %d: %s\n", e.code(), e.what());
     return 1;

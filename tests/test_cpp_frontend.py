@@ -24,11 +24,14 @@ def test_cpp_radar_stages(orc, matx_inputs, golden, tmp_path):
     x, w = matx_inputs
     C, P, S, L, R, Pr = 2, 128, 9000, 1000, 8001, 256
     x.astype(np.complex64).tofile(tmp_path / "x.bin")
+    npy_in = (np.arange(21) + 1j * np.arange(21)[::-1]).astype(np.complex64).reshape(3, 7)
+    np.save(tmp_path / "npy_in.npy", npy_in)
     w.astype(np.complex64).tofile(tmp_path / "w.bin")
     r = subprocess.run([os.path.join(BIN, "radar_stage_test"), str(tmp_path), str(C)],
                        capture_output=True, text=True, timeout=180)
     assert r.returncode == 0, r.stderr
     assert "size-check ok" in r.stdout and "stages ok" in r.stdout
+    assert "npy-shape-check ok" in r.stdout
     st = orc.radar_pipeline(x[None], w, mode="matx", stages=True)
 
     def rel(a, b):
@@ -65,3 +68,8 @@ def test_cpp_radar_stages(orc, matx_inputs, golden, tmp_path):
     assert rel(op("conv", np.complex64, (4, 108)), conv) < 1e-5
     corr = np.array([np.convolve(r, np.conj(fh[::-1]))[4:104] for r in xs])  # SAME start (9-1)/2
     assert rel(op("corr", np.complex64, (4, 100)), corr) < 1e-5
+    # io::write_npy / io::read_npy: files numpy reads back exactly, and a
+    # numpy-written file read by the C++ side
+    assert np.array_equal(np.load(tmp_path / "op_rfft.npy"), op("rfft", np.complex64, (4, 51)))
+    assert np.array_equal(np.load(tmp_path / "op_irfft.npy"), op("irfft", np.float32, (4, 100)))
+    assert np.array_equal(np.fromfile(tmp_path / "npy_in.bin", np.complex64).reshape(3, 7), npy_in)

@@ -174,6 +174,25 @@ def test_fft2_family(mx, orc):
         assert rel(host(mx.fft2(dev(x), norm=norm)), np.fft.fft2(x.astype(np.complex128), norm=norm)) < RTOL
 
 
+def test_dlpack_import_through_the_abi(mx):
+    """A torch tensor and a permuted view handed over as DLPack capsules
+    (mxr_tensor_from_dlpack, no copy) and transformed by mxr_fft_c2c."""
+    from matx_amd import _lib as L
+    rng = np.random.default_rng(41)
+    x = crandn(rng, 3, 64, 40)
+    xd = dev(x)
+    for view, axis in ((xd, 2), (xd.permute(0, 2, 1), 2)):
+        d, cap = L.desc_from_dlpack(view)
+        assert d.data == view.data_ptr()
+        assert list(d.shape[:3]) == list(view.shape) and list(d.stride[:3]) == list(view.stride())
+        out = torch.empty(view.shape, dtype=torch.complex64, device="cuda")
+        L.check(L.lib().mxr_fft_c2c(L.ref(L.desc(out)), L.ctypes.byref(d), 0, L.FORWARD,
+                                    L.NORM_BACKWARD, axis, L.stream_handle(None)))
+        ref = np.fft.fft(host(view).astype(np.complex128), axis=axis)
+        assert rel(host(out), ref) < RTOL
+        del cap
+
+
 # --------------------------------------------------------- fft_conv / PC
 def lfm(L, bw, rng=None):
     n = np.arange(L)

@@ -243,3 +243,41 @@ def test_loss_head_checks(L):
     r_, _ = status(L, lib.mxr_psl_adjoint, ctypes.byref(gp), ctypes.byref(g), ctypes.byref(pw),
                    ctypes.byref(tg), 2, 8, None)
     assert r_ == 2                                  # gpow must be contiguous
+
+
+def test_dlpack_descriptor(L):
+    """mxr_tensor_from_dlpack on hand-built DLTensor structs (DLPack's
+    layout): offsets, compact and explicit strides, dtype codes, and the
+    refusals (host memory, unsupported dtypes, lanes).  No device calls."""
+    lib = L.lib()
+    shape = (ctypes.c_int64 * 3)(2, 3, 5)
+
+    def dl(device=10, code=5, bits=64, lanes=1, strides=None, offset=0):
+        t = L.mxr_dl_tensor()
+        t.data = 0x100000
+        t.device_type, t.device_id, t.ndim = device, 0, 3
+        t.dtype_code, t.dtype_bits, t.dtype_lanes = code, bits, lanes
+        t.shape = ctypes.cast(shape, ctypes.POINTER(ctypes.c_int64))
+        if strides is not None:
+            arr = (ctypes.c_int64 * 3)(*strides)
+            t._keep = arr
+            t.strides = ctypes.cast(arr, ctypes.POINTER(ctypes.c_int64))
+        t.byte_offset = offset
+        return t
+
+    d = L.mxr_tensor()
+    assert lib.mxr_tensor_from_dlpack(ctypes.byref(d), ctypes.byref(dl(offset=16))) == 0
+    assert d.data == 0x100000 + 16 and d.dtype == L.C64 and d.rank == 3
+    assert list(d.shape[:3]) == [2, 3, 5] and list(d.stride[:3]) == [15, 5, 1]
+    assert lib.mxr_tensor_from_dlpack(ctypes.byref(d), ctypes.byref(dl(code=2, bits=32, strides=(1, 2, 6)))) == 0
+    assert d.dtype == L.F32 and list(d.stride[:3]) == [1, 2, 6]
+    assert lib.mxr_tensor_from_dlpack(ctypes.byref(d), ctypes.byref(dl(code=0, bits=32))) == 0
+    assert d.dtype == L.I32
+    r, msg = status(L, lib.mxr_tensor_from_dlpack, ctypes.byref(d), ctypes.byref(dl(device=1)))
+    assert r == 1 and "ROCm" in msg
+    r, _ = status(L, lib.mxr_tensor_from_dlpack, ctypes.byref(d), ctypes.byref(dl(code=2, bits=64)))
+    assert r == 6
+    r, _ = status(L, lib.mxr_tensor_from_dlpack, ctypes.byref(d), ctypes.byref(dl(lanes=2)))
+    assert r == 6
+    r, _ = status(L, lib.mxr_tensor_from_dlpack, ctypes.byref(d), ctypes.byref(dl(offset=4)))
+    assert r == 2
