@@ -850,7 +850,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by channel with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(the path shards by channel with no exchange between GPUs)", file=sys.stderr)
         sys.exit(2)
     import torch
     if args.calibrate_copy:

@@ -136,7 +136,7 @@ mxr_status mxr_fft_c2r(const mxr_tensor *out, const mxr_tensor *in, int64_t n,
                        int norm, int axis, hipStream_t stream);
 
 /* fft_conv: out = ifft(fft(x, n) .* conj(fft(w, n)))  (conj_w = 1, circular
- * cross-correlation, the BASELINE operator) or ifft(fft(x,n) .* fft(w,n))
+ * cross-correlation, the benchmarked operator) or ifft(fft(x,n) .* fft(w,n))
  * (conj_w = 0, circular convolution), along the last axis of x.
  * x: [..., S] c64; w: [L] c64 (one waveform for every row) or [..., L] with
  * x's batch dims (one waveform per row, n in [1024, 8192]); out: [..., nout]

@@ -1,5 +1,5 @@
 // Row-wise correlation with one waveform per row, and the adjoints that a
-// gradient through fft_conv needs (the BASELINE's differentiable form of
+// gradient through fft_conv needs (the differentiable form of
 // y = ifft(fft(x, n) .* conj(fft(w, n))), examples/fft_conv.cu:90-108).
 //
 // Forward (conj_w = 1: correlation, 0: convolution), per row b:

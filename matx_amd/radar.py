@@ -154,7 +154,7 @@ class FusedRadarPipeline:
     reads it once and writes the detection mask.  The power map and the
     background averages are written only when asked (want_pow / want_ba).
 
-    tpc="valid", nd=128 is the BASELINE workload (126 canceller rows);
+    tpc="valid", nd=128 is the 16 x 128 x 9000 benchmark pipeline (126 canceller rows);
     tpc="same", nd=next_pow2(P, strict) is MatX's stage dataflow.
 
     overlap=True streams consecutive cubes: the compressed cube is double
