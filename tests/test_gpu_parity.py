@@ -202,7 +202,7 @@ def lfm(L, bw, rng=None):
     return (np.exp(1j * ph) * np.hamming(L)).astype(np.complex64)
 
 
-def test_fft_conv_config1(mx, orc):
+def test_fft_conv_4x32x2048_chirp256(mx, orc):
     """4 x 32 x 2048 cube, 256-sample LFM (0.5 fs), N = 4096, all lags."""
     rng = np.random.default_rng(3)
     x = crandn(rng, 4, 32, 2048)
