@@ -8,7 +8,9 @@
  *     twiddle tables and waveform spectra, released by mxr_clear_caches().
  *   - Every call is asynchronous on the caller's hipStream_t; nothing
  *     synchronises the device.  Plans are created (and their device buffers
- *     allocated) on first use, so warm a shape once before hipGraph capture.
+ *     allocated) on first use, so warm a shape once before hipGraph capture;
+ *     a call that would have to grow a per-stream workspace while its stream
+ *     is being captured fails with MXR_NOT_SUPPORTED instead of allocating.
  *   - Every entry returns an mxr_status (0 = success).  On failure the
  *     thread-local text from mxr_last_error() says what was wrong.  Shape and
  *     size errors are detected on the host before any launch, as MatX throws
@@ -70,8 +72,14 @@ typedef enum mxr_fft_dir { MXR_FFT_FORWARD = 0, MXR_FFT_INVERSE = 1 } mxr_fft_di
 /* ------------------------------------------------------------------ misc */
 const char *mxr_last_error(void);
 int mxr_version(void);
-/* Frees every cached plan, twiddle table and waveform spectrum
- * (cf. ClearCachesAndAllocations, test/main.cu). */
+/* Frees every cached plan, twiddle table, per-stream waveform spectrum and
+ * per-stream workspace (cf. ClearCachesAndAllocations, test/main.cu).  It
+ * synchronises the device first.  Caller-created plans (mxr_pc_plan_*,
+ * mxr_ragged_plan_*) stay valid: while any of them is alive the twiddle
+ * tables they point at are kept.  A hipGraph captured before the call may
+ * reference per-stream workspaces and spectra that it frees: re-capture
+ * after clearing.  Workspaces that had to grow keep their old buffers until
+ * this call, so earlier captures stay valid until then. */
 mxr_status mxr_clear_caches(void);
 
 /* Kernel timing, the analogue of the executor timers start_timer /
@@ -394,6 +402,10 @@ typedef enum mxr_ew_opcode {
   MXR_EW_LOG = 38, MXR_EW_COS = 39, MXR_EW_SIN = 40,
   MXR_EW_SELECT = 50   /* pop c, y, x: push (re c != 0) ? x : y             */
 } mxr_ew_opcode;
+/* POW, GT, LT, GE, LE, MAX, MIN, SQRT, COS, SIN and SELECT's condition take
+ * real operands only: a program that feeds them a complex value (a complex
+ * input or constant, or arithmetic on one) is rejected with
+ * MXR_INVALID_TYPE before any launch. */
 typedef struct mxr_ew_instr {
   int32_t op, a, b;
 } mxr_ew_instr;

@@ -499,7 +499,7 @@ static mxr_status launch_doppler_adj(DopGradArgs a, const mxr_tensor *gtaps, hip
   if (gtaps) {
     a.partial = (float *)stream_workspace("doppler_taps_partial",
                                           sizeof(float) * (size_t)a.C * a.tiles * a.P, s);
-    MXR_REQUIRE(a.partial != nullptr, MXR_OUT_OF_MEMORY, "doppler adjoint: workspace allocation failed");
+    if (a.partial == nullptr) return workspace_failure("doppler adjoint");
   }
   {
     ProfScope ps("doppler_adj_kernel", s);

@@ -200,6 +200,63 @@ static mxr_status ew_validate_program(const mxr_ew_instr *prog, int nprog, int n
   return MXR_SUCCESS;
 }
 
+// Opcodes whose operands must be real: they read only the real part
+// (ordering, pow, max/min, sqrt, cos, sin, the selector of SELECT).  A value
+// is complex when it comes from a complex input or constant, or from an
+// arithmetic / conj / neg / exp / log of a complex value.  Rejecting complex
+// operands here keeps e.g. sqrt(complex tensor) from silently returning the
+// real-part result.
+static mxr_status ew_check_types(const mxr_ew_instr *prog, int nprog, const int *idt,
+                                 const float *consts) {
+  bool cplx[EW_STACK + 3];
+  int sp = 0;
+  auto need_real = [&](bool c, int pc) -> mxr_status {
+    MXR_REQUIRE(!c, MXR_INVALID_TYPE,
+                "ew: opcode %d at instruction %d takes a real operand, got a complex one",
+                prog[pc].op, pc);
+    return MXR_SUCCESS;
+  };
+  mxr_status st;
+  for (int pc = 0; pc < nprog; ++pc) {
+    const mxr_ew_instr &i = prog[pc];
+    switch (i.op) {
+      case MXR_EW_LOAD: cplx[sp++] = idt[i.a] == MXR_C64; break;
+      case MXR_EW_CONST: cplx[sp++] = consts[2 * i.a + 1] != 0.f; break;
+      case MXR_EW_HAMMING:
+      case MXR_EW_INDEX: cplx[sp++] = false; break;
+      case MXR_EW_SELECT: {
+        const bool c = cplx[--sp], y = cplx[--sp], x = cplx[--sp];
+        if ((st = need_real(c, pc))) return st;
+        cplx[sp++] = x || y;
+        break;
+      }
+      default:
+        if (i.op >= MXR_EW_ADD && i.op <= MXR_EW_MIN) {
+          const bool y = cplx[--sp], x = cplx[--sp];
+          const bool arith = i.op <= MXR_EW_DIV, eq = i.op == MXR_EW_EQ;
+          if (!arith && !eq && (st = need_real(x || y, pc))) return st;
+          cplx[sp++] = arith && (x || y);
+        } else {
+          const bool x = cplx[--sp];
+          switch (i.op) {
+            case MXR_EW_SQRT:
+            case MXR_EW_COS:
+            case MXR_EW_SIN:
+              if ((st = need_real(x, pc))) return st;
+              cplx[sp++] = false;
+              break;
+            case MXR_EW_ABS2:
+            case MXR_EW_ABS:
+            case MXR_EW_REAL:
+            case MXR_EW_IMAG: cplx[sp++] = false; break;
+            default: cplx[sp++] = x; break;  // CONJ, NEG, EXP, LOG
+          }
+        }
+    }
+  }
+  return MXR_SUCCESS;
+}
+
 // ------------------------------------------------------ expression compiler
 // The default path generates one HIP kernel per expression shape (program,
 // operand dtypes, rank, which constants are real) as straight-line source,
@@ -505,6 +562,11 @@ mxr_status ew_impl(const mxr_tensor *out, const mxr_tensor *const *ins, int nin,
     a.in[k].p = ins[k]->data;
     a.in[k].dtype = ins[k]->dtype;
     for (int d = 0; d < out->rank; ++d) a.in[k].st[d] = ins[k]->stride[d];
+  }
+  {
+    int dts[EW_MAXIN] = {0};
+    for (int k = 0; k < nin; ++k) dts[k] = ins[k]->dtype;
+    if ((st = ew_check_types(prog, nprog, dts, consts))) return st;
   }
   a.nprog = nprog;
   for (int i = 0; i < nprog; ++i) a.prog[i] = prog[i];

@@ -193,8 +193,9 @@ static mxr_status fconv_fused(const mxr_tensor *out, const mxr_tensor *in, int64
   a.o0 = o0;
   a.olen = olen;
   a.spec = static_cast<float2 *>(stream_workspace("conv1d_fft_spec", sizeof(float2) * M, s));
+  if (a.spec == nullptr) return workspace_failure("conv1d_fft");
   a.tw = twiddles_device((int)M, false);
-  MXR_REQUIRE(a.spec && a.tw, MXR_OUT_OF_MEMORY, "conv1d_fft: workspace allocation failed");
+  MXR_REQUIRE(a.tw, MXR_OUT_OF_MEMORY, "conv1d_fft: twiddle table allocation failed");
   switch (M) {
     case 8: return launch_fconv<8>(a, s);
     case 16: return launch_fconv<16>(a, s);
@@ -272,7 +273,7 @@ mxr_status conv1d_fft_impl(const mxr_tensor *out, const mxr_tensor *in, const mx
   if (corr) return pc_common(fn, &ot, in, filt, n, 0, 0, 0, 1, s);
   // the correlation filter g = reverse(conj(h))
   float2 *g = static_cast<float2 *>(stream_workspace("conv1d_fft_rev", sizeof(float2) * L, s));
-  MXR_REQUIRE(g != nullptr, MXR_OUT_OF_MEMORY, "%s: workspace allocation failed", fn);
+  if (g == nullptr) return workspace_failure(fn);
   rev_conj_kernel<<<dim3((unsigned)((L + 255) / 256)), dim3(256), 0, s>>>(
       g, static_cast<const float2 *>(filt->data), L, filt->stride[0]);
   MXR_HIP_TRY(hipGetLastError());

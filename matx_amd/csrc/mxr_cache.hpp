@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "mxr.h"
+
 namespace mxr {
 void register_cache_clear(void (*fn)());
 
@@ -13,8 +15,20 @@ struct CacheClearRegistrar {
 };
 
 // Device scratch of at least `bytes` for (current device, stream, tag);
-// owned by the library, released by mxr_clear_caches.  nullptr on failure.
+// owned by the library, released by mxr_clear_caches.  A buffer that has to
+// grow is replaced, and the old one is kept until mxr_clear_caches (a graph
+// captured earlier may still use it).  Growing is refused while `s` is being
+// captured (nullptr, with the reason in mxr_last_error): run the call once
+// before capturing.  nullptr on failure.
 void *stream_workspace(const char *tag, size_t bytes, hipStream_t s);
+// Status for a null stream_workspace result: the capture refusal, or
+// MXR_OUT_OF_MEMORY with a message naming fn.
+mxr_status workspace_failure(const char *fn);
+
+// Live-plan count: twiddle tables stay allocated through mxr_clear_caches
+// while it is non-zero (plans hold raw pointers to them).
+void plan_tables_retain();
+void plan_tables_release();
 
 // Brackets one kernel launch with profiling events when profiling is on.
 class ProfScope {

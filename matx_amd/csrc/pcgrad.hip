@@ -314,7 +314,7 @@ static mxr_status wgrad_run(const WgradJob &j, hipStream_t s) {
       cus = 256;
     const int64_t grid = items < (int64_t)per_cu * cus ? items : (int64_t)per_cu * cus;
     float2 *ws = (float2 *)stream_workspace("pc_wgrad", sizeof(float2) * (size_t)(grid + 1) * M, s);
-    MXR_REQUIRE(ws != nullptr, MXR_OUT_OF_MEMORY, "pc waveform gradient: workspace allocation failed");
+    if (ws == nullptr) return workspace_failure("pc waveform gradient");
     spec = ws + grid * M;
     WgArgs a{};
     a.x = j.x;

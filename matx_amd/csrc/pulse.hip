@@ -1002,6 +1002,18 @@ mxr_status dispatch_pc(int M, const WfArgs *wa, const PcArgs *pa, hipStream_t s)
 
 }  // namespace
 
+// Builds every table the block kernels of size M read (forward and reversed
+// pass twiddles, the power-base rows of the dedicated 4096 and 8192 kernels),
+// so that later launches allocate nothing (and can be graph-captured).
+mxr_status pc_tables_prepare(int M, const char *fn) {
+  bool ok = twiddles_device(M, false) && twiddles_device(M, true);
+  if (M == 4096) ok = ok && pow_base_device(4096, 16) && pow_base_device(4096, 256);
+  if (M == 8192)
+    ok = ok && pow_base_device(8192, 2) && pow_base_device(8192, 32) && pow_base_device(8192, 512);
+  MXR_REQUIRE(ok, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
+  return MXR_SUCCESS;
+}
+
 // Overlap-save block plan of an out-of-place correlation (the planner's
 // choice, or the given block size M_req when it fits).
 mxr_status pc_plan_blocks(int64_t seff, int64_t L, int64_t n, int64_t nout, int M_req, int *M,
@@ -1144,17 +1156,12 @@ static mxr_status pc_plan_create(mxr_pc_plan *out, int64_t S, int64_t L, int64_t
   mxr_status st;
   if ((st = plan_pc(S < n ? S : n, L, n, nout, true, false, &plan))) return st;
   MXR_REQUIRE(L <= plan.M, MXR_NOT_SUPPORTED, "pc_plan_create: waveform longer than the block");
+  if ((st = pc_tables_prepare(plan.M, "pc_plan_create"))) return st;
   const float2 *twf = twiddles_device(plan.M, false);
   const float2 *twi = twiddles_device(plan.M, true);
-  MXR_REQUIRE(twf && twi, MXR_OUT_OF_MEMORY, "pc_plan_create: twiddle table allocation failed");
-  if (plan.M == 4096)
-    MXR_REQUIRE(pow_base_device(4096, 16) && pow_base_device(4096, 256), MXR_OUT_OF_MEMORY,
-                "pc_plan_create: twiddle table allocation failed");
-  if (plan.M == 8192)
-    MXR_REQUIRE(pow_base_device(8192, 2) && pow_base_device(8192, 32) && pow_base_device(8192, 512),
-                MXR_OUT_OF_MEMORY, "pc_plan_create: twiddle table allocation failed");
   float2 *wc = nullptr;
   MXR_HIP_TRY(hipMalloc(&wc, sizeof(float2) * plan.M));
+  plan_tables_retain();
   *out = reinterpret_cast<mxr_pc_plan>(new PcPlanObj{device_id(), S, L, n, nout, flags, plan, wc, twf, twi});
   return MXR_SUCCESS;
 }
@@ -1264,6 +1271,7 @@ static mxr_status pc_plan_destroy(mxr_pc_plan h) {
   const hipError_t e = hipFree(p->wc);
   (void)hipSetDevice(cur);
   delete p;
+  plan_tables_release();
   MXR_HIP_TRY(e);
   return MXR_SUCCESS;
 }

@@ -174,6 +174,7 @@ void free_plan(RaggedPlanObj *p) {
   (void)hipFree(p->tables);
   (void)hipSetDevice(cur);
   delete p;
+  plan_tables_release();
 }
 
 }  // namespace
@@ -199,6 +200,7 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
                 MXR_INVALID_PARAMETER, "%s: job %d strides overlap rows", fn, i);
   }
   auto *p = new RaggedPlanObj();
+  plan_tables_retain();  // released by free_plan
   p->dev = device_id();
   p->jobs.assign(jobs, jobs + njobs);
   p->M.resize(njobs);
@@ -247,12 +249,8 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
     PcClass cl;
     cl.M = M;
     const size_t first = hpc.size();
+    if ((st = pc_tables_prepare(M, fn))) break;
     const float2 *tw = twiddles_device(M, false);
-    if (!tw) {
-      set_error(MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
-      st = MXR_OUT_OF_MEMORY;
-      break;
-    }
     for (int i = 0; i < njobs; ++i) {
       if (p->M[i] != M) continue;
       const mxr_ragged_job &j = jobs[i];

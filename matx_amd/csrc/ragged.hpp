@@ -79,6 +79,8 @@ __device__ __forceinline__ T *opaque(T *p) {
   return p;
 }
 
+// Builds the tables block kernels of size M read (no allocation at launch).
+mxr_status pc_tables_prepare(int M, const char *fn);
 mxr_status pc_plan_blocks(int64_t seff, int64_t L, int64_t n, int64_t nout, int M_req, int *M,
                           int64_t *V, int64_t *nblk);
 mxr_status launch_wfspec_batch(int M, const WfArgs *d_args, int njobs, hipStream_t s);

@@ -231,14 +231,38 @@ struct WsBuf {
   size_t bytes = 0;
 };
 std::map<std::tuple<int, uintptr_t, std::string>, WsBuf> g_ws;
+
+// Superseded buffers (device, pointer) stay allocated until mxr_clear_caches:
+// a hipGraph captured earlier may still write them on replay.
+std::vector<std::pair<int, void *>> g_ws_retired;
+thread_local bool t_ws_capture_refused = false;
 }  // namespace
 
+mxr_status workspace_failure(const char *fn) {
+  if (t_ws_capture_refused) {
+    t_ws_capture_refused = false;
+    return MXR_NOT_SUPPORTED;  // message already set by stream_workspace
+  }
+  set_error(MXR_OUT_OF_MEMORY, "%s: workspace allocation failed", fn);
+  return MXR_OUT_OF_MEMORY;
+}
+
 void *stream_workspace(const char *tag, size_t bytes, hipStream_t s) {
-  const auto key = std::make_tuple(device_id(), (uintptr_t)s, std::string(tag));
+  const int dev = device_id();
+  const auto key = std::make_tuple(dev, (uintptr_t)s, std::string(tag));
   std::lock_guard<std::mutex> g(g_ws_mu);
   WsBuf &b = g_ws[key];
+  t_ws_capture_refused = false;
   if (b.bytes >= bytes && b.p) return b.p;
-  if (b.p) (void)hipFree(b.p);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+    set_error(MXR_NOT_SUPPORTED,
+              "workspace '%s' must grow to %zu bytes while the stream is being captured: run "
+              "the call once outside the capture first", tag, bytes);
+    t_ws_capture_refused = true;
+    return nullptr;
+  }
+  if (b.p) g_ws_retired.emplace_back(dev, b.p);
   b.p = nullptr;
   b.bytes = 0;
   if (hipMalloc(&b.p, bytes) != hipSuccess) {
@@ -251,19 +275,40 @@ void *stream_workspace(const char *tag, size_t bytes, hipStream_t s) {
 
 static void clear_workspaces() {
   std::lock_guard<std::mutex> g(g_ws_mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
   for (auto &kv : g_ws) {
-    int cur = 0;
-    (void)hipGetDevice(&cur);
     (void)hipSetDevice(std::get<0>(kv.first));
     (void)hipFree(kv.second.p);
-    (void)hipSetDevice(cur);
   }
+  for (auto &r : g_ws_retired) {
+    (void)hipSetDevice(r.first);
+    (void)hipFree(r.second);
+  }
+  (void)hipSetDevice(cur);
   g_ws.clear();
+  g_ws_retired.clear();
 }
 CacheClearRegistrar g_reg_ws(clear_workspaces);
 
+// Plans (pulse-compression and ragged) keep raw pointers to the twiddle and
+// power-base tables; while any plan is alive mxr_clear_caches leaves the
+// tables in place (a few hundred KiB), so a live plan never reads freed memory.
+namespace {
+int g_live_plans = 0;  // guarded by g_tw_mu
+}
+void plan_tables_retain() {
+  std::lock_guard<std::mutex> g(g_tw_mu);
+  ++g_live_plans;
+}
+void plan_tables_release() {
+  std::lock_guard<std::mutex> g(g_tw_mu);
+  if (g_live_plans > 0) --g_live_plans;
+}
+
 static void clear_twiddles() {
   std::lock_guard<std::mutex> g(g_tw_mu);
+  if (g_live_plans > 0) return;
   for (auto &kv : g_tw) {
     int cur = 0;
     (void)hipGetDevice(&cur);

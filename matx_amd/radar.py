@@ -156,18 +156,12 @@ class FusedRadarPipeline:
 
     tpc="valid", nd=128 is the 16 x 128 x 9000 benchmark pipeline (126 canceller rows);
     tpc="same", nd=next_pow2(P, strict) is MatX's stage dataflow.
-
-    overlap=True streams consecutive cubes: the compressed cube is double
-    buffered, pulse compression runs on one stream and the pulse-axis pass
-    on another, so cube k+1's pulse compression fills the tail of cube k's
-    pulse-axis pass (events keep each buffer's reader before its next
-    writer).  Results are identical; sync() joins the caller's stream.
     """
 
     def __init__(self, C: int, P: int, S: int, L: int, nd: int | None = None,
                  tpc: str = "valid", n: int | None = None, win=CFAR_WIN, guard=CFAR_GUARD,
                  pfa: float = PFA, want_pow: bool = False, want_ba: bool = False,
-                 device="cuda", stream=None, overlap: bool = False):
+                 device="cuda", stream=None):
         self.C, self.P, self.S, self.L = C, P, S, L
         self.R = S - L + 1
         self.n = next_pow2(S) if n is None else n
@@ -184,14 +178,6 @@ class FusedRadarPipeline:
         R = self.R
         self.plan = ops.PulseCompressionPlan(S, L, self.n, R)
         self.y = torch.empty((C, P, self.ld), dtype=torch.complex64, **kw)[:, :, :R]
-        self.overlap = overlap
-        if overlap:
-            self._ys = [self.y, torch.empty((C, P, self.ld), dtype=torch.complex64, **kw)[:, :, :R]]
-            self._s_pc = torch.cuda.Stream(device=device)
-            self._s_dc = torch.cuda.Stream(device=device)
-            self._pc_done = [torch.cuda.Event(), torch.cuda.Event()]
-            self._dc_done = [torch.cuda.Event(), torch.cuda.Event()]
-            self._k = 0
         self.dets = torch.empty((C, self.nd, self.ld), dtype=torch.int32, **kw)[:, :, :R]
         self.pw = self.ba = None
         self.set_outputs(want_pow, want_ba)
@@ -212,35 +198,15 @@ class FusedRadarPipeline:
         s = self.stream
         if w is not None:
             self.set_waveform(w)
-        if self.overlap:
-            return self._step_overlapped(x)
         self.plan(x, out=self.y, stream=s)
         ops.doppler_cfar(self.y, self.nd, tpc=self.tpc, win_rows=self.win_rows, win=self.win,
                          guard=self.guard, pfa=self.pfa, dets=self.dets, ba=self.ba, pw=self.pw,
                          stream=s)
         return self.dets
 
-    def _step_overlapped(self, x):
-        b = self._k & 1
-        self._k += 1
-        caller = self.stream or torch.cuda.current_stream()
-        self._s_pc.wait_stream(caller)                 # x and the spectrum are ready
-        self._s_pc.wait_event(self._dc_done[b])        # buffer b's previous reader is done
-        y = self._ys[b]
-        self.plan(x, out=y, stream=self._s_pc)
-        self._pc_done[b].record(self._s_pc)
-        self._s_dc.wait_event(self._pc_done[b])
-        ops.doppler_cfar(y, self.nd, tpc=self.tpc, win_rows=self.win_rows, win=self.win,
-                         guard=self.guard, pfa=self.pfa, dets=self.dets, ba=self.ba, pw=self.pw,
-                         stream=self._s_dc)
-        self._dc_done[b].record(self._s_dc)
-        self.y = y
-        return self.dets
-
     def sync(self):
-        """Makes the caller's stream wait for every cube issued so far."""
-        if self.overlap:
-            (self.stream or torch.cuda.current_stream()).wait_stream(self._s_dc)
+        """Waits for every cube issued so far on the pipeline's stream."""
+        (self.stream or torch.cuda.current_stream()).synchronize()
 
     def algorithmic_bytes(self) -> int:
         """Stage-boundary byte model per cube: read x, write+read the

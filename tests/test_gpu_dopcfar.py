@@ -183,25 +183,3 @@ def test_pulse_compression_plan(mx, orc):
     with pytest.raises(mx.MatxError):
         plan(dev(np.zeros((2, S - 1), np.complex64)))   # wrong row length
     plan.close()
-
-
-def test_overlapped_pipeline_matches_single_stream(mx):
-    """Consecutive cubes on two streams with a double-buffered compressed
-    cube: three cubes issued back to back (buffer 0 reused by the third)
-    give the single-stream result of the last one."""
-    rng = np.random.default_rng(15)
-    C, P, S, L, nd = 2, 64, 3000, 400, 64
-    cubes = [dev((rng.standard_normal((C, P, S)) + 1j * rng.standard_normal((C, P, S)))
-                 .astype(np.complex64)) for _ in range(3)]
-    w = dev(np.exp(1j * np.pi * 0.4 * (np.arange(L) - L / 2) ** 2 / L).astype(np.complex64))
-    ref = mx.FusedRadarPipeline(C, P, S, L, nd=nd, tpc="valid", want_pow=True)
-    ref.set_waveform(w)
-    ref.step(cubes[2])
-    want_d, want_p = host(ref.dets).copy(), host(ref.pw).copy()
-    ov = mx.FusedRadarPipeline(C, P, S, L, nd=nd, tpc="valid", want_pow=True, overlap=True)
-    ov.set_waveform(w)
-    for xc in cubes:
-        ov.step(xc)
-    ov.sync()
-    assert np.array_equal(host(ov.dets), want_d)
-    assert np.array_equal(host(ov.pw), want_p)

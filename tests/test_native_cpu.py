@@ -281,3 +281,32 @@ def test_dlpack_descriptor(L):
     assert r == 6
     r, _ = status(L, lib.mxr_tensor_from_dlpack, ctypes.byref(d), ctypes.byref(dl(offset=4)))
     assert r == 2
+
+
+def test_expression_rejects_complex_operands_of_real_ops(L):
+    """sqrt / cos / pow / ordering of a complex value read only the real part
+    in the kernels, so the host rejects such programs (MXR_INVALID_TYPE)
+    instead of returning a silently wrong result; the same ops on a real
+    input pass validation (they are not launched here: the output pointer is
+    fake, so only the rejecting calls are made)."""
+    lib = L.lib()
+    out = fake(L, [4, 8], L.C64)
+    xc = fake(L, [4, 8], L.C64, addr=0x90000)
+    ins = (ctypes.POINTER(L.mxr_tensor) * 1)(ctypes.pointer(xc))
+    I = ctypes.c_int32 * 3
+    LOAD, CONST, SQRT, COS, POW, GT, ABS2, SELECT = 1, 2, 33, 39, 14, 15, 31, 50
+    def prog(*ops):
+        arr = (I * len(ops))(*[I(*o) for o in ops])
+        return arr, len(ops)
+    consts = (ctypes.c_float * 4)(2.0, 0.0, 1.0, 0.5)   # const 0 real, const 1 complex
+    cases = [
+        [(LOAD, 0, 0), (SQRT, 0, 0)],
+        [(LOAD, 0, 0), (COS, 0, 0)],
+        [(LOAD, 0, 0), (CONST, 0, 0), (POW, 0, 0)],
+        [(CONST, 1, 0), (CONST, 0, 0), (GT, 0, 0)],
+        [(LOAD, 0, 0), (LOAD, 0, 0), (LOAD, 0, 0), (SELECT, 0, 0)],
+    ]
+    for ops in cases:
+        p, n = prog(*ops)
+        r, msg = status(L, lib.mxr_ew_eval, ctypes.byref(out), ins, 1, p, n, consts, 2, None)
+        assert r == 6 and "real operand" in msg, (ops, r, msg)
