@@ -1,12 +1,17 @@
 // matx_amd.h -- C++ front end with MatX's operator API over libmxr.so.
 //
-// Host-only, header-only C++17: a MatX-style program (tensor_t, make_tensor,
+// Header-only C++17/20: a MatX-style program (tensor_t, make_tensor,
 // slice / Permute / Clone views, (lhs = op).run(exec), fft / ifft / fft_conv /
 // conv1d / conv2d / sum, elementwise operators and generators) compiles with
-// g++ against this header and links libmxr.so; every operation reaches the
-// gfx950 kernels through the C ABI in mxr.h.  The executor is HIPExecutor (a
-// hipStream_t), the analogue of cudaExecutor
+// g++ (host only) or hipcc against this header and links libmxr.so; every
+// built-in operation reaches the gfx950 kernels through the C ABI in mxr.h.
+// The executor is HIPExecutor (a hipStream_t), the analogue of cudaExecutor
 // (include/matx/executors/cuda.h:57-231 in MatX).
+//
+// User-defined operators derived from BaseOp<Op> (MatX's custom-operator
+// form, e.g. calcDets in examples/simple_radar_pipeline.h:50-150) carry their
+// own __device__ operator(); their run() launches a kernel instantiated in
+// the user's translation unit, so that unit is compiled with hipcc.
 //
 // Differences from MatX, by design:
 //  * elementwise expressions are lowered at run() time to one postfix program
@@ -17,7 +22,15 @@
 //  * errors throw matxException carrying the mxr_status and message.
 #pragma once
 
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MATX_AMD_HD __host__ __device__
+#else
 #include <hip/hip_runtime_api.h>
+#define MATX_AMD_HD
+#endif
+
+#include <cmath>
 
 #include <array>
 #include <complex>
@@ -34,6 +47,38 @@
 #include <vector>
 
 #include "mxr.h"
+
+// MatX user code spells the complex type, fixed-size arrays and scalar math
+// with libcu++'s names (cuda::std::complex<float>, cuda::std::array,
+// cuda::std::pow in examples/simple_radar_pipeline.h:94,118,156).  On this
+// toolchain they are the C++ standard library's, whose math functions hipcc
+// also provides in device code.
+namespace cuda {
+namespace std {
+using ::std::array;
+template <typename T>
+using complex = ::std::complex<T>;
+using ::std::abs;
+using ::std::exp;
+using ::std::log;
+using ::std::pow;
+using ::std::sqrt;
+}  // namespace std
+}  // namespace cuda
+
+// Macros MatX operator code uses (include/matx/core/defines.h).
+#ifndef __MATX_INLINE__
+#define __MATX_INLINE__ inline
+#endif
+#ifndef __MATX_HOST__
+#define __MATX_HOST__ __host__
+#endif
+#ifndef __MATX_DEVICE__
+#define __MATX_DEVICE__ __device__
+#endif
+#ifndef MATX_LOG_DEBUG
+#define MATX_LOG_DEBUG(...) ((void)0)
+#endif
 
 namespace matx {
 
@@ -159,9 +204,9 @@ class tensor_t {
            const std::array<index_t, RANK> &stride, matxMemorySpace_t space)
       : storage_(std::move(st)), data_(data), shape_(shape), stride_(stride), space_(space) {}
 
-  index_t Size(int d) const { return shape_[d]; }
-  index_t Stride(int d) const { return stride_[d]; }
-  T *Data() const { return data_; }
+  MATX_AMD_HD index_t Size(int d) const { return shape_[d]; }
+  MATX_AMD_HD index_t Stride(int d) const { return stride_[d]; }
+  MATX_AMD_HD T *Data() const { return data_; }
   const std::array<index_t, RANK> &Shape() const { return shape_; }
   const std::array<index_t, RANK> &Strides() const { return stride_; }
   const std::shared_ptr<void> &Storage() const { return storage_; }
@@ -179,9 +224,11 @@ class tensor_t {
     }
     return true;
   }
-  // host element access (managed or host memory; synchronise first)
+  // element access: on the host for managed or host memory (synchronise
+  // first); in device code of a user-defined operator (BaseOp) for any
+  // device-visible memory
   template <typename... I>
-  T &operator()(I... idx) const {
+  MATX_AMD_HD T &operator()(I... idx) const {
     static_assert(sizeof...(I) == RANK, "index count");
     const index_t ix[] = {(index_t)idx..., 0};
     index_t off = 0;
@@ -254,6 +301,27 @@ class tensor_t {
   template <typename Op>
   auto operator=(const Op &op) const;
 
+  // DLPack export (include/matx/core/tensor.h:1471-1555): a DLManagedTensor
+  // of this view, no copy, that holds a reference to the storage until the
+  // consumer calls its deleter.  Device memory and managed memory export as
+  // ROCm device memory, pinned host memory as ROCm host memory.
+  mxr_dl_managed *ToDlPack() const {
+    int dev = 0;
+    detail::hip_check(hipGetDevice(&dev), "hipGetDevice");
+    auto *hold = new std::shared_ptr<void>(storage_);
+    mxr_tensor d = desc();
+    if (RANK == 0) d.rank = 0;
+    mxr_dl_managed *m = nullptr;
+    const mxr_status st = mxr_tensor_to_dlpack(
+        &d, space_ == MATX_HOST_MALLOC_MEMORY ? 11 : 10, dev,
+        [](void *p) { delete static_cast<std::shared_ptr<void> *>(p); }, hold, &m);
+    if (st != MXR_SUCCESS) {
+      delete hold;
+      detail::check(st);
+    }
+    return m;
+  }
+
  private:
   void put_host(const std::vector<T> &h) {
     if ((index_t)h.size() != TotalSize() || !IsContiguous())
@@ -316,6 +384,7 @@ struct Node {
   cfloat c{};                     // CONST
   int op = 0, a = 0, b = 0;       // GEN (op, dim a, length b) or OP
   std::vector<NodeP> args;        // OP
+  std::vector<index_t> shape;     // GEN / CONST from a shaped generator (ones, hamming)
 };
 }  // namespace detail
 
@@ -364,6 +433,36 @@ inline Expr mk(int op, std::vector<Expr> a) {
   for (auto &e : a) n->args.push_back(e.n);
   return Expr{n};
 }
+// Shape of an expression: its tensor leaves' and shaped generators' extents
+// broadcast against each other (aligned at the last dimension, extent 1
+// stretches), as MatX derives an operator's Size() from its operands.
+// Empty when nothing in the expression has a shape.
+inline void merge_shape(std::vector<index_t> &acc, const std::vector<index_t> &sh) {
+  if (acc.size() < sh.size()) acc.insert(acc.begin(), sh.size() - acc.size(), 1);
+  const size_t pad = acc.size() - sh.size();
+  for (size_t d = 0; d < sh.size(); ++d) {
+    index_t &a = acc[pad + d];
+    if (a == sh[d] || sh[d] == 1) continue;
+    if (a != 1) throw matxException(MXR_INVALID_SIZE, "expression operands have incompatible shapes");
+    a = sh[d];
+  }
+}
+inline void expr_shape_into(const NodeP &n, std::vector<index_t> &acc, bool &any) {
+  if (n->kind == Node::TENSOR) {
+    merge_shape(acc, std::vector<index_t>(n->t.shape, n->t.shape + n->t.rank));
+    any = true;
+  } else if (!n->shape.empty()) {
+    merge_shape(acc, n->shape);
+    any = true;
+  }
+  for (auto &a : n->args) expr_shape_into(a, acc, any);
+}
+inline std::vector<index_t> expr_shape(const Expr &e) {
+  std::vector<index_t> acc;
+  bool any = false;
+  expr_shape_into(e.n, acc, any);
+  return any ? acc : std::vector<index_t>{};
+}
 }  // namespace detail
 
 #define MATX_AMD_BINOP(OPSYM, CODE)                                                      \
@@ -407,22 +506,32 @@ Expr hamming(const std::array<index_t, Rank> &shape) {
   n->op = MXR_EW_HAMMING;
   n->a = Dim;
   n->b = (int)shape[Dim];
+  n->shape.assign(shape.begin(), shape.end());
   return Expr{n};
 }
-// ones(shape) (include/matx/generators/ones.h)
-template <typename T = float, int Rank = 1>
-Expr ones(const std::array<index_t, Rank> &) { return detail::to_expr(1.0f); }
+// ones(shape) / zeros(shape) (include/matx/generators/ones.h, zeros.h): a
+// constant that carries its shape, so a transform can take it as an operand
+// (conv2d(ones({C, P, R}), mask, FULL)); the rank comes from the braced list.
+template <typename T = float, int Rank>
+Expr ones(const index_t (&shape)[Rank]) {
+  Expr e = detail::to_expr(1.0f);
+  auto n = std::make_shared<detail::Node>(*e.n);
+  n->shape.assign(shape, shape + Rank);
+  return Expr{n};
+}
+template <typename T = float, int Rank>
+Expr zeros(const index_t (&shape)[Rank]) {
+  Expr e = detail::to_expr(0.0f);
+  auto n = std::make_shared<detail::Node>(*e.n);
+  n->shape.assign(shape, shape + Rank);
+  return Expr{n};
+}
 
 // ------------------------------------------------------------ interop
 // DLPack import (make_tensor(DLManagedTensor*), include/matx/core/
 // make_tensor.h:1136-1190): the tensor takes ownership and runs the
-// producer's deleter when its last view goes.  mxr_dl_managed has the layout
-// of DLPack's DLManagedTensor.
-struct mxr_dl_managed {
-  mxr_dl_tensor dl_tensor;
-  void *manager_ctx;
-  void (*deleter)(mxr_dl_managed *self);
-};
+// producer's deleter when its last view goes.  mxr_dl_managed (mxr.h) has
+// the layout of DLPack's DLManagedTensor.
 template <typename T, int RANK>
 tensor_t<T, RANK> make_tensor_from_dlpack(mxr_dl_managed *m) {
   mxr_tensor d{};
@@ -655,8 +764,13 @@ template <typename T, int R>
 SumOp sum(const tensor_t<T, R> &a) {
   return SumOp{detail::to_expr(a), std::vector<index_t>(a.Shape().begin(), a.Shape().end())};
 }
-// sum(expr) needs the expression's shape: pass it explicitly
-inline SumOp sum(const Expr &e, std::vector<index_t> shape) { return SumOp{e, std::move(shape)}; }
+// sum(expr): the reduction runs over the expression's shape, taken from its
+// operands (sum(abs2(waveformPart)), examples/simple_radar_pipeline.h:291)
+inline SumOp sum(const Expr &e) {
+  auto sh = detail::expr_shape(e);
+  if (sh.empty()) throw matxException(MXR_INVALID_SIZE, "sum: the expression has no shaped operand");
+  return SumOp{e, std::move(sh)};
+}
 
 // ---------------------------------------------------------------- run()
 namespace detail {
@@ -722,13 +836,14 @@ struct Operand {
   mxr_tensor t{};
   std::shared_ptr<void> tmp;
 };
-inline Operand operand(const Expr &e, const std::vector<index_t> &shape, int dtype, hipStream_t s) {
+inline Operand operand(const Expr &e, const std::vector<index_t> &shape_in, int dtype, hipStream_t s) {
   Operand o;
   if (e.n->kind == Node::TENSOR) {
     o.t = e.n->t;
     return o;
   }
-  if (shape.empty()) throw matxException(MXR_INVALID_PARAMETER, "expression operand needs an explicit shape");
+  const std::vector<index_t> shape = shape_in.empty() ? expr_shape(e) : shape_in;
+  if (shape.empty()) throw matxException(MXR_INVALID_PARAMETER, "expression operand has no shape");
   index_t total = 1;
   mxr_tensor t{};
   t.rank = (int)shape.size();
@@ -823,6 +938,123 @@ auto tensor_t<T, RANK>::operator=(const Op &op) const {
     return SetOp<tensor_t, Expr>{*this, detail::to_expr(op)};
   }
 }
+
+// ------------------------------------------------ user-defined operators
+// MatX's custom-operator form (examples/simple_radar_pipeline.h:50-150): a
+// class derived from BaseOp<Op> with a templated __device__ operator() over
+// the output's indices, Size(i) and Rank(); `Op(...).run(exec)` evaluates it
+// once per output element.  The operator is copied byte for byte into the
+// kernel's argument block (its tensor members are views: pointer, shape,
+// strides), so the device code never constructs or destroys one.
+namespace detail {
+// Capability vocabulary user operators query in get_capability
+// (include/matx/core/capabilities.h).  This launcher runs one element per
+// thread, so CapType::ept is always ONE.
+enum class ElementsPerThread : int {
+  INVALID = 0, ONE = 1, TWO = 2, FOUR = 4, EIGHT = 8, SIXTEEN = 16, THIRTY_TWO = 32
+};
+enum class OperatorCapability : int { NONE = 0, ELEMENTS_PER_THREAD, ALIASED_MEMORY, SUPPORTS_JIT };
+template <OperatorCapability Cap>
+struct capability_attributes {
+  using type = bool;
+  static constexpr bool default_value = false;
+};
+template <>
+struct capability_attributes<OperatorCapability::ELEMENTS_PER_THREAD> {
+  using type = std::array<ElementsPerThread, 2>;  // {lowest, highest} supported
+  static constexpr type default_value{ElementsPerThread::ONE, ElementsPerThread::THIRTY_TWO};
+};
+template <ElementsPerThread EPT, bool JIT = false>
+struct CapabilityParams {
+  static constexpr ElementsPerThread ept = EPT;
+  static constexpr bool jit = JIT;
+};
+template <typename Op>
+struct is_tensor_view : std::false_type {};
+template <typename T, int R>
+struct is_tensor_view<tensor_t<T, R>> : std::true_type {};
+// Tensors and scalars have the defaults; an operator answers for itself.
+template <OperatorCapability Cap, typename Op, typename In>
+auto get_operator_capability(const Op &op, const In &in) {
+  if constexpr (is_tensor_view<Op>::value || std::is_arithmetic_v<Op>) return capability_attributes<Cap>::default_value;
+  else return op.template get_capability<Cap>(in);
+}
+// Element-per-thread ranges intersect; ALIASED_MEMORY is true if any operand
+// aliases; every other flag holds only if it holds for all.
+template <OperatorCapability Cap, typename A>
+auto combine_capabilities(A a) { return a; }
+template <OperatorCapability Cap, typename A, typename B, typename... Rest>
+auto combine_capabilities(A a, B b, Rest... rest) {
+  if constexpr (Cap == OperatorCapability::ELEMENTS_PER_THREAD) {
+    A c{(int)a[0] > (int)b[0] ? a[0] : b[0], (int)a[1] < (int)b[1] ? a[1] : b[1]};
+    return combine_capabilities<Cap>(c, rest...);
+  } else if constexpr (Cap == OperatorCapability::ALIASED_MEMORY) {
+    return combine_capabilities<Cap>(A(a || b), rest...);
+  } else {
+    return combine_capabilities<Cap>(A(a && b), rest...);
+  }
+}
+
+#if defined(__HIPCC__)
+template <typename Op>
+struct alignas(alignof(Op)) OpImage {
+  unsigned char b[sizeof(Op)];
+};
+template <int R>
+struct OpExtents {
+  index_t n[R > 0 ? R : 1];
+};
+template <typename Op, size_t... I>
+__device__ __forceinline__ void call_user_op(Op &op, const index_t *ix, std::index_sequence<I...>) {
+  op.template operator()<CapabilityParams<ElementsPerThread::ONE, false>>(ix[I]...);
+}
+template <typename Op, int R>
+__global__ __launch_bounds__(256) void user_op_kernel(OpImage<Op> img, OpExtents<R> ext, index_t total) {
+  index_t i = (index_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  index_t ix[R > 0 ? R : 1];
+  for (int d = R - 1; d >= 0; --d) {
+    ix[d] = i % ext.n[d];
+    i /= ext.n[d];
+  }
+  Op &op = const_cast<Op &>(*reinterpret_cast<const Op *>(img.b));
+  call_user_op(op, ix, std::make_index_sequence<(size_t)R>{});
+}
+template <typename Op>
+void run_user_op(const Op &op, hipStream_t s) {
+  constexpr int R = Op::Rank();
+  OpExtents<R> ext{};
+  index_t total = 1;
+  for (int d = 0; d < R; ++d) {
+    ext.n[d] = op.Size((uint32_t)d);
+    total *= ext.n[d];
+  }
+  if (total <= 0) return;
+  const index_t blocks = (total + 255) / 256;
+  if (blocks > (index_t)std::numeric_limits<int32_t>::max())
+    throw matxException(MXR_INVALID_SIZE, "user operator: output too large for one launch");
+  OpImage<Op> img;
+  std::memcpy(img.b, static_cast<const void *>(&op), sizeof(Op));
+  user_op_kernel<Op, R><<<dim3((unsigned)blocks), dim3(256), 0, s>>>(img, ext, total);
+  hip_check(hipGetLastError(), "user operator launch");
+}
+#else
+template <typename Op>
+void run_user_op(const Op &, hipStream_t) {
+  static_assert(sizeof(Op) == 0,
+                "a user-defined operator (BaseOp) runs its own device code: compile this "
+                "translation unit with hipcc");
+}
+#endif
+}  // namespace detail
+
+template <typename Derived>
+class BaseOp {
+ public:
+  using matxop = bool;
+  void run(const HIPExecutor &ex) const { run(ex.getStream()); }
+  void run(hipStream_t s = nullptr) const { detail::run_user_op(static_cast<const Derived &>(*this), s); }
+};
 
 // --------------------------------------------------- fused radar stages
 // One-call stage entries (pulse compression in place, canceller + window +

@@ -9,8 +9,8 @@
 // fft + conj, pulse compression as fft -> multiply -> ifft, the canceller as
 // conv1d over permuted views, the Doppler stage as a windowed fft along the
 // pulse axis, CFAR as conv2d + divide + a threshold expression (the
-// expression form of calcDets).  FusedRadarPipeline runs the same dataflow
-// as three fused kernels.
+// expression form of calcDets; MatX's user-operator form of it runs from
+// tests/cpp/matx_radar_pipeline.h).
 #pragma once
 
 #include "matx_amd.h"
@@ -58,7 +58,7 @@ class RadarPipeline {
     auto wFull = slice(waveformView, {0}, {numSamplesRnd});
     auto wT = waveformView.template Clone<3>({numChannels, numPulses, matxKeepDim});
     (wPart = wPart * hamming<0, 1>({waveformLength})).run(exec);
-    (norms = sum(abs2(wPart), {waveformLength})).run(exec);
+    (norms = sum(abs2(wPart))).run(exec);
     (norms = sqrt(norms)).run(exec);
     (wPart = wPart / norms).run(exec);
     (wFull = fft(wPart, numSamplesRnd)).run(exec);
@@ -104,9 +104,7 @@ class RadarPipeline {
  private:
   void ComputeCFARNorm() {
     if (cfarNormReady) return;
-    tensor_t<float, 3> ones3({numChannels, numPulsesRnd, R}, MATX_DEVICE_MEMORY);
-    (ones3 = ones<float, 3>({numChannels, numPulsesRnd, R})).run(exec);
-    (normT = conv2d(ones3, cfarMask, MATX_C_MODE_FULL)).run(exec);
+    (normT = conv2d(ones({numChannels, numPulsesRnd, R}), cfarMask, MATX_C_MODE_FULL)).run(exec);
     exec.sync();
     cfarNormReady = true;
   }
@@ -123,14 +121,6 @@ class RadarPipeline {
   tensor_t<float, 2> cfarMask;
   HIPExecutor exec;
   bool cfarNormReady = false;
-};
-
-// The same stage interface with each stage as one fused kernel pass (two
-// for pulse compression, whose waveform spectrum is rebuilt every call as
-// MatX does).
-class FusedRadarPipeline : public RadarPipeline {
- public:
-  using RadarPipeline::RadarPipeline;
 };
 
 }  // namespace matx_amd_examples

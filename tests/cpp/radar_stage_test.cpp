@@ -1,14 +1,19 @@
-// Stage test of the C++ front end: runs the RadarPipeline example stage by
-// stage on the input written by tests/test_cpp_frontend.py and dumps every
-// stage's output, which the Python side compares with the FP64 oracle (the
-// per-stage checks of MatX's test/01_radar/MultiChannelRadarPipeline.cu).
-// Usage: radar_stage_test <dir> <channels>
+// Stage test of the C++ front end: runs a radar pipeline stage by stage on
+// the input written by tests/test_cpp_frontend.py and dumps every stage's
+// output, which the Python side compares with the FP64 oracle (the per-stage
+// checks of MatX's test/01_radar/MultiChannelRadarPipeline.cu).
+//   mode "matx":    MatX's own RadarPipeline and calcDets source
+//                   (matx_radar_pipeline.h), unchanged but for the executor
+//                   and stream types;
+//   mode "example": this library's example (matx_amd/examples).
+// Usage: radar_stage_test <dir> <channels> [matx|example]
 #include <cstdio>
 #include <cstdlib>
 #include <string>
 #include <vector>
 
 #include "../../matx_amd/examples/simple_radar_pipeline.h"
+#include "matx_radar_pipeline.h"
 
 using namespace matx;
 
@@ -33,36 +38,49 @@ static void dump(const std::string &path, const tensor_t<T, R> &t, HIPExecutor &
   fclose(f);
 }
 
+template <typename Pipeline>
+static void run_stages(Pipeline &p, const std::string &dir, index_t C, index_t P, index_t S, index_t L,
+                       HIPExecutor &ex) {
+  auto x = read_file<cfloat>(dir + "/x.bin", (size_t)P * S);
+  auto w = read_file<cfloat>(dir + "/w.bin", (size_t)L);
+  auto in = p.GetInputView();
+  const index_t N = in.Size(2);
+  std::vector<cfloat> h((size_t)C * P * N, cfloat(0.f, 0.f));
+  for (index_t c = 0; c < C; ++c)
+    for (index_t q = 0; q < P; ++q)
+      for (index_t s = 0; s < S; ++s) h[(c * P + q) * N + s] = x[q * S + s];
+  detail::hip_check(hipMemcpy(in.Data(), h.data(), sizeof(cfloat) * h.size(), hipMemcpyDefault), "copy");
+  detail::hip_check(hipMemcpy(p.GetwaveformView().Data(), w.data(), sizeof(cfloat) * L, hipMemcpyDefault),
+                    "copy");
+  p.PulseCompression();
+  dump(dir + "/pc.bin", p.GetInputView(), ex);
+  p.ThreePulseCanceller();
+  dump(dir + "/tpc.bin", p.GetTPCView(), ex);
+  p.DopplerProcessing();
+  dump(dir + "/dop.bin", p.GetTPCView(), ex);
+  p.CFARDetections();
+  dump(dir + "/dets.bin", p.GetDetections(), ex);
+  dump(dir + "/ba.bin", p.GetBackgroundAverages(), ex);
+  dump(dir + "/normT.bin", p.GetnormT(), ex);
+}
+
 int main(int argc, char **argv) {
   if (argc < 3) return 2;
   const std::string dir = argv[1];
   const index_t C = atoi(argv[2]), P = 128, S = 9000, L = 1000;
+  const std::string mode = argc > 3 ? argv[3] : "matx";
   hipStream_t stream;
   detail::hip_check(hipStreamCreate(&stream), "hipStreamCreate");
   HIPExecutor ex(stream);
   try {
-    matx_amd_examples::RadarPipeline p(P, S, L, C, stream);
+    if (mode == "matx") {
+      RadarPipeline<> p(P, S, L, C, stream);
+      run_stages(p, dir, C, P, S, L, ex);
+    } else {
+      matx_amd_examples::RadarPipeline p(P, S, L, C, stream);
+      run_stages(p, dir, C, P, S, L, ex);
+    }
     auto x = read_file<cfloat>(dir + "/x.bin", (size_t)P * S);
-    auto w = read_file<cfloat>(dir + "/w.bin", (size_t)L);
-    auto in = p.GetInputView();
-    const index_t N = in.Size(2);
-    std::vector<cfloat> h((size_t)C * P * N, cfloat(0.f, 0.f));
-    for (index_t c = 0; c < C; ++c)
-      for (index_t q = 0; q < P; ++q)
-        for (index_t s = 0; s < S; ++s) h[(c * P + q) * N + s] = x[q * S + s];
-    detail::hip_check(hipMemcpy(in.Data(), h.data(), sizeof(cfloat) * h.size(), hipMemcpyDefault), "copy");
-    detail::hip_check(hipMemcpy(p.GetwaveformView().Data(), w.data(), sizeof(cfloat) * L, hipMemcpyDefault),
-                      "copy");
-    p.PulseCompression();
-    dump(dir + "/pc.bin", p.GetInputView(), ex);
-    p.ThreePulseCanceller();
-    dump(dir + "/tpc.bin", p.GetTPCView(), ex);
-    p.DopplerProcessing();
-    dump(dir + "/dop.bin", p.GetTPCView(), ex);
-    p.CFARDetections();
-    dump(dir + "/dets.bin", p.GetDetections(), ex);
-    dump(dir + "/ba.bin", p.GetBackgroundAverages(), ex);
-    dump(dir + "/normT.bin", p.GetnormT(), ex);
     // size checks throw before launch (FFT.cu:661-719)
     auto tc = make_tensor<cfloat, 1>({16});
     auto t2 = make_tensor<cfloat, 1>({32});
@@ -130,6 +148,20 @@ int main(int argc, char **argv) {
       npy_threw = true;
     }
     printf(npy_threw ? "npy-shape-check ok\n" : "npy-shape-check MISSING\n");
+    // DLPack export of a permuted view and import back: same memory, shape
+    // and strides; the imported tensor owns the export and releases it
+    {
+      auto t = make_tensor<float, 2>({6, 9}, MATX_DEVICE_MEMORY);
+      auto v = t.Permute({1, 0});
+      mxr_dl_managed *m = v.ToDlPack();
+      bool ok = m->dl_tensor.data == (void *)v.Data() && m->dl_tensor.ndim == 2 &&
+                m->dl_tensor.shape[0] == 9 && m->dl_tensor.shape[1] == 6 &&
+                m->dl_tensor.strides[0] == 1 && m->dl_tensor.strides[1] == 9 &&
+                m->dl_tensor.dtype_code == 2 && m->dl_tensor.device_type == 10;
+      auto back = make_tensor_from_dlpack<float, 2>(m);
+      ok = ok && back.Data() == v.Data() && back.Size(0) == 9 && back.Stride(1) == 9;
+      printf(ok ? "dlpack ok\n" : "dlpack MISMATCH\n");
+    }
   } catch (const matxException &e) {
     fprintf(stderr, "matx error %d: %s\n", e.code(), e.what());
     return 1;

@@ -1,9 +1,15 @@
-"""The C++ front end (include/matx_amd.h) end to end: the RadarPipeline
-example (matx_amd/examples/simple_radar_pipeline.h), written with MatX's
-operator API, runs stage by stage on MatX's seed-1234 test input through the
-compiled tests/cpp/radar_stage_test; every stage is compared with the FP64
-oracle in MatX's own dataflow (the per-stage checks of
-test/01_radar/MultiChannelRadarPipeline.cu, end to end)."""
+"""The C++ front end (include/matx_amd.h) end to end: two radar pipelines
+run stage by stage on MatX's seed-1234 test input through the compiled
+tests/cpp/radar_stage_test --
+
+  * "matx": MatX's own RadarPipeline and calcDets user operator
+    (tests/cpp/matx_radar_pipeline.h: examples/simple_radar_pipeline.h:50-520
+    with only the executor, stream and memset names changed);
+  * "example": this library's example (matx_amd/examples/);
+
+every stage is compared with the FP64 oracle in MatX's own dataflow (the
+per-stage checks of test/01_radar/MultiChannelRadarPipeline.cu, end to
+end).  A CPU test compiles a host-only program against the header with g++."""
 import os
 import subprocess
 
@@ -19,19 +25,52 @@ def test_cpp_programs_built():
         assert os.path.exists(os.path.join(BIN, name)), f"{name} not built"
 
 
+HOST_ONLY = r"""
+#include "matx_amd.h"
+using namespace matx;
+int main() {
+  auto a = make_tensor<cfloat, 2>({4, 64});
+  auto b = make_tensor<cfloat, 2>({4, 64});
+  auto p = make_tensor<float, 2>({4, 64});
+  tensor_t<float, 0> n;
+  make_tensor(n);
+  HIPExecutor ex(nullptr);
+  (b = fft(a)).run(ex);
+  (p = abs2(b) * hamming<1, 2, float>({4, 64})).run(ex);
+  (n = sum(abs2(a))).run(ex);
+  auto c = make_tensor<float, 2>({8, 76});
+  (c = conv2d(ones({4, 64}), p, MATX_C_MODE_FULL)).run(ex);
+  return 0;
+}
+"""
+
+
+def test_cpp_header_host_only_gxx(tmp_path):
+    """Code without user-defined operators needs no device compiler: the
+    header compiles with g++ (syntax and semantics, no link)."""
+    src = tmp_path / "host_only.cpp"
+    src.write_text(HOST_ONLY)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-D__HIP_PLATFORM_AMD__",
+                        "-I" + os.path.join(ROOT, "include"), "-I/opt/rocm/include", str(src)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+
 @pytest.mark.gpu
-def test_cpp_radar_stages(orc, matx_inputs, golden, tmp_path):
+@pytest.mark.parametrize("mode", ["matx", "example"])
+def test_cpp_radar_stages(orc, matx_inputs, golden, tmp_path, mode):
     x, w = matx_inputs
     C, P, S, L, R, Pr = 2, 128, 9000, 1000, 8001, 256
     x.astype(np.complex64).tofile(tmp_path / "x.bin")
     npy_in = (np.arange(21) + 1j * np.arange(21)[::-1]).astype(np.complex64).reshape(3, 7)
     np.save(tmp_path / "npy_in.npy", npy_in)
     w.astype(np.complex64).tofile(tmp_path / "w.bin")
-    r = subprocess.run([os.path.join(BIN, "radar_stage_test"), str(tmp_path), str(C)],
+    r = subprocess.run([os.path.join(BIN, "radar_stage_test"), str(tmp_path), str(C), mode],
                        capture_output=True, text=True, timeout=180)
     assert r.returncode == 0, r.stderr
     assert "size-check ok" in r.stdout and "stages ok" in r.stdout
     assert "npy-shape-check ok" in r.stdout
+    assert "dlpack ok" in r.stdout
     st = orc.radar_pipeline(x[None], w, mode="matx", stages=True)
 
     def rel(a, b):
