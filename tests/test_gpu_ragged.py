@@ -99,3 +99,64 @@ def test_ragged_graph_replays(mx):
         for p, e in zip(rb.pw, eager):
             assert torch.equal(p, e), k
     rb.close()
+
+
+def config3_jobs(seed, n=256):
+    """n jobs from the ragged workload's distribution: C uniform in {1..8},
+    P in {16, 32, 64, 128, 256}, S uniform in [1024, 16384], L uniform in
+    [64, min(2048, S)]; unit complex noise plus 5 echoes of the job's
+    Hamming-windowed LFM (0.5 fs) at 10 dB.  The first five jobs are pinned
+    to the corners of the distribution (P = 128 and 256, S = 16384,
+    L = 2048, C = 8, the smallest job) so every kernel class is drawn."""
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    pinned = [(8, 256, 16384, 2048), (3, 128, 16384, 1500), (2, 256, 9000, 2048),
+              (8, 128, 12000, 64), (1, 16, 1024, 64)]
+    jobs, shapes = [], []
+    for i in range(n):
+        if i < len(pinned):
+            C, P, S, L = pinned[i]
+        else:
+            C = int(rng.integers(1, 9))
+            P = int(rng.choice([16, 32, 64, 128, 256]))
+            S = int(rng.integers(1024, 16385))
+            L = int(rng.integers(64, min(2048, S) + 1))
+        t = np.arange(L)
+        w = torch.tensor((np.exp(2j * np.pi * (-0.25 * t + 0.25 / L * t * t)) *
+                          np.hamming(L)).astype(np.complex64), device="cuda")
+        x = torch.randn((C, P, S), dtype=torch.complex64, device="cuda", generator=g)
+        for d in rng.integers(0, S - L + 1, 5):
+            x[:, :, int(d):int(d) + L] += np.sqrt(10.0) * w
+        jobs.append((x, w))
+        shapes.append((C, P, S, L))
+    return jobs, shapes
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_ragged_config3_distribution(mx, orc, persistent):
+    """The full 256-job batch: every job against the per-job entries on the
+    GPU, and the pinned corner jobs plus three drawn jobs against the FP64
+    oracle (rel L2 <= 1e-5), for the default and the persistent launch."""
+    jobs, shapes = config3_jobs(31)
+    rb = mx.ragged.RaggedBatch(jobs, autotune=True, persistent=persistent)
+    out = rb.execute()
+    torch.cuda.synchronize()
+    ps = {s[1] for s in shapes}
+    assert {128, 256} <= ps
+    sizes = {rb.block_size(i) for i in range(len(jobs))}
+    assert len(sizes) >= 2, sizes
+    for i, (x, w) in enumerate(jobs):
+        ref = plain(mx, x, w)
+        r = float((out[i] - ref).norm() / ref.norm())
+        assert r < RTOL, (i, shapes[i], rb.block_size(i), r)
+    check = list(range(5)) + [17, 101, 222]
+    for i in check:
+        x, w = jobs[i]
+        xh, wh = x.cpu().numpy(), w.cpu().numpy()
+        C, P, S, L = shapes[i]
+        n = mx.ragged.next_pow2(S)
+        Wc = orc.waveform_spectrum(wh, n, True, True)
+        y = orc.pulse_compress(xh.reshape(C * P, S), Wc, S - L + 1).reshape(C, P, S - L + 1)
+        D = orc.doppler(y, P, mx.ragged.next_pow2(P))
+        assert rel(out[i].cpu().numpy(), orc.abs2(D)) < RTOL, (i, shapes[i], rb.block_size(i))
+    rb.close()
