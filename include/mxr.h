@@ -113,15 +113,40 @@ typedef struct mxr_dl_tensor {
 } mxr_dl_tensor;
 mxr_status mxr_tensor_from_dlpack(mxr_tensor *out, const mxr_dl_tensor *src);
 
+/* DLPack's DLManagedTensor layout: the tensor, the producer's context and
+ * the deleter the consumer calls once when it is done. */
+typedef struct mxr_dl_managed {
+  mxr_dl_tensor dl_tensor;
+  void *manager_ctx;
+  void (*deleter)(struct mxr_dl_managed *self);
+} mxr_dl_managed;
+
+/* Export (tensor_t::ToDlPack, include/matx/core/tensor.h:1471-1555): a
+ * DLManagedTensor describing t's memory, no copy, with device {device_type,
+ * device_id} (10 = ROCm device memory, 11 = ROCm pinned host memory) and
+ * t's shape and element strides.  The library allocates the struct and its
+ * shape / stride arrays; the consumer's single call of its deleter frees
+ * them and then calls release(release_ctx) (release may be NULL), where the
+ * producer drops the reference it took for the export, as MatX's export
+ * holds a reference to the tensor's storage.  complex64, float32, int32. */
+mxr_status mxr_tensor_to_dlpack(const mxr_tensor *t, int32_t device_type, int32_t device_id,
+                                void (*release)(void *ctx), void *release_ctx,
+                                mxr_dl_managed **out);
+
 /* ------------------------------------------------------------------ FFT */
 /* Batched 1-D complex-to-complex FFT along `axis`.
  * n = transform size; 0 means in->shape[axis].  The input is zero-padded or
  * truncated to n, and out->shape[axis] must equal n; every other dimension of
  * in and out must match (batch dims).  Any strides; in == out allowed.
- * Power-of-two n from 1 to 16384 (Stockham radix-16/8/4/2 passes in LDS);
- * any other n from 3 to 8192 through Bluestein's chirp-z identity (one
- * power-of-two convolution of length >= 2n - 1 in LDS; cuFFT's arbitrary
- * sizes, which MatX inherits, are mixed-radix).
+ * Power-of-two n from 1 to 16384: one workgroup per transform (Stockham
+ * radix-16/8/4/2 passes in LDS); from 32768 to 2^24: two passes of short
+ * transforms through a per-stream work matrix of batch * n elements (the
+ * four-step factorisation).  Any other n from 3 to 8192 through Bluestein's
+ * chirp-z identity (one power-of-two convolution of length >= 2n - 1 in
+ * LDS), and from 8193 to 2^23 through Bluestein with two-pass convolution
+ * transforms (two work matrices).  The work matrices grow on first use:
+ * run a size once before capturing it in a graph.  (cuFFT's arbitrary sizes,
+ * which MatX inherits, are mixed-radix.)
  * Replaces fft_impl / ifft_impl (include/matx/transforms/fft/fft_cuda.h:680-774,
  * size rules include/matx/operators/fft.h:111-199 and fft_common.h:39-140,
  * normalisation fft_cuda.h:126-188). */

@@ -369,6 +369,292 @@ static mxr_status dispatch_czt(int64_t n, FftArgs a, bool inv, hipStream_t s) {
   return MXR_NOT_SUPPORTED;
 }
 
+// ------------------------------------------- long transforms: two passes
+// n = N1 * N2 > 16384 (powers of two up to 2^24) as two passes of short
+// transforms over the n-point sequence viewed as a matrix (the "four-step"
+// factorisation with the transposes folded into the passes' addressing):
+//   input index  i = n1 + N1 n2,  output index k = N2 k1 + k2,
+//   X[N2 k1 + k2] = sum_n1 W_N1^(n1 k1) W_n^(n1 k2) sum_n2 x[n1 + N1 n2] W_N2^(n2 k2).
+// Pass 1: for each n1, an N2-point transform over n2 (input stride N1),
+//         times W_n^(n1 k2), stored row n1 of a work matrix [N1][N2].
+// Pass 2: for each k2, an N1-point transform over n1 (work stride N2),
+//         stored at output index N2 k1 + k2 with the norm scale.
+// A workgroup owns TB neighbouring columns (n1 in pass 1, k2 in pass 2):
+// pass 1 gathers TB consecutive input samples per n2 and stores whole work
+// rows; pass 2 gathers TB consecutive work elements per n1 and stores TB
+// consecutive outputs per k1, so every HBM access is a contiguous run.
+// W_n^m (m = n1 k2 < n) is the product of two table entries,
+// lo[m mod 2^s] * hi[m >> s], each built in double and rounded once.
+//
+// Sizes that are not powers of two above the single-workgroup Bluestein
+// range use the same identity as czt_kernel with M = next_pow2(2n - 1)
+// points through these two passes: pass 1 of the forward gathers x.c (zero
+// beyond n), pass 2 of the forward stores the product with the chirp
+// spectrum, pass 2 of the inverse stores c.y.
+enum BigMode {
+  BIG_VIEW_TO_WORK = 0,   // pass 1 of a plain transform
+  BIG_WORK_TO_VIEW = 1,   // pass 2 of a plain transform
+  BIG_CHIRP_TO_WORK = 2,  // Bluestein: pass 1 of the forward, gathering x.c
+  BIG_WORK_SPEC = 3,      // Bluestein: pass 2 of the forward, times the spectrum
+  BIG_WORK_TO_WORK = 4,   // Bluestein: pass 1 of the inverse
+  BIG_WORK_CHIRP = 5,     // Bluestein: pass 2 of the inverse, c.y to the view
+};
+
+struct BigArgs {
+  FftArgs f;               // the caller's views (gather: pass 1, scatter: pass 2)
+  const float2 *src;       // work matrix read by pass 2, or by a Bluestein inverse pass 1
+  float2 *dst;             // work matrix written by pass 1, or the Bluestein spectrum product
+  int64_t n;               // N1 * N2
+  int64_t N1, N2;
+  const float2 *tw_lo, *tw_hi;  // W_n^m = tw_lo[m & lo_mask] * tw_hi[m >> lo_bits]
+  int lo_bits;
+  const float2 *tw_pass;   // Stockham table of the short transform
+  int64_t czt_n;           // Bluestein: the user's transform size
+  const float2 *chirp, *spec;
+  int conj_io;             // Bluestein inverse DFT: conjugate in and out
+  int64_t cols;            // columns per transform (N1 in pass 1, N2 in pass 2)
+};
+
+template <int N>
+struct BigCfg {
+  static constexpr int TB = (8192 / N) > 64 ? 64 : ((8192 / N) < 2 ? 2 : 8192 / N);
+  static constexpr int NT = 256;
+};
+
+template <int N, bool INV, int MODE>
+__global__ __launch_bounds__(256) void fft_big_kernel(BigArgs a) {
+  constexpr int TB = BigCfg<N>::TB;
+  constexpr int NT = BigCfg<N>::NT;
+  constexpr bool PASS1 = MODE == BIG_VIEW_TO_WORK || MODE == BIG_CHIRP_TO_WORK || MODE == BIG_WORK_TO_WORK;
+  using LY = Layout<N, TB, true>;
+  using SQ = Seq<N, false>;
+  __shared__ float2 lds[LY::LDS_ELEMS];
+  const FftArgs &f = a.f;
+  const int64_t nblk = a.cols / TB;
+  const int64_t b = blockIdx.x / nblk;           // transform (batch) index
+  const int64_t c0 = (blockIdx.x % nblk) * TB;   // first column of this workgroup
+  int64_t base_in = 0, base_out = 0;
+  batch_offsets(f.bi, b, &base_in, &base_out);
+  const int64_t wbase = b * a.n;
+
+  // gather: element i of column c0 + t, lanes over t (consecutive addresses)
+  for (int e = threadIdx.x; e < TB * N; e += NT) {
+    const int t = e % TB, i = e / TB;
+    const int64_t c = c0 + t;
+    float2 v;
+    if constexpr (MODE == BIG_VIEW_TO_WORK) {
+      v = fft_gather(f, base_in, c + a.N1 * i);
+    } else if constexpr (MODE == BIG_CHIRP_TO_WORK) {
+      const int64_t g = c + a.N1 * i;
+      v = make_float2(0.f, 0.f);
+      if (g < a.czt_n) {
+        float2 x = fft_gather(f, base_in, g);
+        if (a.conj_io) x.y = -x.y;
+        v = cmul(x, a.chirp[g]);
+      }
+    } else if constexpr (MODE == BIG_WORK_TO_WORK) {
+      v = a.src[wbase + c + a.N1 * i];
+    } else {  // pass 2 reads the work matrix [N1][N2]: row i, column c
+      v = a.src[wbase + i * a.N2 + c];
+    }
+    lds[LY::addr(t, i)] = v;
+  }
+  __syncthreads();
+  lds_passes<LY, SQ, N, NT, TB, INV, 0, SQ::n>(lds, a.tw_pass);
+
+  if constexpr (PASS1) {
+    // row c of the work matrix, times W_n^(c k); lanes over k
+    for (int e = threadIdx.x; e < TB * N; e += NT) {
+      const int t = e / N, k = e % N;
+      const int64_t c = c0 + t;
+      const int64_t m = c * k;
+      float2 w = cmul(a.tw_lo[m & ((1LL << a.lo_bits) - 1)], a.tw_hi[m >> a.lo_bits]);
+      if (INV) w.y = -w.y;
+      a.dst[wbase + c * a.N2 + k] = cmul(lds[LY::addr(t, k)], w);
+    }
+  } else {
+    // output index N2 k + c; lanes over t
+    for (int e = threadIdx.x; e < TB * N; e += NT) {
+      const int t = e % TB, k = e / TB;
+      const int64_t kg = a.N2 * k + c0 + t;
+      float2 v = lds[LY::addr(t, k)];
+      if constexpr (MODE == BIG_WORK_SPEC) {
+        a.dst[wbase + kg] = cmul(v, a.spec[kg]);
+      } else if constexpr (MODE == BIG_WORK_CHIRP) {
+        if (kg < f.out_len) {
+          float2 y = cscale(cmul(v, a.chirp[kg]), f.scale);
+          if (a.conj_io) y.y = -y.y;
+          fft_scatter(f, base_out, kg, y);
+        }
+      } else {
+        if (kg < f.out_len) fft_scatter(f, base_out, kg, cscale(v, f.scale));
+      }
+    }
+  }
+}
+
+namespace {
+std::mutex g_big_mu;
+std::map<std::pair<int, int64_t>, float2 *> g_big_tw;  // (device, n) -> [lo | hi]
+
+void clear_big_tw() {
+  std::lock_guard<std::mutex> g(g_big_mu);
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  for (auto &kv : g_big_tw) {
+    (void)hipSetDevice(kv.first.first);
+    (void)hipFree(kv.second);
+  }
+  (void)hipSetDevice(cur);
+  g_big_tw.clear();
+}
+CacheClearRegistrar g_reg_big(clear_big_tw);
+
+int ilog2(int64_t v) {
+  int r = 0;
+  while ((1LL << r) < v) ++r;
+  return r;
+}
+}  // namespace
+
+// W_n^m split tables for m < n: lo[j] = W^j (j < 2^s), hi[j] = W^(j 2^s)
+// (j < n / 2^s), s = ceil(log2(n) / 2), exponents reduced in integers.
+static const float2 *big_twiddles(int64_t n, int *lo_bits) {
+  const int lb = (ilog2(n) + 1) / 2;
+  *lo_bits = lb;
+  const auto key = std::make_pair(device_id(), n);
+  std::lock_guard<std::mutex> g(g_big_mu);
+  auto it = g_big_tw.find(key);
+  if (it != g_big_tw.end()) return it->second;
+  const int64_t nlo = 1LL << lb, nhi = n >> lb;
+  std::vector<float2> host((size_t)(nlo + nhi));
+  const double two_pi = 6.283185307179586476925286766559;
+  auto w = [&](int64_t m) {
+    const double ang = -two_pi * (double)(m % n) / (double)n;
+    return make_float2((float)std::cos(ang), (float)std::sin(ang));
+  };
+  for (int64_t j = 0; j < nlo; ++j) host[(size_t)j] = w(j);
+  for (int64_t j = 0; j < nhi; ++j) host[(size_t)(nlo + j)] = w(j << lb);
+  float2 *d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * host.size()) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, host.data(), sizeof(float2) * host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  g_big_tw[key] = d;
+  return d;
+}
+
+// Largest transform of the two-pass path, and its split: N1 <= N2, both
+// powers of two of at most 4096 points.
+constexpr int64_t BIG_MAX_N = 1LL << 24;
+
+template <int N, bool INV, int MODE>
+static mxr_status launch_big_one(const BigArgs &a, hipStream_t s) {
+  constexpr int TB = BigCfg<N>::TB;
+  BigArgs b = a;
+  b.tw_pass = twiddles_device(N, false);
+  MXR_REQUIRE(b.tw_pass != nullptr, MXR_OUT_OF_MEMORY, "fft: twiddle table allocation failed");
+  const int64_t grid = a.f.nbatch * (b.cols / TB);
+  MXR_REQUIRE(grid <= 0x7fffffff, MXR_INVALID_SIZE, "fft: batch too large");
+  ProfScope ps("fft_big_kernel", s);
+  fft_big_kernel<N, INV, MODE><<<dim3((unsigned)grid), dim3(256), 0, s>>>(b);
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
+template <bool INV, int MODE>
+static mxr_status launch_big(int64_t N, const BigArgs &a, hipStream_t s) {
+  switch (N) {
+    case 8: return launch_big_one<8, INV, MODE>(a, s);
+    case 16: return launch_big_one<16, INV, MODE>(a, s);
+    case 32: return launch_big_one<32, INV, MODE>(a, s);
+    case 64: return launch_big_one<64, INV, MODE>(a, s);
+    case 128: return launch_big_one<128, INV, MODE>(a, s);
+    case 256: return launch_big_one<256, INV, MODE>(a, s);
+    case 512: return launch_big_one<512, INV, MODE>(a, s);
+    case 1024: return launch_big_one<1024, INV, MODE>(a, s);
+    case 2048: return launch_big_one<2048, INV, MODE>(a, s);
+    case 4096: return launch_big_one<4096, INV, MODE>(a, s);
+    default: break;
+  }
+  set_error(MXR_NOT_SUPPORTED, "fft: no two-pass kernel for a %lld-point pass", (long long)N);
+  return MXR_NOT_SUPPORTED;
+}
+
+// Sets n, N1, N2 and the twiddle tables of an m-point two-pass transform.
+static mxr_status big_setup(BigArgs &b, int64_t m) {
+  const int lg = ilog2(m);
+  b.n = m;
+  b.N1 = 1LL << (lg / 2);
+  b.N2 = m / b.N1;
+  const float2 *t = big_twiddles(m, &b.lo_bits);
+  MXR_REQUIRE(t != nullptr, MXR_OUT_OF_MEMORY, "fft: twiddle table allocation failed");
+  b.tw_lo = t;
+  b.tw_hi = t + (1LL << b.lo_bits);
+  return MXR_SUCCESS;
+}
+
+// A plain transform of power-of-two n in (16384, 2^24]: two passes through a
+// per-stream work matrix of nbatch * n elements.
+static mxr_status dispatch_big(int64_t n, const FftArgs &f, bool inv, hipStream_t s) {
+  BigArgs b{};
+  b.f = f;
+  mxr_status st = big_setup(b, n);
+  if (st) return st;
+  float2 *work = (float2 *)stream_workspace("fft_big", sizeof(float2) * (size_t)(f.nbatch * n), s);
+  if (!work) return workspace_failure("fft");
+  b.dst = work;
+  b.src = work;
+  b.cols = b.N1;
+  st = inv ? launch_big<true, BIG_VIEW_TO_WORK>(b.N2, b, s) : launch_big<false, BIG_VIEW_TO_WORK>(b.N2, b, s);
+  if (st) return st;
+  b.cols = b.N2;
+  return inv ? launch_big<true, BIG_WORK_TO_VIEW>(b.N1, b, s) : launch_big<false, BIG_WORK_TO_VIEW>(b.N1, b, s);
+}
+
+// Chirp and spectrum tables of an n-point Bluestein transform with an
+// M-point convolution (czt_tables), for M above the one-workgroup range.
+static const float2 *czt_tables(int n, int M);
+
+// Any size n in (8192, 2^23] that is not a power of two: Bluestein through
+// the two-pass M-point transforms, M = next_pow2(2n - 1) <= 2^24.
+static mxr_status dispatch_czt_big(int64_t n, const FftArgs &f, bool inv, hipStream_t s) {
+  int64_t M = 1;
+  while (M < 2 * n - 1) M <<= 1;
+  BigArgs b{};
+  b.f = f;
+  mxr_status st = big_setup(b, M);
+  if (st) return st;
+  const float2 *tab = czt_tables((int)n, (int)M);
+  MXR_REQUIRE(tab != nullptr, MXR_OUT_OF_MEMORY, "fft: table allocation failed for size %lld",
+              (long long)n);
+  b.czt_n = n;
+  b.chirp = tab;
+  b.spec = tab + n;
+  b.conj_io = inv ? 1 : 0;
+  const size_t bytes = sizeof(float2) * (size_t)(f.nbatch * M);
+  float2 *w1 = (float2 *)stream_workspace("fft_big", bytes, s);
+  float2 *w2 = w1 ? (float2 *)stream_workspace("fft_big_czt", bytes, s) : nullptr;
+  if (!w1 || !w2) return workspace_failure("fft");
+  // forward M-point transform of x.c, times the chirp spectrum (1/M in it)
+  b.dst = w1;
+  b.cols = b.N1;
+  if ((st = launch_big<false, BIG_CHIRP_TO_WORK>(b.N2, b, s))) return st;
+  b.src = w1;
+  b.dst = w2;
+  b.cols = b.N2;
+  if ((st = launch_big<false, BIG_WORK_SPEC>(b.N1, b, s))) return st;
+  // inverse M-point transform, then c.y for the first out_len outputs
+  b.src = w2;
+  b.dst = w1;
+  b.cols = b.N1;
+  if ((st = launch_big<true, BIG_WORK_TO_WORK>(b.N2, b, s))) return st;
+  b.src = w1;
+  b.cols = b.N2;
+  return launch_big<true, BIG_WORK_CHIRP>(b.N1, b, s);
+}
+
 static mxr_status fft_impl(const char *fn, const mxr_tensor *out, const mxr_tensor *in, int64_t n,
                            int dir, int norm, int axis, int kind, hipStream_t stream) {
   mxr_status st;
@@ -402,9 +688,9 @@ static mxr_status fft_impl(const char *fn, const mxr_tensor *out, const mxr_tens
                 (long long)out->shape[d]);
   }
   const bool pow2 = n >= 1 && (n & (n - 1)) == 0;
-  MXR_REQUIRE((pow2 && n <= 16384) || (n >= 1 && n <= CZT_MAX_N), MXR_NOT_SUPPORTED,
-              "%s: size %lld unsupported (powers of two up to 16384, other sizes up to %lld)", fn,
-              (long long)n, (long long)CZT_MAX_N);
+  MXR_REQUIRE((pow2 && n <= BIG_MAX_N) || (n >= 1 && n <= BIG_MAX_N / 2), MXR_NOT_SUPPORTED,
+              "%s: size %lld unsupported (powers of two up to %lld, other sizes up to %lld)", fn,
+              (long long)n, (long long)BIG_MAX_N, (long long)(BIG_MAX_N / 2));
   if (views_overlap(*in, *out)) {
     // a transform may overwrite its own input view (every element is read
     // before any is written), not a different view of the same memory
@@ -450,7 +736,8 @@ static mxr_status fft_impl(const char *fn, const mxr_tensor *out, const mxr_tens
     MXR_HIP_TRY(hipGetLastError());
     return MXR_SUCCESS;
   }
-  if (!pow2) return dispatch_czt(n, a, inv, stream);
+  if (!pow2) return n <= CZT_MAX_N ? dispatch_czt(n, a, inv, stream) : dispatch_czt_big(n, a, inv, stream);
+  if (n > 16384) return dispatch_big(n, a, inv, stream);
   a.tw = twiddles_device((int)n, false);
   MXR_REQUIRE(a.tw != nullptr, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
   return dispatch_generic(n, a, inv, stream);

@@ -115,9 +115,52 @@ def test_fft_bluestein_pad_norms_strided_inplace(mx, orc):
 
 
 def test_fft_unsupported_size_raises(mx):
-    x = torch.zeros((2, 8193), dtype=torch.complex64, device="cuda")
+    x = torch.zeros((1, 3), dtype=torch.complex64, device="cuda")
     with pytest.raises(Exception, match="unsupported"):
-        mx.fft(x)
+        mx.fft(x, 1 << 25)
+
+
+@pytest.mark.parametrize("n,b", [(32768, 3), (65536, 2), (1 << 18, 2), (1 << 20, 1), (1 << 22, 1)])
+def test_fft_long_sizes(mx, orc, n, b):
+    """Powers of two above one workgroup (two passes through the work
+    matrix) against the FP64 oracle FFT, and the inverse round trip."""
+    rng = np.random.default_rng(n)
+    x = crandn(rng, b, n)
+    X = host(mx.fft(dev(x)))
+    assert rel(X, orc.fft(x)) < RTOL
+    assert rel(host(mx.ifft(dev(X))), x) < RTOL
+
+
+@pytest.mark.parametrize("n,b", [(8193, 3), (10000, 2), (50000, 2), (100003, 1), (1000000, 1)])
+def test_fft_long_bluestein(mx, n, b):
+    """Sizes that are not powers of two above the one-workgroup Bluestein
+    range (two-pass convolution transforms) against numpy's FP64 FFT."""
+    rng = np.random.default_rng(n)
+    x = crandn(rng, b, n)
+    X = host(mx.fft(dev(x)))
+    assert rel(X, np.fft.fft(x.astype(np.complex128))) < RTOL
+    assert rel(host(mx.ifft(dev(X))), x) < RTOL
+
+
+def test_fft_long_pad_norms_strided_inplace_real(mx):
+    """The size rules, norms, views and real forms on the two-pass paths."""
+    rng = np.random.default_rng(4)
+    x = crandn(rng, 2, 30000).astype(np.complex128)
+    for n in (32768, 20000, 65536, 40000):  # pad / truncate, power of two or not
+        for norm in ("backward", "forward", "ortho"):
+            assert rel(host(mx.fft(dev(x), n, norm=norm)), np.fft.fft(x, n, norm=norm)) < RTOL
+            assert rel(host(mx.ifft(dev(x), n, norm=norm)), np.fft.ifft(x, n, norm=norm)) < RTOL
+    y = crandn(rng, 3, 32768, 4).astype(np.complex128)
+    yd = dev(y)
+    assert rel(host(mx.fft(yd, axis=1)), np.fft.fft(y, axis=1)) < RTOL
+    v = yd.permute(0, 2, 1)
+    mx.fft(v, out=v)
+    assert rel(host(yd), np.fft.fft(y, axis=1)) < RTOL
+    r = rng.standard_normal((2, 50000))
+    for n in (65536, 50000):
+        R = host(mx.rfft(dev(r, torch.float32), n))
+        assert rel(R, np.fft.rfft(r, n)) < RTOL
+        assert rel(host(mx.irfft(dev(R), n)), np.fft.irfft(np.fft.rfft(r, n), n)) < RTOL
 
 
 @pytest.mark.parametrize("n", [1, 2, 8, 100, 1000, 1024, 4096, 4097, 5000, 16384])
@@ -191,6 +234,31 @@ def test_dlpack_import_through_the_abi(mx):
         ref = np.fft.fft(host(view).astype(np.complex128), axis=axis)
         assert rel(host(out), ref) < RTOL
         del cap
+
+
+def test_dlpack_export_torch_round_trip(mx):
+    """mxr_tensor_to_dlpack (DLPack export) consumed by torch: the torch
+    tensor aliases the exported view (same memory, shape, strides), writes
+    through either side are seen by the other, and the export is released
+    when torch drops it."""
+    from torch.utils import dlpack as tdl
+    from matx_amd import _lib as L
+    rng = np.random.default_rng(42)
+    base = dev(crandn(rng, 4, 32, 50))
+    for view in (base, base.permute(0, 2, 1), base[:, 3:20, ::2]):
+        n0 = len(L._exports)
+        y = tdl.from_dlpack(L.to_dlpack(view))
+        assert y.data_ptr() == view.data_ptr()
+        assert tuple(y.shape) == tuple(view.shape) and y.stride() == view.stride()
+        assert y.dtype == torch.complex64 and y.device == view.device
+        y.mul_(2)
+        assert torch.equal(y, view)
+        assert len(L._exports) == n0 + 1
+        del y
+        assert len(L._exports) == n0
+    pw = torch.rand((3, 7), device="cuda")
+    y = tdl.from_dlpack(L.to_dlpack(pw))
+    assert torch.equal(y, pw) and y.dtype == torch.float32
 
 
 # --------------------------------------------------------- fft_conv / PC

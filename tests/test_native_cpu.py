@@ -72,13 +72,14 @@ def test_fft_size_checks(L):
     t3 = fake(L, [9], L.C64, addr=0x90000)
     r, _ = status(L, lib.mxr_fft_c2c, ctypes.byref(t3), ctypes.byref(tc), 0, 0, 0, 0, None)
     assert r == 4
-    # sizes past both paths (powers of two to 16384, others to 8192) are
+    # sizes past every path (powers of two to 2^24, others to 2^23) are
     # reported as unsupported before any device work
-    t5 = fake(L, [8193], L.C64, addr=0x90000)
-    r, msg = status(L, lib.mxr_fft_c2c, ctypes.byref(t5), ctypes.byref(tc), 8193, 0, 0, 0, None)
+    n5 = (1 << 23) + 1
+    t5 = fake(L, [n5], L.C64, addr=0x90000)
+    r, msg = status(L, lib.mxr_fft_c2c, ctypes.byref(t5), ctypes.byref(tc), n5, 0, 0, 0, None)
     assert r == 1 and "unsupported" in msg
-    t6 = fake(L, [32768], L.C64, addr=0x90000)
-    r, _ = status(L, lib.mxr_fft_c2c, ctypes.byref(t6), ctypes.byref(tc), 32768, 0, 0, 0, None)
+    t6 = fake(L, [1 << 25], L.C64, addr=0x90000)
+    r, _ = status(L, lib.mxr_fft_c2c, ctypes.byref(t6), ctypes.byref(tc), 1 << 25, 0, 0, 0, None)
     assert r == 1
     # wrong dtype
     tf = fake(L, [16], L.F32, addr=0x90000)
@@ -310,3 +311,53 @@ def test_expression_rejects_complex_operands_of_real_ops(L):
         p, n = prog(*ops)
         r, msg = status(L, lib.mxr_ew_eval, ctypes.byref(out), ins, 1, p, n, consts, 2, None)
         assert r == 6 and "real operand" in msg, (ops, r, msg)
+
+
+def test_dlpack_export_descriptor(L):
+    """mxr_tensor_to_dlpack (tensor_t::ToDlPack, include/matx/core/
+    tensor.h:1471-1555) on a host-built descriptor: the DLManagedTensor
+    carries the view's pointer, shape, element strides, dtype and device;
+    importing it back gives the same descriptor; the deleter runs the
+    producer's release callback exactly once."""
+    lib = L.lib()
+    released = []
+
+    @L.RELEASE_FN
+    def rel(ctx):
+        released.append(ctx)
+
+    for dt, code, bits in ((L.C64, 5, 64), (L.F32, 2, 32), (L.I32, 0, 32)):
+        t = fake(L, [3, 5, 7], dt, addr=0x200000, strides=[70, 1, 10])   # a permuted view
+        out = ctypes.POINTER(L.mxr_dl_managed)()
+        assert lib.mxr_tensor_to_dlpack(ctypes.byref(t), 10, 2, rel, ctypes.c_void_p(77 + dt),
+                                        ctypes.byref(out)) == 0
+        m = out.contents
+        d = m.dl_tensor
+        assert d.data == 0x200000 and d.device_type == 10 and d.device_id == 2 and d.ndim == 3
+        assert (d.dtype_code, d.dtype_bits, d.dtype_lanes) == (code, bits, 1)
+        assert [d.shape[i] for i in range(3)] == [3, 5, 7]
+        assert [d.strides[i] for i in range(3)] == [70, 1, 10]
+        assert d.byte_offset == 0
+        back = L.mxr_tensor()
+        assert lib.mxr_tensor_from_dlpack(ctypes.byref(back), ctypes.byref(d)) == 0
+        assert back.data == t.data and back.dtype == dt and back.rank == 3
+        assert list(back.shape[:3]) == [3, 5, 7] and list(back.stride[:3]) == [70, 1, 10]
+        assert not released
+        m.deleter(out)
+        assert released == [77 + dt]
+        released.clear()
+    # a scalar (rank 0) exports with no shape entries
+    s = fake(L, [], L.F32, addr=0x300000)
+    out = ctypes.POINTER(L.mxr_dl_managed)()
+    assert lib.mxr_tensor_to_dlpack(ctypes.byref(s), 11, 0, rel, None, ctypes.byref(out)) == 0
+    assert out.contents.dl_tensor.ndim == 0 and out.contents.dl_tensor.device_type == 11
+    out.contents.deleter(out)
+    assert released == [None]
+    # rejected: a device type that is not ROCm memory, a bad dtype, null output
+    r, msg = status(L, lib.mxr_tensor_to_dlpack, ctypes.byref(s), 2, 0, rel, None, ctypes.byref(out))
+    assert r == 1 and "device type" in msg
+    bad = fake(L, [4], 9)
+    r, _ = status(L, lib.mxr_tensor_to_dlpack, ctypes.byref(bad), 10, 0, rel, None, ctypes.byref(out))
+    assert r == 6
+    r, _ = status(L, lib.mxr_tensor_to_dlpack, ctypes.byref(s), 10, 0, rel, None, None)
+    assert r == 2
