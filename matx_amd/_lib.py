@@ -22,7 +22,8 @@ PC_WINDOW, PC_NORMALIZE, PC_WRITEBACK = 1, 2, 4
 
 # Every entry point include/mxr.h declares.
 EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_enable",
-           "mxr_profile_reset", "mxr_profile_query", "mxr_tensor_from_dlpack", "mxr_fft_c2c", "mxr_fft_r2c", "mxr_fft_c2r",
+           "mxr_profile_reset", "mxr_profile_query", "mxr_tensor_from_dlpack", "mxr_tensor_to_dlpack",
+           "mxr_fft_c2c", "mxr_fft_r2c", "mxr_fft_c2r",
            "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar", "mxr_doppler_cfar",
            "mxr_conv1d", "mxr_conv1d_fft", "mxr_conv2d", "mxr_ew_eval", "mxr_sum", "mxr_pc_plan_create",
            "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy",
@@ -58,6 +59,17 @@ class mxr_dl_tensor(ctypes.Structure):
                 ("dtype_code", ctypes.c_uint8), ("dtype_bits", ctypes.c_uint8),
                 ("dtype_lanes", ctypes.c_uint16), ("shape", ctypes.POINTER(ctypes.c_int64)),
                 ("strides", ctypes.POINTER(ctypes.c_int64)), ("byte_offset", ctypes.c_uint64)]
+
+
+RELEASE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+
+
+class mxr_dl_managed(ctypes.Structure):
+    pass
+
+
+mxr_dl_managed._fields_ = [("dl_tensor", mxr_dl_tensor), ("manager_ctx", ctypes.c_void_p),
+                           ("deleter", ctypes.CFUNCTYPE(None, ctypes.POINTER(mxr_dl_managed)))]
 
 
 class mxr_cfar_params(ctypes.Structure):
@@ -107,6 +119,8 @@ def lib():
     L.mxr_profile_query.restype = I
     sigs = {
         "mxr_tensor_from_dlpack": [T, V],
+        "mxr_tensor_to_dlpack": [T, ctypes.c_int32, ctypes.c_int32, RELEASE_FN, V,
+                                 ctypes.POINTER(ctypes.POINTER(mxr_dl_managed))],
         "mxr_fft_c2c": [T, T, I64, I, I, I, V],
         "mxr_fft_r2c": [T, T, I64, I, I, V],
         "mxr_fft_c2r": [T, T, I64, I, I, V],
@@ -198,6 +212,38 @@ def desc_from_dlpack(obj):
     d = mxr_tensor()
     check(lib().mxr_tensor_from_dlpack(ctypes.byref(d), ptr))
     return d, cap
+
+
+# Exports whose consumer has not called the deleter yet: export id -> (the
+# producer object kept alive, the release callback object).
+_exports = {}
+_export_seq = [0]
+
+
+@RELEASE_FN
+def _release_export(ctx):
+    _exports.pop(ctx, None)
+
+
+def to_dlpack(t):
+    """A "dltensor" capsule (DLPack export through mxr_tensor_to_dlpack, no
+    copy) of a device tensor; the tensor is kept alive until the consumer
+    calls the capsule's deleter."""
+    d = desc(t)
+    _export_seq[0] += 1
+    key = _export_seq[0]
+    _exports[key] = t
+    out = ctypes.POINTER(mxr_dl_managed)()
+    try:
+        check(lib().mxr_tensor_to_dlpack(ctypes.byref(d), 10, t.device.index or 0, _release_export,
+                                         ctypes.c_void_p(key), ctypes.byref(out)))
+    except Exception:
+        _exports.pop(key, None)
+        raise
+    new = ctypes.pythonapi.PyCapsule_New
+    new.restype = ctypes.py_object
+    new.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+    return new(ctypes.cast(out, ctypes.c_void_p), b"dltensor", None)
 
 
 def stream_handle(stream=None):

@@ -424,4 +424,67 @@ mxr_status mxr_tensor_from_dlpack(mxr_tensor *out, const mxr_dl_tensor *src) {
   });
 }
 
+namespace {
+// One exported view: the DLManagedTensor first, then its arrays and the
+// producer's release callback.
+struct DlExport {
+  mxr_dl_managed m;
+  int64_t shape[MXR_MAX_RANK];
+  int64_t strides[MXR_MAX_RANK];
+  void (*release)(void *);
+  void *release_ctx;
+};
+void dl_export_deleter(mxr_dl_managed *self) {
+  if (!self) return;
+  DlExport *e = reinterpret_cast<DlExport *>(self);
+  void (*rel)(void *) = e->release;
+  void *ctx = e->release_ctx;
+  delete e;
+  if (rel) rel(ctx);
+}
+}  // namespace
+
+mxr_status mxr_tensor_to_dlpack(const mxr_tensor *t, int32_t device_type, int32_t device_id,
+                                void (*release)(void *), void *release_ctx, mxr_dl_managed **out) {
+  return mxr::guarded("mxr_tensor_to_dlpack", [&]() -> mxr_status {
+    MXR_REQUIRE(t != nullptr && out != nullptr, MXR_INVALID_PARAMETER,
+                "tensor_to_dlpack: null argument");
+    MXR_REQUIRE(device_type == 10 || device_type == 11, MXR_NOT_SUPPORTED,
+                "tensor_to_dlpack: device type %d is not ROCm memory (10 device, 11 pinned host)",
+                (int)device_type);
+    mxr_status st = mxr::check_tensor(t, "tensor_to_dlpack", 0, 0, MXR_MAX_RANK);
+    if (st) return st;
+    DlExport *e = new DlExport{};
+    mxr_dl_tensor &d = e->m.dl_tensor;
+    d.data = t->data;
+    d.device_type = device_type;
+    d.device_id = device_id;
+    d.ndim = t->rank;
+    d.dtype_lanes = 1;
+    if (t->dtype == MXR_C64) {
+      d.dtype_code = 5;
+      d.dtype_bits = 64;
+    } else if (t->dtype == MXR_F32) {
+      d.dtype_code = 2;
+      d.dtype_bits = 32;
+    } else {
+      d.dtype_code = 0;
+      d.dtype_bits = 32;
+    }
+    for (int i = 0; i < t->rank; ++i) {
+      e->shape[i] = t->shape[i];
+      e->strides[i] = t->stride[i];
+    }
+    d.shape = e->shape;
+    d.strides = e->strides;
+    d.byte_offset = 0;
+    e->m.manager_ctx = e;
+    e->m.deleter = dl_export_deleter;
+    e->release = release;
+    e->release_ctx = release_ctx;
+    *out = &e->m;
+    return MXR_SUCCESS;
+  });
+}
+
 }  // extern "C"
