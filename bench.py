@@ -1,16 +1,18 @@
 #!/usr/bin/env python3
 """Radar-pipeline throughput on one MI355X.
 
-Workload (one step = one cube): 16 channels x 128 pulses x 9000 samples,
-complex64, resident in HBM; 1000-sample LFM (bandwidth 0.4 fs, Hamming applied
-by the matched filter); noise sigma = 1 plus 20 point targets (seeded range,
-Doppler, SNR 0-20 dB).  Pulse compression (N = 16384 circular correlation,
-lags 0..8000) -> three-pulse canceller [1,-2,1] (126 rows) -> Hamming(126) +
-128-point Doppler FFT -> |.|^2 -> CA-CFAR (13 range x 5 Doppler window, 2 x 1
-guard cells around the cell under test, i.e. a 5 x 3 hole, Pfa 1e-5) ->
-int32 detections.  The waveform is prepared once (pulse-compression plan);
-every cube then runs two kernels: the overlap-save pulse-compression kernel
-and the fused pulse-axis + CFAR kernel.
+Headline workload (one step = one cube): 16 channels x 128 pulses x 9000
+samples, complex64, resident in HBM; 1000-sample LFM (bandwidth 0.4 fs,
+Hamming applied by the matched filter); noise sigma = 1 plus 20 point targets
+(seeded range, Doppler, SNR 0-20 dB).  Pulse compression (N = 16384 circular
+correlation, lags 0..8000) -> three-pulse canceller [1,-2,1] (126 rows) ->
+Hamming(126) + 128-point Doppler FFT -> |.|^2 -> CA-CFAR (13 range x 5
+Doppler window, 2 x 1 guard cells around the cell under test, i.e. a 5 x 3
+hole, Pfa 1e-5) -> int32 detections.  The waveform is prepared once
+(pulse-compression plan); every cube then runs two kernels: the overlap-save
+pulse-compression kernel and the fused pulse-axis + CFAR kernel.  Four
+distinct cubes stay resident and are processed in turn, so no cube's samples
+can be served from the 256 MB last-level cache.
 
 The cube sits in HBM in MatX's inputView layout {16, 128, 16384} (pulses
 padded to the FFT size; only the 9000 samples are read); intermediate rows
@@ -19,10 +21,14 @@ are padded to 64 elements.
 Prints one JSON line: cubes/s from cubes replayed out of a captured hipGraph
 (no instrumentation in that timed region), the dominant kernel's roofline
 from HIP events recorded around every launch of each kernel in a second,
-eager pass over the same cubes, and the FP64 CPU oracle timed on one channel
-of the same cube (also used there as the parity check for that channel).
+eager pass, the CPU baseline (MatX's host dataflow restated on torch CPU with
+MKL FFTs, all host cores) with the FP64 oracle beside it, and -- with the
+default --workload all -- the other terms of the metric as sub-measurements:
+the ragged-batch speedup, the gradient steps/s, fft_conv us per call, the
+64 x 512 x 32768 pipeline with its backward, and MatX's stage calls on the
+C++ front end.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all]
 """
 from __future__ import annotations
 
@@ -30,6 +36,7 @@ import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -44,23 +51,24 @@ def parse():
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
-                    help="wall budget of the CPU baseline leg (0 disables it)")
-    ap.add_argument("--overlap", action="store_true",
-                    help="radar/large: overlap consecutive cubes on two streams (measured slower "
-                         "on MI355X: the co-resident kernels starve each other of LDS)")
+                    help="wall budget of each CPU baseline leg of the headline workload; the "
+                         "sub-measurements of --workload all get a third of it (0 disables them)")
     ap.add_argument("--no-profile", action="store_true",
                     help="time without per-kernel events (roofline omitted)")
-    ap.add_argument("--workload", default="radar",
-                    choices=["radar", "fftconv", "large", "grad", "ragged", "conv", "ew"],
-                    help="radar: the 16x128x9000 pipeline (default); fftconv: fft_conv on "
-                         "4x32x2048 with a 256-sample chirp at N=4096 (us per call); large: "
-                         "the 64x512x32768 pipeline (8.6 GB cube); grad: waveform/window "
-                         "design loss forward + backward (grad steps/s); ragged: 256 uneven "
-                         "pulse-compression + Doppler jobs, batched vs per-job launches; "
-                         "conv: conv1d by the FFT method on 16 x 1048576 c64 with 209 taps "
-                         "(MatX's large ConvCorr size) and 1000-point Bluestein FFTs; ew: fused "
-                         "elementwise expressions at the radar cube's size (calcDets threshold, "
-                         "Doppler window), generated kernels or, with MXR_EW_JIT=0, the interpreter")
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "radar", "fftconv", "large", "grad", "ragged", "stage", "conv",
+                             "ew"],
+                    help="all (default): the 16x128x9000 pipeline as the headline value, with the "
+                         "fft_conv, ragged-batch, gradient, 64x512x32768 and C++ stage-form "
+                         "workloads as sub-measurements of the same line; radar: the "
+                         "16x128x9000 pipeline alone; fftconv: fft_conv on 4x32x2048 with a "
+                         "256-sample chirp at N=4096 (us per call); large: the 64x512x32768 "
+                         "pipeline (8.6 GB cube) and its backward; grad: waveform/window design "
+                         "loss forward + backward (grad steps/s); ragged: 256 uneven "
+                         "pulse-compression + Doppler jobs, batched vs per-job launches; stage: "
+                         "MatX's stage calls on the C++ front end; conv: conv1d by the FFT "
+                         "method on 16 x 1048576 c64 with 209 taps and 1000-point Bluestein "
+                         "FFTs; ew: fused elementwise expressions at the radar cube's size")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 512 MiB device copies (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -190,8 +198,7 @@ def cpu_standin_leg(x0, w, nd, win, guard, budget_s, C):
                       f"{reps} runs in {el:.1f} s"}
 
 
-def load_traffic(kernel, workload="radar"):
-    tag = {"radar": "radar16x128x9000", "large": "radar64x512x32768"}[workload]
+def load_traffic(kernel, tag):
     p = os.path.join(ROOT, "profiles", f"{kernel}_{tag}_traffic.json")
     if os.path.exists(p):
         with open(p) as f:
@@ -321,7 +328,7 @@ def bench_fftconv(args):
     kern = profiled_kernels(lambda: matx_amd.fft_conv(x, w, N, out=out), 50,
                             ["pc4096_wf_kernel", "pc4096_kernel", "pc_ols_kernel", "wfspec_kernel"])
     roof = kernel_roofline(kern, {"pc4096_wf_kernel": B, "pc4096_kernel": B, "pc_ols_kernel": B})
-    print(json.dumps({
+    return ({
         "metric": "fft_conv us per call (4x32x2048 c64, L=256, N=4096)",
         "value": round(best, 3), "unit": "us/call", "n_gpus": 1, "steps": steps,
         "warmup": warm, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
@@ -342,7 +349,7 @@ def bench_fftconv(args):
                          int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0)))),
                          "kind": "port",
                          "sample": f"FP64 oracle fft_conv on the same input, {cpu_reps} calls"},
-    }))
+    })
 
 
 def bench_conv(args):
@@ -409,7 +416,7 @@ def bench_conv(args):
     x2h = x2[:64].cpu().numpy()
     ref2 = orc.fft(x2h)
     rel_fft = float(np.linalg.norm(out2[:64].cpu().numpy() - ref2) / np.linalg.norm(ref2))
-    print(json.dumps({
+    return ({
         "metric": "conv1d FFT method us per call (16 x 1048576 c64, 209 taps, FULL)",
         "value": round(us_conv, 2), "unit": "us/call", "n_gpus": 1, "steps": k,
         "warmup": max(args.warmup, 3), "higher_is_better": False, "scaling": "weak",
@@ -430,7 +437,7 @@ def bench_conv(args):
         "cpu_baseline": {"value": round(cpu_row_s * rows * 1e6, 1), "unit": "us/call",
                          "cores": 1, "kind": "port",
                          "sample": "FP64 oracle direct convolution of row 0, scaled by 16 rows"},
-    }))
+    })
 
 
 def bench_ew(args):
@@ -507,7 +514,7 @@ def bench_ew(args):
     pw, th = np.abs(xs) ** 2, ns * (np.power(1e-5, -1.0 / ns) - 1.0) * bs
     near = np.abs(pw - th) <= 1e-5 * th
     mismatch = int(((got != ref) & ~near).sum())
-    print(json.dumps({
+    return ({
         "metric": "fused elementwise expression GB/s (calcDets threshold, 16x128x8001)",
         "value": round(B_thr / (us_thr * 1e-6) / 1e9, 1), "unit": "GB/s", "n_gpus": 1, "steps": k,
         "warmup": 3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -522,7 +529,7 @@ def bench_ew(args):
                    "mismatches_vs_fp64_channel0": mismatch},
         "roofline": roof,
         "cpu_baseline": None,
-    }))
+    })
 
 
 def bench_grad(args):
@@ -623,7 +630,7 @@ def bench_grad(args):
     gp, gt = p.grad.detach().cpu().numpy(), t.grad.detach().cpu().numpy()
     best = min(eager_step, graph_step)
     fwd_best = min(eager_fwd, graph_fwd)
-    print(json.dumps({
+    return ({
         "metric": "grad steps/s (waveform/window design: 64x1024 phases + 128 taps, "
                   "16x128x4096 Doppler cube)",
         "value": round(1000.0 / best, 2), "unit": "steps/s", "n_gpus": 1, "steps": steps,
@@ -647,7 +654,7 @@ def bench_grad(args):
                          "kind": "port",
                          "sample": f"FP64 numpy oracle loss + analytic gradient (oracle/grad.py), "
                                    f"{cpu_runs} runs"},
-    }))
+    })
 
 
 def stage_dataflow_leg(jobs, shapes, ref_pws, s, reps):
@@ -809,7 +816,7 @@ def bench_ragged(args):
         if time.perf_counter() - t0 >= max(args.cpu_seconds, 0.5):
             break
     cpu_jobs_s = done / (time.perf_counter() - t0)
-    print(json.dumps({
+    return ({
         "metric": "ragged-batch speedup (256 uneven pulse-compression + Doppler jobs, "
                   "hipGraph of the batched launches vs per-job launches)",
         "value": round(t_plain / t_graph, 3), "unit": "x", "n_gpus": 1, "steps": reps,
@@ -843,7 +850,44 @@ def bench_ragged(args):
                          "kind": "port",
                          "sample": f"FP64 oracle pulse compression + Doppler on the first {done} "
                                    "jobs of the batch"},
-    }))
+    })
+
+
+METRIC = ("cubes/s (16x128x9000 c64) & % HBM peak; ragged-batch speedup; grad steps/s")
+
+
+def bench_all(args):
+    """The headline line: the 16 x 128 x 9000 pipeline is `value`; every other
+    workload of the metric runs after it in the same process and lands in
+    config.sub_measurements, each with its own parity check and baseline."""
+    import torch
+    head = bench_pipeline(args, WORKLOADS["radar"])
+    sub_args = argparse.Namespace(**vars(args))
+    sub_args.cpu_seconds = args.cpu_seconds / 3.0
+    sub_args.steps, sub_args.warmup = min(args.steps, 200), min(args.warmup, 20)
+    legs = [("ragged_batch", bench_ragged), ("grad_design", bench_grad),
+            ("fft_conv", bench_fftconv),
+            ("large_pipeline", lambda a: bench_pipeline(a, WORKLOADS["large"])),
+            ("matx_stage_form", bench_stage_form)]
+    subs = {}
+    for name, fn in legs:
+        print(f"bench: {name}", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        try:
+            subs[name] = fn(sub_args)
+        except Exception as e:  # a failing leg is reported, the headline stands
+            subs[name] = {"error": f"{type(e).__name__}: {e}"[:400]}
+        subs[name]["wall_s"] = round(time.perf_counter() - t0, 1)
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    head["metric"] = METRIC
+    cfg = head["config"]
+    rg, gd = subs.get("ragged_batch", {}), subs.get("grad_design", {})
+    cfg["ragged_speedup"] = rg.get("value")
+    cfg["grad_steps_per_s"] = gd.get("value")
+    cfg["grad_fwd_bwd_over_fwd"] = gd.get("config", {}).get("fwd_bwd_over_fwd")
+    cfg["sub_measurements"] = subs
+    return head
 
 
 def main():
@@ -861,36 +905,26 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": a.numel() * 4, "copies": 5}))
         return
-    if args.workload == "fftconv":
-        bench_fftconv(args)
-        return
-    if args.workload == "grad":
-        bench_grad(args)
-        return
-    if args.workload == "ragged":
-        bench_ragged(args)
-        return
-    if args.workload == "conv":
-        bench_conv(args)
-        return
-    if args.workload == "ew":
-        bench_ew(args)
-        return
-    bench_pipeline(args, WORKLOADS[args.workload])
+    runners = {"all": bench_all, "fftconv": bench_fftconv, "grad": bench_grad,
+               "ragged": bench_ragged, "conv": bench_conv, "ew": bench_ew,
+               "stage": bench_stage_form,
+               "radar": lambda a: bench_pipeline(a, WORKLOADS["radar"]),
+               "large": lambda a: bench_pipeline(a, WORKLOADS["large"])}
+    print(json.dumps(runners[args.workload](args)))
 
 
 # (Doppler, range) window and guard sizes.  radar: 13 range x 5 Doppler
 # window with 2 x 1 guard cells around the cell under test; large: 21 x 7.
 WORKLOADS = {
     "radar": dict(C=16, P=128, S=9000, L=1000, nd=128, bw=0.4, win=(5, 13), guard=(3, 5),
-                  targets=20, snr=(0.0, 20.0), steps=None,
+                  targets=20, snr=(0.0, 20.0), steps=None, cubes=4, tag="radar16x128x9000",
                   metric="cubes/s (16x128x9000 c64 radar pipeline)",
                   desc="radar pipeline 16ch x 128 pulses x 9000 samples c64, L=1000 LFM (0.4 fs), "
                        "N=16384 pulse compression, [1,-2,1] canceller (126 rows), Hamming + "
                        "128-pt Doppler FFT, CA-CFAR 13x5 window / 2x1 guard cells (5x3 hole), "
                        "Pfa 1e-5; waveform spectrum prepared once per waveform"),
     "large": dict(C=64, P=512, S=32768, L=4096, nd=512, bw=0.5, win=(7, 21), guard=(3, 5),
-                  targets=200, snr=(0.0, 25.0), steps=20,
+                  targets=200, snr=(0.0, 25.0), steps=20, cubes=1, tag="radar64x512x32768",
                   metric="cubes/s (64x512x32768 c64 radar pipeline)",
                   desc="radar pipeline 64ch x 512 pulses x 32768 samples c64 (8.6 GB), L=4096 "
                        "LFM (0.5 fs), N=32768 pulse compression, [1,-2,1] canceller (510 rows), "
@@ -905,19 +939,24 @@ def bench_pipeline(args, cfg):
     from matx_amd import _lib
 
     dev = "cuda:0"
+    cpu_s = args.cpu_seconds
     C, P, S, L, nd = cfg["C"], cfg["P"], cfg["S"], cfg["L"], cfg["nd"]
     win, guard = cfg["win"], cfg["guard"]
     steps = args.steps if cfg["steps"] is None else min(args.steps, cfg["steps"])
     warmup = args.warmup if cfg["steps"] is None else min(args.warmup, 3)
-    x, w, targets = synth_cube(C, P, S, L, nd, cfg["targets"], cfg["snr"], args.seed, dev,
-                               bw=cfg["bw"])
+    # distinct resident cubes, replayed in turn: together they exceed the
+    # 256 MB last-level cache, so every cube's samples come from HBM
+    ncubes = cfg["cubes"]
+    cubes = [synth_cube(C, P, S, L, nd, cfg["targets"], cfg["snr"], args.seed + i, dev, bw=cfg["bw"])
+             for i in range(ncubes)]
+    xs = [c[0] for c in cubes]
+    x, w, targets = cubes[0]
     fused = nd <= 128
-    overlap = args.overlap
     pipe = matx_amd.FusedRadarPipeline(C, P, S, L, nd=nd, tpc="valid", win=win, guard=guard,
-                                       want_pow=not fused, overlap=overlap)
+                                       want_pow=not fused)
     pipe.set_waveform(w)
-    for _ in range(warmup):
-        pipe.step(x)
+    for i in range(warmup):
+        pipe.step(xs[i % ncubes])
     pipe.sync()
     torch.cuda.synchronize()
 
@@ -926,27 +965,27 @@ def bench_pipeline(args, cfg):
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     # throughput: the cubes replayed from a captured hipGraph (G cubes per
-    # graph), with no per-kernel instrumentation in the timed region
-    G = max(1, min(steps, 50))
+    # graph, cycling through the resident cubes), with no per-kernel
+    # instrumentation in the timed region
+    G = max(ncubes, min(steps, 48) // ncubes * ncubes)
     reps = max(1, steps // G)
-    graph = None
-    if not overlap:
-        gs = torch.cuda.Stream()
-        gs.wait_stream(stream)
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph, stream=gs):
-            for _ in range(G):
-                pipe.step(x)
-        torch.cuda.synchronize()
+    gs = torch.cuda.Stream()
+    gs.wait_stream(stream)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=gs):
+        for i in range(G):
+            pipe.step(xs[i % ncubes])
+    torch.cuda.synchronize()
+    graph.replay()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
         graph.replay()
-        torch.cuda.synchronize()
-        e0.record(stream)
-        for _ in range(reps):
-            graph.replay()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / (reps * G)
-        steps_timed = reps * G
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / (reps * G)
+    steps_timed = reps * G
+    del graph
     # eager loop (instrumented when profiling): per-kernel durations from
     # HIP events recorded around every launch on its own stream
     prof = not args.no_profile
@@ -955,17 +994,14 @@ def bench_pipeline(args, cfg):
         _lib.check(lib.mxr_profile_enable(1))
     torch.cuda.synchronize()
     e0.record(stream)
-    for _ in range(steps):
-        pipe.step(x)
+    for i in range(steps):
+        pipe.step(xs[i % ncubes])
     pipe.sync()
     e1.record(stream)
     torch.cuda.synchronize()
     ms_eager = e0.elapsed_time(e1) / steps
     if prof:
         _lib.check(lib.mxr_profile_enable(0))
-    if graph is None:
-        ms, steps_timed = ms_eager, steps
-    del graph
 
     B = pipe.algorithmic_bytes()
     value = 1000.0 / ms
@@ -996,7 +1032,7 @@ def bench_pipeline(args, cfg):
             ach = b / (kern[k]["avg_us"] * 1e-6) / 1e9
             rooflines[k] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                            "traffic": load_traffic(k, args.workload), "kernel": k,
+                            "traffic": load_traffic(k, cfg["tag"]), "kernel": k,
                             "algorithmic_bytes_per_launch": b,
                             "avg_us": round(kern[k]["avg_us"], 2)}
     dominant = max(rooflines, key=lambda k: kern[k]["avg_us"]) if rooflines else None
@@ -1012,22 +1048,21 @@ def bench_pipeline(args, cfg):
     # count the injected targets that land on a detection at (Doppler, range)
     hit = sum(int(dets[:, dop, rb].any().item()) for rb, dop, _ in targets)
 
-    cpu, parity = None, None
-    if args.cpu_seconds > 0:
-        cpu, parity = cpu_baseline_leg(x[0].cpu().numpy(), w.cpu().numpy(),
-                                       pipe.pw[0].cpu().numpy(), dets[0].cpu().numpy(), nd,
-                                       args.cpu_seconds, C, cfar_mask(win, guard))
+    oracle, parity = None, None
+    if cpu_s > 0:
+        oracle, parity = cpu_baseline_leg(x[0].cpu().numpy(), w.cpu().numpy(),
+                                          pipe.pw[0].cpu().numpy(), dets[0].cpu().numpy(), nd,
+                                          cpu_s, C, cfar_mask(win, guard))
     standin = None
-    if args.cpu_seconds > 0 and args.workload == "radar":
-        standin = cpu_standin_leg(x[0].cpu().numpy(), w.cpu().numpy(), nd, win, guard,
-                                  args.cpu_seconds, C)
+    if cpu_s > 0 and cfg["tag"] == "radar16x128x9000":
+        standin = cpu_standin_leg(x[0].cpu().numpy(), w.cpu().numpy(), nd, win, guard, cpu_s, C)
     fwd_bwd = None
-    if args.workload == "large":
-        del pipe
+    if cfg["tag"] == "radar64x512x32768":
+        del pipe, cubes, xs
         torch.cuda.empty_cache()
         fwd_bwd = bench_large_grad(args, cfg, x, w, targets)
 
-    out = {
+    return {
         "metric": cfg["metric"],
         "value": round(value, 2),
         "unit": "cubes/s",
@@ -1039,28 +1074,49 @@ def bench_pipeline(args, cfg):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64",
-        "data": f"synthetic (noise + {len(targets)} LFM point targets, seeded)",
+        "data": f"synthetic ({ncubes} resident cube(s), each noise + {len(targets)} LFM point "
+                f"targets, seeded)",
         "config": {"workload": cfg["desc"],
                    "channels": C, "pulses": P, "samples": S, "waveform": L, "doppler": nd,
-                   "cube_overlap": "consecutive cubes on two streams, double-buffered compressed "
-                                   "cube" if overlap else "one stream",
+                   "resident_cubes": ncubes,
                    "timing": (f"value: {steps_timed} cubes replayed from a hipGraph of {G} "
-                              f"captured cubes, no instrumentation; eager loop of {steps} cubes "
-                              f"with per-launch HIP events: {round(ms_eager, 5)} ms per cube"
-                              if not overlap else "eager two-stream loop"),
+                              f"captured cubes cycling over {ncubes} resident cube(s), no "
+                              f"instrumentation; eager loop of {steps} cubes with per-launch HIP "
+                              f"events: {round(ms_eager, 5)} ms per cube"),
                    "ms_per_step_eager_instrumented": round(ms_eager, 5),
                    "algorithmic_bytes_per_cube": B,
+                   "pipeline_hbm_gbs": round(B * value / 1e9, 1),
                    "pipeline_hbm_frac": round(B * value / (HBM_PEAK_GBS * 1e9), 4),
                    "per_kernel_us": {k: round(v["avg_us"], 2) for k, v in kern.items() if v["launches"]},
                    "detections": det_cells, "targets_detected": f"{hit}/{len(targets)}",
                    "kernel_rooflines": {k: v for k, v in rooflines.items() if k != dominant},
                    "parity": parity,
-                   **({"cpu_matx_host_standin": standin} if standin else {}),
+                   **({"cpu_fp64_oracle": oracle} if oracle else {}),
                    **({"fwd_bwd": fwd_bwd} if fwd_bwd else {})},
         "roofline": roofline,
-        "cpu_baseline": cpu,
+        "cpu_baseline": standin if standin else oracle,
     }
-    print(json.dumps(out))
+
+
+def bench_stage_form(args):
+    """MatX's stage-by-stage dataflow through the C++ operator API
+    (matx_amd/examples: PulseCompression, ThreePulseCanceller,
+    DopplerProcessing, CFARDetections written with fft / ifft / conv1d /
+    conv2d / expressions, MatX's sizes: N = 16384, 256-point Doppler FFT over
+    all 128 pulses, 13 x 5 CFAR mask) on the 16 x 128 x 9000 cube, run by the
+    compiled example program in a child process."""
+    exe = os.path.join(ROOT, "matx_amd", "_build", "bin", "simple_radar_pipeline")
+    if not os.path.exists(exe):
+        return {"error": "simple_radar_pipeline is not built"}
+    iters = max(20, min(args.steps, 100))
+    r = subprocess.run([exe, str(iters), "--json"], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        return {"error": r.stderr.strip()[-400:]}
+    line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
+    out = json.loads(line)
+    out["what"] = ("MatX's four stage calls on the C++ front end (generic operators, about 20 "
+                   "launches per cube, MatX's buffer sizes), timed with the executor's events")
+    return out
 
 
 def bench_large_grad(args, cfg, x, w, targets):
