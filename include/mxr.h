@@ -443,6 +443,16 @@ mxr_status mxr_ew_eval(const mxr_tensor *out, const mxr_tensor *const *ins, int 
  * include/matx/transforms/reduce.h:646-655, transforms/cub.h:1828-1860). */
 mxr_status mxr_sum(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stream);
 
+/* mxr_sum without any allocation.  No counterpart in MatX: it is for callers
+ * that cannot allocate (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and result as
+ * mxr_sum (bit-identical); the 512 per-workgroup partials (4 KB) live in a
+ * file-scope device array that each GPU holds until the process ends (the
+ * first launch caps its grid at 512 workgroups, so any input size fits).
+ * Every call on one GPU shares that array: calls that overlap in time on
+ * one GPU (e.g. on two streams) race and must be serialised by the caller. */
+mxr_status mxr_sum_noalloc(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stream);
+
 /* --------------------------------------------------------- convolution */
 /* Direct 1-D convolution along the last axis, batched over the leading
  * dims (filter rank 1 is broadcast, as conv1d_impl clones it,

@@ -25,7 +25,8 @@ EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_ena
            "mxr_profile_reset", "mxr_profile_query", "mxr_tensor_from_dlpack", "mxr_tensor_to_dlpack",
            "mxr_fft_c2c", "mxr_fft_r2c", "mxr_fft_c2r",
            "mxr_fft_conv", "mxr_pulse_compress", "mxr_doppler", "mxr_cfar", "mxr_doppler_cfar",
-           "mxr_conv1d", "mxr_conv1d_fft", "mxr_conv2d", "mxr_ew_eval", "mxr_sum", "mxr_pc_plan_create",
+           "mxr_conv1d", "mxr_conv1d_fft", "mxr_conv2d", "mxr_ew_eval", "mxr_sum", "mxr_sum_noalloc",
+           "mxr_pc_plan_create",
            "mxr_pc_plan_set_waveform", "mxr_pc_plan_execute", "mxr_pc_plan_destroy",
            "mxr_fft_conv_adjoint", "mxr_autocorr_adjoint", "mxr_doppler_taps", "mxr_doppler_taps_adjoint",
            "mxr_ragged_plan_create", "mxr_ragged_execute", "mxr_ragged_plan_block_size",
@@ -134,6 +135,7 @@ def lib():
         "mxr_conv2d": [T, T, T, I, V],
         "mxr_ew_eval": [T, ctypes.c_void_p, I, ctypes.c_void_p, I, ctypes.c_void_p, I, V],
         "mxr_sum": [T, T, V],
+        "mxr_sum_noalloc": [T, T, V],
         "mxr_pc_plan_create": [ctypes.POINTER(V), I64, I64, I64, I64, I],
         "mxr_pc_plan_set_waveform": [V, T, V],
         "mxr_pc_plan_execute": [V, T, T, V],

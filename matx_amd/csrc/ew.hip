@@ -611,32 +611,46 @@ mxr_status ew_impl(const mxr_tensor *out, const mxr_tensor *const *ins, int nin,
 }
 
 // ------------------------------------------------------------------ sum
-__global__ __launch_bounds__(256) void sum_kernel(const void *in, int is_c, int rank,
-                                                  const int64_t *shape_stride, int64_t n,
-                                                  float2 *partial, void *out, int out_c,
-                                                  int stage) {
+// Two launches: each of up to SUM_BLOCKS workgroups reduces a grid-strided
+// share of the input in double into one partial (float2), then one
+// workgroup adds the partials in order.  The view's shape and strides
+// travel in the argument block.
+constexpr int SUM_BLOCKS = 512;
+
+struct SumArgs {
+  const void *in;
+  int is_c, rank;
+  int64_t n;
+  int64_t shape[MXR_MAX_RANK], stride[MXR_MAX_RANK];
+  float2 *partial;  // SUM_BLOCKS entries
+  void *out;
+  int out_c;
+  int64_t blocks;   // partials written by stage 0
+};
+
+__global__ __launch_bounds__(256) void sum_kernel(SumArgs a, int stage) {
   __shared__ double sr[256], si[256];
   double ar = 0.0, ai = 0.0;
   if (stage == 0) {
-    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < a.n; e += (int64_t)gridDim.x * 256) {
       int64_t rem = e, off = 0;
-      for (int d = rank - 1; d >= 0; --d) {
-        const int64_t s = shape_stride[d];
-        off += (rem % s) * shape_stride[MXR_MAX_RANK + d];
+      for (int d = a.rank - 1; d >= 0; --d) {
+        const int64_t s = a.shape[d];
+        off += (rem % s) * a.stride[d];
         rem /= s;
       }
-      if (is_c) {
-        const float2 z = reinterpret_cast<const float2 *>(in)[off];
+      if (a.is_c) {
+        const float2 z = reinterpret_cast<const float2 *>(a.in)[off];
         ar += z.x;
         ai += z.y;
       } else {
-        ar += reinterpret_cast<const float *>(in)[off];
+        ar += reinterpret_cast<const float *>(a.in)[off];
       }
     }
   } else {
-    for (int e = threadIdx.x; e < (int)n; e += 256) {
-      ar += partial[e].x;
-      ai += partial[e].y;
+    for (int e = threadIdx.x; e < (int)a.blocks; e += 256) {
+      ar += a.partial[e].x;
+      ai += a.partial[e].y;
     }
   }
   sr[threadIdx.x] = ar;
@@ -651,40 +665,17 @@ __global__ __launch_bounds__(256) void sum_kernel(const void *in, int is_c, int 
   }
   if (threadIdx.x == 0) {
     if (stage == 0) {
-      partial[blockIdx.x] = make_float2((float)sr[0], (float)si[0]);
-    } else if (out_c) {
-      *reinterpret_cast<float2 *>(out) = make_float2((float)sr[0], (float)si[0]);
+      a.partial[blockIdx.x] = make_float2((float)sr[0], (float)si[0]);
+    } else if (a.out_c) {
+      *reinterpret_cast<float2 *>(a.out) = make_float2((float)sr[0], (float)si[0]);
     } else {
-      *reinterpret_cast<float *>(out) = (float)sr[0];
+      *reinterpret_cast<float *>(a.out) = (float)sr[0];
     }
   }
 }
 
-namespace {
-struct SumWs {
-  float2 *partial = nullptr;
-  int64_t *meta = nullptr;  // 1024 shape/stride slots
-  int next = 0;
-};
-std::mutex g_sum_mu;
-std::map<std::pair<int, uintptr_t>, SumWs> g_sum;
-
-void clear_ew() {
-  std::lock_guard<std::mutex> g(g_sum_mu);
-  for (auto &kv : g_sum) {
-    int cur = 0;
-    (void)hipGetDevice(&cur);
-    (void)hipSetDevice(kv.first.first);
-    (void)hipFree(kv.second.partial);
-    (void)hipFree(kv.second.meta);
-    (void)hipSetDevice(cur);
-  }
-  g_sum.clear();
-}
-CacheClearRegistrar g_reg_ew(clear_ew);
-}  // namespace
-
-mxr_status sum_impl(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stream) {
+// Validates a sum call and fills everything but the partials buffer.
+static mxr_status sum_args(const mxr_tensor *out, const mxr_tensor *in, SumArgs *a) {
   mxr_status st;
   if ((st = check_tensor(in, "sum input", 0, 1, MXR_MAX_RANK))) return st;
   if ((st = check_tensor(out, "sum output", 0, 0, MXR_MAX_RANK))) return st;
@@ -692,39 +683,81 @@ mxr_status sum_impl(const mxr_tensor *out, const mxr_tensor *in, hipStream_t str
               "sum: input must be c64 or f32");
   MXR_REQUIRE(out->dtype == in->dtype, MXR_INVALID_TYPE, "sum: output dtype must match input");
   MXR_REQUIRE(numel(*out) == 1, MXR_INVALID_SIZE, "sum: output must hold one element");
-  constexpr int NBLK = 512;
-  const int64_t n = numel(*in);
-  SumWs *ws;
-  int64_t *meta;
-  {
-    std::lock_guard<std::mutex> g(g_sum_mu);
-    ws = &g_sum[{device_id(), (uintptr_t)stream}];
-    if (!ws->partial) {
-      MXR_HIP_TRY(hipMalloc(&ws->partial, sizeof(float2) * NBLK));
-      MXR_HIP_TRY(hipMalloc(&ws->meta, sizeof(int64_t) * 2 * MXR_MAX_RANK * 64));
-    }
-    meta = ws->meta + 2 * MXR_MAX_RANK * ws->next;
-    ws->next = (ws->next + 1) % 64;
-  }
-  int64_t hm[2 * MXR_MAX_RANK] = {0};
+  *a = SumArgs{};
+  a->in = in->data;
+  a->is_c = in->dtype == MXR_C64;
+  a->rank = in->rank;
+  a->n = numel(*in);
   for (int d = 0; d < in->rank; ++d) {
-    hm[d] = in->shape[d];
-    hm[MXR_MAX_RANK + d] = in->stride[d];
+    a->shape[d] = in->shape[d];
+    a->stride[d] = in->stride[d];
   }
-  MXR_HIP_TRY(hipMemcpyAsync(meta, hm, sizeof(hm), hipMemcpyHostToDevice, stream));
-  int64_t blocks = (n + 255) / 256;
-  if (blocks > NBLK) blocks = NBLK;
+  a->out = out->data;
+  a->out_c = out->dtype == MXR_C64;
+  int64_t blocks = (a->n + 255) / 256;
+  if (blocks > SUM_BLOCKS) blocks = SUM_BLOCKS;
   if (blocks < 1) blocks = 1;
-  {
-    ProfScope ps("sum_kernel", stream);
-    sum_kernel<<<dim3((unsigned)blocks), dim3(256), 0, stream>>>(in->data, in->dtype == MXR_C64,
-                                                                  in->rank, meta, n, ws->partial,
-                                                                  nullptr, 0, 0);
-    sum_kernel<<<dim3(1), dim3(256), 0, stream>>>(nullptr, 0, 0, nullptr, blocks, ws->partial,
-                                                  out->data, out->dtype == MXR_C64, 1);
-  }
+  a->blocks = blocks;
+  return MXR_SUCCESS;
+}
+
+static mxr_status sum_launch(const SumArgs &a, hipStream_t stream) {
+  ProfScope ps("sum_kernel", stream);
+  sum_kernel<<<dim3((unsigned)a.blocks), dim3(256), 0, stream>>>(a, 0);
+  sum_kernel<<<dim3(1), dim3(256), 0, stream>>>(a, 1);
   MXR_HIP_TRY(hipGetLastError());
   return MXR_SUCCESS;
+}
+
+mxr_status sum_impl(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stream) {
+  SumArgs a;
+  mxr_status st = sum_args(out, in, &a);
+  if (st) return st;
+  a.partial = (float2 *)stream_workspace("sum", sizeof(float2) * SUM_BLOCKS, stream);
+  if (!a.partial) return workspace_failure("sum");
+  return sum_launch(a, stream);
+}
+
+// ----------------------------------------------- sum without allocation
+// mxr_sum_noalloc keeps its partials in this file-scope array: one copy per
+// GPU, resident until the process ends, shared by every call on that GPU.
+// Stage 0 writes slots [0, blocks) and stage 1 reads only those, so nothing
+// has to start at zero.
+static __device__ __align__(256) float2 scratch[SUM_BLOCKS];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float2 *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};  // per device: the address of `scratch`
+}  // namespace
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the host-side name is not a device address).
+static mxr_status scratch_address(float2 **p) {
+  const int dev = device_id();
+  MXR_REQUIRE(dev >= 0 && dev < SCRATCH_MAX_DEVICES, MXR_NOT_SUPPORTED,
+              "sum_noalloc: device %d beyond the %d devices served", dev, SCRATCH_MAX_DEVICES);
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *q = nullptr;
+    MXR_HIP_TRY(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<float2 *>(q);
+  }
+  *p = g_scratch_ptr[dev];
+  return MXR_SUCCESS;
+}
+
+mxr_status sum_noalloc_impl(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stream) {
+  SumArgs a;
+  mxr_status st = sum_args(out, in, &a);
+  if (st) return st;
+  // the first launch's grid is capped at the scratch's SUM_BLOCKS slots, so
+  // every input size fits; the check guards that invariant
+  MXR_REQUIRE(a.blocks >= 1 && a.blocks <= SUM_BLOCKS, MXR_INVALID_SIZE,
+              "sum_noalloc: %lld partials exceed the %d-slot scratch", (long long)a.blocks,
+              SUM_BLOCKS);
+  if ((st = scratch_address(&a.partial))) return st;
+  return sum_launch(a, stream);
 }
 
 }  // namespace mxr
@@ -741,6 +774,10 @@ mxr_status mxr_ew_eval(const mxr_tensor *out, const mxr_tensor *const *ins, int 
 
 mxr_status mxr_sum(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stream) {
   return mxr::guarded("mxr_sum", [&] { return mxr::sum_impl(out, in, stream); });
+}
+
+mxr_status mxr_sum_noalloc(const mxr_tensor *out, const mxr_tensor *in, hipStream_t stream) {
+  return mxr::guarded("mxr_sum_noalloc", [&] { return mxr::sum_noalloc_impl(out, in, stream); });
 }
 
 }  // extern "C"
