@@ -89,30 +89,68 @@ __device__ __forceinline__ void dft8(float2 (&a)[8]) {
   for (int i = 0; i < 8; ++i) a[i] = y[i];
 }
 
+// a * W_16^e for e in {2, 6} equals H * hrot(a), H = sqrt(1/2), with hrot
+// made of additions only (W_16^2 = H (1 - i), W_16^6 = H (-1 - i) forward;
+// the inverse conjugates).
+template <int E, bool INV>
+__device__ __forceinline__ float2 hrot(float2 a) {
+  static_assert(E == 2 || E == 6, "hrot: e in {2, 6}");
+  if constexpr (E == 2)
+    return INV ? make_float2(a.x - a.y, a.x + a.y) : make_float2(a.x + a.y, a.y - a.x);
+  else
+    return INV ? make_float2(-a.x - a.y, a.x - a.y) : make_float2(a.y - a.x, -a.x - a.y);
+}
+// p = u + H r, m = u - H r, as fused multiply-adds
+__device__ __forceinline__ void hpair(float2 u, float2 r, float2 &p, float2 &m) {
+  constexpr float H = 0.70710678118654752440f;
+  p = make_float2(fmaf(H, r.x, u.x), fmaf(H, r.y, u.y));
+  m = make_float2(fmaf(-H, r.x, u.x), fmaf(-H, r.y, u.y));
+}
+
 template <bool INV>
 __device__ __forceinline__ void dft16(float2 (&a)[16]) {
-  // 16 = 4 x 4: inner DFT4 over m2 (stride 4), twiddle W16^{m1 c}, DFT4 over m1.
+  // 16 = 4 x 4: inner DFT4 over m2 (stride 4), twiddle W16^{m1 c}, DFT4 over
+  // m1.  Z[m1][c] sits in a[m1 + 4c]; column c's outputs go to y[c + 4r].
+  // The W16^2 / W16^6 twiddles are folded into the column butterflies as
+  // H-scaled fused multiply-adds.
 #pragma unroll
   for (int m1 = 0; m1 < 4; ++m1) dft4<INV>(a[m1], a[m1 + 4], a[m1 + 8], a[m1 + 12]);
-  // Z[m1][c] sits in a[m1 + 4c]
-  a[5] = mul_w16<1, INV>(a[5]);
-  a[9] = mul_w16<2, INV>(a[9]);
-  a[13] = mul_w16<3, INV>(a[13]);
-  a[6] = mul_w16<2, INV>(a[6]);
-  a[10] = mul_w16<4, INV>(a[10]);
-  a[14] = mul_w16<6, INV>(a[14]);
-  a[7] = mul_w16<3, INV>(a[7]);
-  a[11] = mul_w16<6, INV>(a[11]);
-  a[15] = mul_w16<9, INV>(a[15]);
   float2 y[16];
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    float2 z0 = a[4 * c + 0], z1 = a[4 * c + 1], z2 = a[4 * c + 2], z3 = a[4 * c + 3];
+  {  // c = 0: no twiddles
+    float2 z0 = a[0], z1 = a[1], z2 = a[2], z3 = a[3];
     dft4<INV>(z0, z1, z2, z3);
-    y[c] = z0;
-    y[c + 4] = z1;
-    y[c + 8] = z2;
-    y[c + 12] = z3;
+    y[0] = z0;
+    y[4] = z1;
+    y[8] = z2;
+    y[12] = z3;
+  }
+  {  // c = 1: W^1, W^2, W^3
+    const float2 z1 = mul_w16<1, INV>(a[5]), z3 = mul_w16<3, INV>(a[7]);
+    float2 t0, t1;
+    hpair(a[4], hrot<2, INV>(a[6]), t0, t1);
+    const float2 t2 = cadd(z1, z3), t3 = mul_w4<INV>(csub(z1, z3));
+    y[1] = cadd(t0, t2);
+    y[9] = csub(t0, t2);
+    y[5] = cadd(t1, t3);
+    y[13] = csub(t1, t3);
+  }
+  {  // c = 2: W^2, W^4, W^6 = W^2 W^4
+    const float2 z2 = mul_w4<INV>(a[10]);
+    const float2 t0 = cadd(a[8], z2), t1 = csub(a[8], z2);
+    const float2 q = mul_w4<INV>(a[11]);
+    const float2 ru = hrot<2, INV>(cadd(a[9], q)), rv = hrot<2, INV>(csub(a[9], q));
+    hpair(t0, ru, y[2], y[10]);
+    hpair(t1, mul_w4<INV>(rv), y[6], y[14]);
+  }
+  {  // c = 3: W^3, W^6, W^9
+    const float2 z1 = mul_w16<3, INV>(a[13]), z3 = mul_w16<9, INV>(a[15]);
+    float2 t0, t1;
+    hpair(a[12], hrot<6, INV>(a[14]), t0, t1);
+    const float2 t2 = cadd(z1, z3), t3 = mul_w4<INV>(csub(z1, z3));
+    y[3] = cadd(t0, t2);
+    y[11] = csub(t0, t2);
+    y[7] = cadd(t1, t3);
+    y[15] = csub(t1, t3);
   }
 #pragma unroll
   for (int i = 0; i < 16; ++i) a[i] = y[i];
@@ -240,6 +278,33 @@ __device__ __forceinline__ void tw_from_base(float2 (&v)[16], float4 b0, float4 
 // Ls of an N-point transform: row k (k < Ls) holds W^k, W^2k, W^4k, W^8k with
 // W = e^{-2 pi i / (16 Ls)}, each from double precision rounded once.
 const float4 *pow_base_device(int N, int Ls);
+
+// Raw buffer access (MUBUF) for the block kernels: a resource covering
+// `bytes` bytes from `base`.  The hardware range check returns zeros for
+// loads at offsets at or past `bytes` and drops such stores, so a block that
+// runs past the end of its row needs no per-element tests.  Offsets are
+// byte offsets in the vector operand (the scalar offset stays 0: the range
+// check does not cover it).
+// The base and size must be uniform across the workgroup (they are read from
+// the first lane, so the resource lives in scalar registers).
+using buf_rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ buf_rsrc make_rsrc(const void *base, int bytes) {
+  const uint64_t b = reinterpret_cast<uint64_t>(base);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+  void *ub = reinterpret_cast<void *>(((uint64_t)hi << 32) | lo);
+  return __builtin_amdgcn_make_buffer_rsrc(ub, (short)0, __builtin_amdgcn_readfirstlane(bytes), 0x00020000);
+}
+__device__ __forceinline__ float2 buf_load2(buf_rsrc r, int voff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+}
+__device__ __forceinline__ float4 buf_load4(buf_rsrc r, int voff) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0));
+}
+__device__ __forceinline__ void buf_store2(buf_rsrc r, int voff, float2 v) {
+  using u2 = __attribute__((ext_vector_type(2))) unsigned;
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, 0, 0);
+}
 
 // Workgroup barrier that is also a compiler fence: no load or store is moved
 // across it, so twiddle and spectrum loads stay inside their pass instead of

@@ -244,17 +244,23 @@ void pc_ols_kernel(PcArgs a) {
 // the inverse passes reuse the forward rows conjugated.
 struct Pc4096Args {
   PcArgs a;
-  const float4 *base1;  // [16][2] float4: pass Ls = 16
-  const float4 *base2;  // [256][2] float4: pass Ls = 256
+  const float4 *base1;  // [16][2] float4: power-base rows of pass Ls = 16
+  const float4 *base2;  // [256][2] float4: power-base rows of pass Ls = 256
 };
 
 
 // One M = 4096 overlap-save block (256 threads); lds holds 4096 padded
 // elements.  Pre: the workgroup's previous use of lds is finished.
-// (The base rows are deliberately not __restrict__: re-fetching them for the
-// inverse passes is cheaper than keeping them live across the passes.)
-// spec(i) gives spectrum bin i (the block's 1/M included); thread j reads
-// bins j + 256 q only.
+// Thread j owns butterfly j of every radix-16 pass: its twiddles of the
+// Ls = 16 and Ls = 256 passes come from power-base rows (j & 15) and j
+// (W^k, W^2k, W^4k, W^8k), fetched before each LDS exchange so their
+// latency hides behind it (the inverse passes re-fetch them from cache and
+// conjugate; keeping them live costs more registers than the reload).  A block that
+// does not wrap reads x through a raw buffer resource ending at the row's
+// last sample (the hardware returns zeros past it) and writes its kept lags
+// through one ending at its last kept lag (stores past it are dropped), so
+// the block needs no per-element tests.  spec(i) gives spectrum bin i (the
+// block's 1/M included); thread j reads bins j + 256 q only.
 template <class Spec>
 __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *base1,
                                               const float4 *base2, float2 *lds, Spec spec) {
@@ -262,31 +268,39 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
   using LY = Layout<4096, 1, false>;
   const int j = threadIdx.x;
   const float2 *__restrict__ xr = ob.xr;
-  float2 *__restrict__ yr = ob.yr;
   const int lo = ob.lo, n32 = ob.n, seff32 = ob.seff;
 
   float2 v[16];
-  // Stockham exchange: write this pass's outputs, read the next pass's inputs
+  // Stockham exchange: write this pass's outputs, read the next pass's
+  // inputs.  In the padded layout (one spare slot per 16) the outputs
+  // base + q ls_out of butterfly j sit at pad(base) + q (ls_out = 1) or
+  // pad(base) + 17 q (ls_out = 16), and input j + 256 m at pad(j) + 272 m:
+  // one address per phase, the rest are instruction offsets.  (An unpadded
+  // 32 KiB layout with an exclusive-or swizzle admits a fifth workgroup per
+  // CU and measured 2 % slower.)
+  const int rbase = lds_pad(j);
   auto xchg = [&](int ls_out) {
     const int k = j & (ls_out - 1), base = (j - k) * 16 + k;
+    const int wbase = lds_pad(base), wstep = ls_out == 1 ? 1 : 17;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) lds[LY::addr(0, base + q * ls_out)] = v[q];
+    for (int q = 0; q < 16; ++q) lds[wbase + q * wstep] = v[q];
     wg_barrier();
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = lds[LY::addr(0, j + m * NT)];
+    for (int m = 0; m < 16; ++m) v[m] = lds[rbase + m * (NT + NT / 16)];
     wg_barrier();
   };
   const float4 *b1p = base1 + (j & 15) * 2;
   const float4 *b2p = base2 + j * 2;
+  float4 t0, t1;
 
-  // forward pass 0 (Ls = 1) straight from HBM (zero padding and wrap
-  // implicit); a block inside the row (the common case) loads without
-  // per-element checks
-  if (lo + 4096 <= seff32 && lo + 4096 <= n32) {
-    const float2 *xb = xr + lo + j;
+  // forward pass 0 (Ls = 1) straight from HBM
+  if (lo + 4096 <= n32) {
+    int avail = seff32 - lo;
+    avail = avail < 0 ? 0 : (avail > 4096 ? 4096 : avail);
+    const buf_rsrc rx = make_rsrc(xr + lo, avail * 8);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = xb[m * NT];
-  } else {
+    for (int m = 0; m < 16; ++m) v[m] = buf_load2(rx, (j + m * NT) * 8);
+  } else {  // the block wraps around the circular period n (full fft_conv)
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
       int gi = lo + j + m * NT;
@@ -295,7 +309,8 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
     }
   }
   dft16<false>(v);
-  float4 t0 = b1p[0], t1 = b1p[1];
+  t0 = b1p[0];
+  t1 = b1p[1];
   xchg(1);
   tw_from_base(v, t0, t1, false);  // forward pass 1 (Ls = 16)
   dft16<false>(v);
@@ -306,7 +321,7 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
   dft16<false>(v);
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], spec(j + q * 256));
-  dft16<true>(v);                   // inverse pass 0 (Ls = 1)
+  dft16<true>(v);            // inverse pass 0 (Ls = 1)
   t0 = b1p[0];
   t1 = b1p[1];
   xchg(1);
@@ -317,15 +332,11 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
   xchg(16);
   tw_from_base(v, t0, t1, true);   // inverse pass 2 (Ls = 256)
   dft16<true>(v);
-  // lag j + 256q is kept when below lim: whole rows q < lim/256 are a
-  // uniform test, only row lim/256 compares per lane
+  // lags lo + j + 256 q below lo + lim are kept (the spectrum carries the 1/M)
   const int lim = (ob.nout - lo) < ob.V ? (ob.nout - lo) : ob.V;
-  const int qn = lim >> 8, rem = lim & 255;
-  float2 *yb = yr + lo + j;
+  const buf_rsrc ry = make_rsrc(ob.yr + lo, lim * 8);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    if (q < qn || (q == qn && j < rem)) yb[q * 256] = v[q];  // the spectrum carries the 1/M
-  }
+  for (int q = 0; q < 16; ++q) buf_store2(ry, (j + q * 256) * 8, v[q]);
 }
 
 // The block's spectrum from its buffer (ob.wc).

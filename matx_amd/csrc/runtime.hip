@@ -222,8 +222,8 @@ const float4 *pow_base_device(int N, int Ls) {
 
 // Per (device, stream, tag) scratch buffers that only grow: partial sums of
 // reductions that a launch on that stream writes and a later launch on the
-// same stream consumes.  Growing frees the old buffer (hipFree waits for the
-// device), so work already queued never sees a freed buffer.
+// same stream consumes.  A buffer that has to grow is replaced and the old
+// one retired until mxr_clear_caches (see stream_workspace).
 namespace {
 std::mutex g_ws_mu;
 struct WsBuf {
