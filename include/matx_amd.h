@@ -110,6 +110,15 @@ inline void check(mxr_status s) {
 inline void hip_check(hipError_t e, const char *what) {
   if (e != hipSuccess) throw matxException(MXR_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
 }
+// Whether managed memory lives where the current device computes on it
+// (pages migrate on demand); otherwise hipMallocManaged falls back to host
+// memory.
+inline bool managed_memory_migrates() {
+  int dev = 0, v = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeConcurrentManagedAccess, dev) != hipSuccess) return false;
+  return v != 0;
+}
 template <typename T>
 struct dtype_of;
 template <>
@@ -178,8 +187,13 @@ class tensor_t {
   static constexpr int Rank() { return RANK; }
 
   tensor_t() = default;
-  // Allocates a contiguous tensor (managed memory by default, as MatX's
-  // make_tensor, include/matx/core/make_tensor.h:57-58).
+  // Allocates a contiguous tensor, in managed memory by default as MatX's
+  // make_tensor does (include/matx/core/make_tensor.h:57-58).  Where the
+  // device cannot migrate managed pages (no concurrent managed access, as
+  // on an MI355X without XNACK), hipMallocManaged returns host memory that
+  // kernels reach over the host link at under 100 GB/s; there the managed
+  // default allocates device memory instead, and host code reads elements
+  // through copies (host element access throws).
   explicit tensor_t(const std::array<index_t, RANK> &shape, matxMemorySpace_t space = MATX_MANAGED_MEMORY) {
     shape_ = shape;
     index_t s = 1;
@@ -189,6 +203,7 @@ class tensor_t {
     }
     const size_t bytes = sizeof(T) * (size_t)(s > 0 ? s : 1);
     void *p = nullptr;
+    if (space == MATX_MANAGED_MEMORY && !detail::managed_memory_migrates()) space = MATX_DEVICE_MEMORY;
     if (space == MATX_MANAGED_MEMORY) detail::hip_check(hipMallocManaged(&p, bytes), "hipMallocManaged");
     else if (space == MATX_DEVICE_MEMORY) detail::hip_check(hipMalloc(&p, bytes), "hipMalloc");
     else detail::hip_check(hipHostMalloc(&p, bytes), "hipHostMalloc");
@@ -230,6 +245,10 @@ class tensor_t {
   template <typename... I>
   MATX_AMD_HD T &operator()(I... idx) const {
     static_assert(sizeof...(I) == RANK, "index count");
+#if !defined(__HIP_DEVICE_COMPILE__)
+    if (space_ == MATX_DEVICE_MEMORY)
+      throw matxException(MXR_INVALID_PARAMETER, "host access to an element of a tensor in device memory");
+#endif
     const index_t ix[] = {(index_t)idx..., 0};
     index_t off = 0;
     for (int d = 0; d < RANK; ++d) off += ix[d] * stride_[d];

@@ -32,9 +32,11 @@ static std::vector<T> read_file(const std::string &path, size_t n) {
 template <typename T, int R>
 static void dump(const std::string &path, const tensor_t<T, R> &t, HIPExecutor &ex) {
   ex.sync();
-  // the tensors here are contiguous managed allocations
+  // the tensors here are contiguous (device or managed memory)
+  std::vector<T> h((size_t)t.TotalSize());
+  detail::hip_check(hipMemcpy(h.data(), t.Data(), sizeof(T) * h.size(), hipMemcpyDefault), "copy");
   FILE *f = fopen(path.c_str(), "wb");
-  fwrite(t.Data(), sizeof(T), (size_t)t.TotalSize(), f);
+  fwrite(h.data(), sizeof(T), h.size(), f);
   fclose(f);
 }
 
