@@ -94,6 +94,126 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a) {
   else reinterpret_cast<float *>(a.out)[obase] = acc.x;
 }
 
+// ------------------------------------------------ tiled real 2-D (CFAR)
+// Real map, real filter (the CA-CFAR window sum conv2d(xPow, mask), and the
+// cell counts conv2d(ones, mask)): a workgroup computes a 32 x 64 output
+// tile of one batch index.  It stages the input rows oy0 - dy ..
+// oy0 + 31 + fh - 1 - dy and columns ox0 - dx .. ox0 + 63 + fw - 1 - dx
+// (zero outside the map) in LDS once; thread (tx, ty) then accumulates the
+// 8 outputs of rows 8 ty .. 8 ty + 7 of column tx, reading each tap's input
+// from LDS and the tap weight (uniform) through the scalar cache.
+constexpr int C2_TH = 32, C2_TW = 64, C2_FHMAX = 16, C2_FWMAX = 64;
+constexpr int C2_LW = C2_TW + C2_FWMAX;  // staged row pitch (floats)
+
+struct Conv2TileArgs {
+  const float *in;
+  float *out;
+  const float *f;      // filter, flipped on read: f[(fh-1-fy) fs0 + (fw-1-fx) fs1]
+  int64_t fs0, fs1;
+  int fh, fw, dy, dx;
+  int H, W, Ho, Wo;
+  int64_t is0, is1, os0, os1;  // strides of the conv dims
+  int nb;                      // batch dims (rank - 2)
+  int64_t bshape[CONV_MAX_RANK], bis[CONV_MAX_RANK], bos[CONV_MAX_RANK];
+  int tiles_x;
+};
+
+__global__ __launch_bounds__(256) void conv2d_tile_kernel(Conv2TileArgs a) {
+  __shared__ float tile[(C2_TH + C2_FHMAX) * C2_LW];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int ox0 = (blockIdx.x % a.tiles_x) * C2_TW;
+  const int oy0 = (blockIdx.x / a.tiles_x) * C2_TH;
+  int64_t ib = 0, ob = 0, rem = blockIdx.y;
+  for (int d = a.nb - 1; d >= 0; --d) {
+    const int64_t i = rem % a.bshape[d];
+    rem /= a.bshape[d];
+    ib += i * a.bis[d];
+    ob += i * a.bos[d];
+  }
+  const int rows = C2_TH + a.fh - 1, cols = C2_TW + a.fw - 1;
+  const int iy0 = oy0 - a.dy, ix0 = ox0 - a.dx;
+  for (int e = threadIdx.x; e < rows * C2_LW; e += 256) {
+    const int r = e / C2_LW, c = e % C2_LW;
+    const int iy = iy0 + r, ix = ix0 + c;
+    float v = 0.f;
+    if (c < cols && iy >= 0 && iy < a.H && ix >= 0 && ix < a.W) v = a.in[ib + iy * a.is0 + ix * a.is1];
+    tile[e] = v;
+  }
+  __syncthreads();
+  float acc[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) acc[r] = 0.f;
+  const float *tb = tile + (8 * ty) * C2_LW + tx;
+  for (int fy = 0; fy < a.fh; ++fy) {
+    for (int fx = 0; fx < a.fw; ++fx) {
+      const float w = a.f[(a.fh - 1 - fy) * a.fs0 + (a.fw - 1 - fx) * a.fs1];
+      const float *t = tb + fy * C2_LW + fx;
+#pragma unroll
+      for (int r = 0; r < 8; ++r) acc[r] = fmaf(t[r * C2_LW], w, acc[r]);
+    }
+  }
+  const int ox = ox0 + tx;
+  if (ox >= a.Wo) return;
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int oy = oy0 + 8 * ty + r;
+    if (oy < a.Ho) a.out[ob + oy * a.os0 + ox * a.os1] = acc[r];
+  }
+}
+
+// --------------------------------------------- 1-D along a strided axis
+// conv1d along the last view dim when that dim is strided and a batch dim
+// has unit stride in both views (the three-pulse canceller on permuted
+// views, conv1d(inputView.Permute({0,2,1}), [1,-2,1], SAME)): lanes run
+// along the unit-stride batch dim, so every load and store is a
+// contiguous run; each thread computes a strip of 16 outputs along the conv
+// axis (its taps' inputs are re-read from the L1).  The remaining batch
+// dims come from blockIdx.z.
+constexpr int C1_STRIP = 16;
+
+struct Conv1StripArgs {
+  const void *in;
+  void *out;
+  const void *f;
+  int in_c, out_c, f_c;
+  int64_t fl, fs, off;     // taps, filter stride, start offset (index rule of conv1d)
+  int64_t n_in, n_out;     // conv-axis lengths
+  int64_t is_c, os_c;      // conv-axis strides
+  int64_t n_lane;          // extent of the unit-stride batch dim
+  int nb;                  // other batch dims
+  int64_t bshape[CONV_MAX_RANK], bis[CONV_MAX_RANK], bos[CONV_MAX_RANK];
+};
+
+__global__ __launch_bounds__(256) void conv1d_strip_kernel(Conv1StripArgs a) {
+  const int64_t l = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t o0 = (int64_t)blockIdx.y * C1_STRIP;
+  int64_t ib = 0, ob = 0, rem = blockIdx.z;
+  for (int d = a.nb - 1; d >= 0; --d) {
+    const int64_t i = rem % a.bshape[d];
+    rem /= a.bshape[d];
+    ib += i * a.bis[d];
+    ob += i * a.bos[d];
+  }
+  if (l >= a.n_lane) return;
+  ib += l;
+  ob += l;
+  for (int k = 0; k < C1_STRIP; ++k) {
+    const int64_t o = o0 + k;
+    if (o >= a.n_out) break;
+    float2 acc = make_float2(0.f, 0.f);
+    const int64_t full = o + a.off;
+    for (int64_t t = 0; t < a.fl; ++t) {
+      const int64_t si = full - t;
+      if (si < 0 || si >= a.n_in) continue;
+      const float2 x = ld_c(a.in, ib + si * a.is_c, a.in_c);
+      const float2 w = ld_c(a.f, t * a.fs, a.f_c);
+      acc = make_float2(acc.x + (x.x * w.x - x.y * w.y), acc.y + (x.x * w.y + x.y * w.x));
+    }
+    if (a.out_c) reinterpret_cast<float2 *>(a.out)[ob + o * a.os_c] = acc;
+    else reinterpret_cast<float *>(a.out)[ob + o * a.os_c] = acc.x;
+  }
+}
+
 static mxr_status conv_common(const char *fn, const mxr_tensor *out, const mxr_tensor *in,
                               const mxr_tensor *filt, int mode, bool two_d, hipStream_t stream) {
   mxr_status st;
@@ -171,6 +291,85 @@ static mxr_status conv_common(const char *fn, const mxr_tensor *out, const mxr_t
   });
   for (int d = 0; d < r; ++d) a.order[d] = ord[d];
   if (a.total == 0) return MXR_SUCCESS;
+  // real 2-D maps with a window that fits the staged tile
+  if (two_d && !a.in_c && !a.f_c && !a.out_c && a.fl0 <= C2_FHMAX && a.fl1 <= C2_FWMAX &&
+      out->shape[r - 2] < (1 << 30) && out->shape[r - 1] < (1 << 30) &&
+      in->shape[r - 2] < (1 << 30) && in->shape[r - 1] < (1 << 30)) {
+    Conv2TileArgs t{};
+    t.in = (const float *)in->data;
+    t.out = (float *)out->data;
+    t.f = (const float *)filt->data;
+    t.fs0 = filt->stride[0];
+    t.fs1 = filt->stride[1];
+    t.fh = (int)a.fl0;
+    t.fw = (int)a.fl1;
+    t.dy = (int)a.off0;
+    t.dx = (int)a.off1;
+    t.H = (int)in->shape[r - 2];
+    t.W = (int)in->shape[r - 1];
+    t.Ho = (int)out->shape[r - 2];
+    t.Wo = (int)out->shape[r - 1];
+    t.is0 = in->stride[r - 2];
+    t.is1 = in->stride[r - 1];
+    t.os0 = out->stride[r - 2];
+    t.os1 = out->stride[r - 1];
+    t.nb = r - 2;
+    int64_t nbatch = 1;
+    for (int d = 0; d < r - 2; ++d) {
+      t.bshape[d] = out->shape[d];
+      t.bis[d] = in->stride[d];
+      t.bos[d] = out->stride[d];
+      nbatch *= out->shape[d];
+    }
+    t.tiles_x = (t.Wo + C2_TW - 1) / C2_TW;
+    const int64_t tiles = (int64_t)t.tiles_x * ((t.Ho + C2_TH - 1) / C2_TH);
+    MXR_REQUIRE(tiles <= 0x7fffffff && nbatch <= 65535, MXR_INVALID_SIZE, "%s: too large", fn);
+    ProfScope ps("conv2d_tile_kernel", stream);
+    conv2d_tile_kernel<<<dim3((unsigned)tiles, (unsigned)nbatch), dim3(256), 0, stream>>>(t);
+    MXR_HIP_TRY(hipGetLastError());
+    return MXR_SUCCESS;
+  }
+  // 1-D along a strided axis with a unit-stride batch dim
+  if (!two_d && in->stride[r - 1] != 1 && out->stride[r - 1] != 1) {
+    int lane = -1;
+    for (int d = 0; d < r - 1; ++d)
+      if (in->stride[d] == 1 && out->stride[d] == 1 && out->shape[d] > 1) lane = d;
+    int64_t nbatch = 1;
+    for (int d = 0; d < r - 1; ++d)
+      if (d != lane) nbatch *= out->shape[d];
+    const int64_t strips = (out->shape[r - 1] + C1_STRIP - 1) / C1_STRIP;
+    if (lane >= 0 && nbatch <= 65535 && strips <= 65535) {
+      Conv1StripArgs c{};
+      c.in = in->data;
+      c.out = out->data;
+      c.f = filt->data;
+      c.in_c = a.in_c;
+      c.out_c = a.out_c;
+      c.f_c = a.f_c;
+      c.fl = a.fl1;
+      c.fs = a.fs1;
+      c.off = a.off1;
+      c.n_in = in->shape[r - 1];
+      c.n_out = out->shape[r - 1];
+      c.is_c = in->stride[r - 1];
+      c.os_c = out->stride[r - 1];
+      c.n_lane = out->shape[lane];
+      for (int d = 0; d < r - 1; ++d) {
+        if (d == lane) continue;
+        c.bshape[c.nb] = out->shape[d];
+        c.bis[c.nb] = in->stride[d];
+        c.bos[c.nb] = out->stride[d];
+        ++c.nb;
+      }
+      const int64_t lanes = (c.n_lane + 255) / 256;
+      MXR_REQUIRE(lanes <= 0x7fffffff, MXR_INVALID_SIZE, "%s: too large", fn);
+      ProfScope ps("conv1d_strip_kernel", stream);
+      conv1d_strip_kernel<<<dim3((unsigned)lanes, (unsigned)strips, (unsigned)nbatch), dim3(256), 0,
+                            stream>>>(c);
+      MXR_HIP_TRY(hipGetLastError());
+      return MXR_SUCCESS;
+    }
+  }
   const int64_t blocks = (a.total + 255) / 256;
   MXR_REQUIRE(blocks <= 0x7fffffff, MXR_INVALID_SIZE, "%s: too large", fn);
   {

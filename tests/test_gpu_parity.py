@@ -447,6 +447,48 @@ def test_conv_random_strided(mx, orc):
         assert rel(host(mx.conv2d(xd, dev(f2), m)), orc.conv2d(x, f2, m)) < RTOL
 
 
+def test_conv2d_real_tiles(mx, orc):
+    """Real maps and real filters (the CA-CFAR window sums) through the
+    staged-tile kernel: several 32 x 64 tiles per map, batch dims, a strided
+    output view, the 5 x 13 CFAR mask, the largest staged window (16 x 64)
+    and one beyond it (the per-output kernel), all three modes."""
+    rng = np.random.default_rng(9)
+    x = rng.random((2, 3, 70, 150)).astype(np.float32)
+    mask = np.ones((5, 13), np.float32)
+    mask[1:4, 5:8] = 0
+    for f in (mask, rng.standard_normal((16, 64)).astype(np.float32),
+              rng.standard_normal((17, 3)).astype(np.float32), np.ones((1, 1), np.float32)):
+        for m in ("full", "same", "valid"):
+            ref = orc.conv2d(x.reshape(6, 70, 150), f, m).reshape(2, 3, *orc.conv2d(x[0, 0], f, m).shape[-2:])
+            got = host(mx.conv2d(dev(x, torch.float32), dev(f, torch.float32), m))
+            assert rel(got, ref) < RTOL, (f.shape, m)
+    # output written through a transposed view of a larger buffer
+    ref = orc.conv2d(x[0, 0], mask, "same")
+    buf = torch.full((160, 80), -1.0, device="cuda")
+    outv = buf[:150, :70].t()
+    mx.conv2d(dev(x[0, 0], torch.float32), dev(mask, torch.float32), "same", out=outv)
+    assert rel(host(outv), ref) < RTOL
+    assert float(buf[150:].min()) == -1.0 and float(buf[:, 70:].min()) == -1.0
+
+
+def test_conv1d_strided_axis_into_permuted_view(mx, orc):
+    """The canceller form: conv1d along the pulse axis of permuted views, in
+    and out (conv axis strided, range axis unit stride), through the strip
+    kernel; complex input, real and complex taps, every mode."""
+    rng = np.random.default_rng(10)
+    C, P, R = 3, 130, 300
+    y = crandn(rng, C, P, R)
+    yd = dev(y)
+    for f in (np.array([1, -2, 1], np.float32), crandn(rng, 5), rng.standard_normal(40).astype(np.float32)):
+        fd = dev(f, torch.complex64 if np.iscomplexobj(f) else torch.float32)
+        for m in ("full", "same", "valid"):
+            n = {"full": P + len(f) - 1, "same": P, "valid": P - len(f) + 1}[m]
+            outbuf = torch.zeros((C, n, R), dtype=torch.complex64, device="cuda")
+            mx.conv1d(yd.permute(0, 2, 1), fd, m, out=outbuf.permute(0, 2, 1))
+            ref = orc.conv1d(np.transpose(y, (0, 2, 1)), f, m)
+            assert rel(host(outbuf.permute(0, 2, 1)), ref) < RTOL, (len(f), m)
+
+
 # ------------------------------------------------- end to end vs reference
 def test_model_chain_17_detections(mx, orc, matx_inputs, golden):
     """numpy-model dataflow on MatX's test input: 17 detections per channel
