@@ -54,6 +54,13 @@ struct DopClass {
   const int *d_item_job = nullptr;
 };
 
+// Classes of one stage run concurrently on helper streams: the caller's
+// stream records a fork event, each class's launches wait on it from their
+// own stream and record a join event, and the caller's stream waits on every
+// join (the same fork/join, captured into a graph, becomes parallel
+// branches).  All at default priority.
+constexpr int RAGGED_BRANCHES = 4;
+
 struct RaggedPlanObj {
   int dev = 0;
   std::vector<mxr_ragged_job> jobs;
@@ -63,6 +70,8 @@ struct RaggedPlanObj {
   void *tables = nullptr;  // device copies of the job tables
   std::vector<PcClass> pc;
   std::vector<DopClass> dop;
+  hipStream_t branch[RAGGED_BRANCHES] = {};
+  hipEvent_t fork[2] = {}, join[2][RAGGED_BRANCHES] = {};
 };
 
 std::mutex g_tune_mu;
@@ -172,6 +181,13 @@ void free_plan(RaggedPlanObj *p) {
   (void)hipFree(p->y);
   (void)hipFree(p->spec);
   (void)hipFree(p->tables);
+  for (int b = 0; b < RAGGED_BRANCHES; ++b) {
+    if (p->branch[b]) (void)hipStreamDestroy(p->branch[b]);
+    for (int k = 0; k < 2; ++k)
+      if (p->join[k][b]) (void)hipEventDestroy(p->join[k][b]);
+  }
+  for (int k = 0; k < 2; ++k)
+    if (p->fork[k]) (void)hipEventDestroy(p->fork[k]);
   (void)hipSetDevice(cur);
   delete p;
   plan_tables_release();
@@ -388,6 +404,20 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
       p->dop[c].d_item_job = ditem + dop_item0[c];
     }
   }
+  for (int b = 0; b < RAGGED_BRANCHES && !st; ++b) {
+    if (hipStreamCreateWithFlags(&p->branch[b], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&p->join[0][b], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&p->join[1][b], hipEventDisableTiming) != hipSuccess) {
+      set_error(MXR_HIP_ERROR, "%s: branch stream creation failed", fn);
+      st = MXR_HIP_ERROR;
+    }
+  }
+  for (int k = 0; k < 2 && !st; ++k) {
+    if (hipEventCreateWithFlags(&p->fork[k], hipEventDisableTiming) != hipSuccess) {
+      set_error(MXR_HIP_ERROR, "%s: event creation failed", fn);
+      st = MXR_HIP_ERROR;
+    }
+  }
   if (st) {
     free_plan(p);
     return st;
@@ -402,14 +432,32 @@ static mxr_status ragged_execute(mxr_ragged_plan h, hipStream_t s) {
   MXR_REQUIRE(device_id() == p->dev, MXR_INVALID_PARAMETER,
               "ragged_execute: plan was made on device %d, current device is %d", p->dev,
               device_id());
-  mxr_status st;
-  for (const PcClass &c : p->pc) {
-    if ((st = launch_wfspec_batch(c.M, c.d_wf, c.njobs, s))) return st;
-    if ((st = launch_pc_batch(c.M, c.d_jobs, c.d_item_job, c.items, c.grid, s))) return st;
-  }
-  for (const DopClass &c : p->dop)
-    if ((st = launch_doppler_batch(c.nd, c.d_jobs, c.d_item_job, c.items, c.grid, s))) return st;
-  return MXR_SUCCESS;
+  mxr_status st = MXR_SUCCESS;
+  // stage k: its classes spread over the branches (class i on branch
+  // i mod RAGGED_BRANCHES), then joined back into s
+  auto stage = [&](int k, size_t nclass, auto &&launch) -> mxr_status {
+    if (nclass == 0) return MXR_SUCCESS;
+    const int nb = (int)std::min<size_t>(nclass, RAGGED_BRANCHES);
+    MXR_HIP_TRY(hipEventRecord(p->fork[k], s));
+    for (int b = 0; b < nb; ++b) MXR_HIP_TRY(hipStreamWaitEvent(p->branch[b], p->fork[k], 0));
+    mxr_status r = MXR_SUCCESS;
+    for (size_t i = 0; i < nclass && !r; ++i) r = launch(i, p->branch[i % RAGGED_BRANCHES]);
+    for (int b = 0; b < nb; ++b) {
+      MXR_HIP_TRY(hipEventRecord(p->join[k][b], p->branch[b]));
+      MXR_HIP_TRY(hipStreamWaitEvent(s, p->join[k][b], 0));
+    }
+    return r;
+  };
+  st = stage(0, p->pc.size(), [&](size_t i, hipStream_t bs) -> mxr_status {
+    const PcClass &c = p->pc[i];
+    mxr_status r = launch_wfspec_batch(c.M, c.d_wf, c.njobs, bs);
+    return r ? r : launch_pc_batch(c.M, c.d_jobs, c.d_item_job, c.items, c.grid, bs);
+  });
+  if (st) return st;
+  return stage(1, p->dop.size(), [&](size_t i, hipStream_t bs) {
+    const DopClass &c = p->dop[i];
+    return launch_doppler_batch(c.nd, c.d_jobs, c.d_item_job, c.items, c.grid, bs);
+  });
 }
 
 static mxr_status ragged_block_size(mxr_ragged_plan h, int job, int *M) {
