@@ -124,6 +124,7 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t g, int64_t G) {
 }
 
 int device_id();
+int cu_count();  // compute units of the current device (cached)
 int max_lds_per_block();
 
 }  // namespace mxr

@@ -261,16 +261,40 @@ struct Pc4096Args {
 // through one ending at its last kept lag (stores past it are dropped), so
 // the block needs no per-element tests.  spec(i) gives spectrum bin i (the
 // block's 1/M included); thread j reads bins j + 256 q only.
-template <class Spec>
-__device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *base1,
-                                              const float4 *base2, float2 *lds, Spec spec) {
+// Loads samples m0 .. m0 + NV - 1 of thread j of block ob
+// (x[lo + j + 256 m]) into v[0 .. NV).
+template <int NV>
+__device__ __forceinline__ void load_block4096(const OlsBlock &ob, float2 (&v)[NV], int m0 = 0) {
   constexpr int NT = 256;
-  using LY = Layout<4096, 1, false>;
   const int j = threadIdx.x;
   const float2 *__restrict__ xr = ob.xr;
   const int lo = ob.lo, n32 = ob.n, seff32 = ob.seff;
+  if (lo + 4096 <= n32) {
+    int avail = seff32 - lo;
+    avail = avail < 0 ? 0 : (avail > 4096 ? 4096 : avail);
+    const buf_rsrc rx = make_rsrc(xr + lo, avail * 8);
+#pragma unroll
+    for (int m = 0; m < NV; ++m) v[m] = buf_load2(rx, (j + (m0 + m) * NT) * 8);
+  } else {  // the block wraps around the circular period n (full fft_conv)
+#pragma unroll
+    for (int m = 0; m < NV; ++m) {
+      int gi = lo + j + (m0 + m) * NT;
+      if (gi >= n32) gi -= n32;
+      v[m] = gi < seff32 ? xr[gi] : make_float2(0.f, 0.f);
+    }
+  }
+}
 
-  float2 v[16];
+// The block's transforms and stores, from its samples in v (loaded by
+// load_block4096).
+template <class Spec>
+__device__ __forceinline__ void filter_block4096(const OlsBlock &ob, float2 (&v)[16],
+                                                 const float4 *base1, const float4 *base2,
+                                                 float2 *lds, Spec spec) {
+  constexpr int NT = 256;
+  using LY = Layout<4096, 1, false>;
+  const int j = threadIdx.x;
+  const int lo = ob.lo;
   // Stockham exchange: write this pass's outputs, read the next pass's
   // inputs.  In the padded layout (one spare slot per 16) the outputs
   // base + q ls_out of butterfly j sit at pad(base) + q (ls_out = 1) or
@@ -293,21 +317,7 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
   const float4 *b2p = base2 + j * 2;
   float4 t0, t1;
 
-  // forward pass 0 (Ls = 1) straight from HBM
-  if (lo + 4096 <= n32) {
-    int avail = seff32 - lo;
-    avail = avail < 0 ? 0 : (avail > 4096 ? 4096 : avail);
-    const buf_rsrc rx = make_rsrc(xr + lo, avail * 8);
-#pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = buf_load2(rx, (j + m * NT) * 8);
-  } else {  // the block wraps around the circular period n (full fft_conv)
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      int gi = lo + j + m * NT;
-      if (gi >= n32) gi -= n32;
-      v[m] = gi < seff32 ? xr[gi] : make_float2(0.f, 0.f);
-    }
-  }
+  // forward pass 0 (Ls = 1) on the samples as loaded
   dft16<false>(v);
   t0 = b1p[0];
   t1 = b1p[1];
@@ -339,24 +349,39 @@ __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *b
   for (int q = 0; q < 16; ++q) buf_store2(ry, (j + q * 256) * 8, v[q]);
 }
 
+// One block, loaded and filtered.  Pre: the workgroup's previous use of lds
+// is finished.
+template <class Spec>
+__device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *base1,
+                                              const float4 *base2, float2 *lds, Spec spec) {
+  float2 v[16];
+  load_block4096(ob, v, 0);
+  filter_block4096(ob, v, base1, base2, lds, spec);
+}
+
 // The block's spectrum from its buffer (ob.wc).
 __device__ __forceinline__ void ols_block4096(const OlsBlock ob, const float4 *base1,
                                               const float4 *base2, float2 *lds) {
   ols_block4096(ob, base1, base2, lds, [&](int i) { return ob.wc[i]; });
 }
 
+__device__ __forceinline__ OlsBlock pc4096_block(const PcArgs &a, int64_t g) {
+  const int64_t row = g / a.nblk;
+  int64_t ox, oy;
+  batch_offsets(a.bi, row, &ox, &oy);
+  return OlsBlock{a.x + ox, a.y + oy, (int)((g % a.nblk) * a.V), (int)a.n, (int)a.seff,
+                  (int)a.nout, (int)a.V, a.wc};
+}
+
+// One workgroup per block.  (A persistent form that loads the next block's
+// samples during the current block's inverse passes needs more registers
+// than four workgroups per CU allow, and measured 11 % slower.)
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
 void pc4096_kernel(Pc4096Args pa) {
   using LY = Layout<4096, 1, false>;
   __shared__ float2 lds[LY::LDS_ELEMS];
   const PcArgs &a = pa.a;
-  const int64_t g = xcd_remap(blockIdx.x, a.groups);
-  const int64_t row = g / a.nblk;
-  int64_t ox, oy;
-  batch_offsets(a.bi, row, &ox, &oy);
-  const OlsBlock ob{a.x + ox, a.y + oy, (int)((g % a.nblk) * a.V), (int)a.n, (int)a.seff,
-                    (int)a.nout, (int)a.V, a.wc};
-  ols_block4096(ob, pa.base1, pa.base2, lds);
+  ols_block4096(pc4096_block(a, xcd_remap(blockIdx.x, a.groups)), pa.base1, pa.base2, lds);
 }
 
 // Small grids (fft_conv on a few hundred pulses, where one workgroup per
@@ -446,20 +471,24 @@ struct Pc8192Args {
 __device__ __forceinline__ void ols_block8192(const OlsBlock ob, const float4 *bA, const float4 *bB,
                                               const float4 *bC, float2 *lds) {
   constexpr int NT = 512;
-  using LY = Layout<8192, 1, false>;
   const int j = threadIdx.x;
   const float2 *__restrict__ xr = ob.xr;
-  float2 *__restrict__ yr = ob.yr;
   const int lo = ob.lo, n32 = ob.n, seff32 = ob.seff;
 
   float2 v[16];
+  // Stockham exchange in the padded layout (pad(i) = i + i / 16): outputs
+  // base + q ls_out of butterfly j sit at pad(base) + 2 q + (q >= 8)
+  // (ls_out = 2) or pad(base) + 34 q (ls_out = 32), input j + 512 m at
+  // pad(j) + 544 m: one address per phase, the rest instruction offsets.
+  const int rbase = lds_pad(j);
   auto xchg = [&](int ls_out) {
     const int k = j & (ls_out - 1), base = (j - k) * 16 + k;
+    const int wbase = lds_pad(base);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) lds[LY::addr(0, base + q * ls_out)] = v[q];
+    for (int q = 0; q < 16; ++q) lds[wbase + (ls_out == 2 ? 2 * q + (q >= 8) : 34 * q)] = v[q];
     wg_barrier();
 #pragma unroll
-    for (int m = 0; m < 16; ++m) v[m] = lds[LY::addr(0, j + m * NT)];
+    for (int m = 0; m < 16; ++m) v[m] = lds[rbase + 544 * m];
     wg_barrier();
   };
   auto ldx = [&](int i) {
@@ -471,18 +500,21 @@ __device__ __forceinline__ void ols_block8192(const OlsBlock ob, const float4 *b
   const float4 *pB = bB + (j & 31) * 2;
   const float4 *pC = bC + j * 2;
 
-  // forward radix-2 pass folded into the load (no per-element checks for a
-  // block inside the row)
+  // forward radix-2 pass folded into the load; a block that does not wrap
+  // reads through a raw buffer resource ending at the row's last sample
+  // (zeros past it from the hardware range check)
   {
     const int jh = j >> 1;
     const bool odd = (j & 1) != 0;
     float2 a[16], b[16];
-    if (lo + 8192 <= seff32 && lo + 8192 <= n32) {
-      const float2 *xb = xr + lo + jh;
+    if (lo + 8192 <= n32) {
+      int avail = seff32 - lo;
+      avail = avail < 0 ? 0 : (avail > 8192 ? 8192 : avail);
+      const buf_rsrc rx = make_rsrc(xr + lo, avail * 8);
 #pragma unroll
       for (int m = 0; m < 16; ++m) {
-        a[m] = xb[256 * m];
-        b[m] = xb[256 * m + 4096];
+        a[m] = buf_load2(rx, (jh + 256 * m) * 8);
+        b[m] = buf_load2(rx, (jh + 256 * m + 4096) * 8);
       }
     } else {
 #pragma unroll
@@ -509,18 +541,19 @@ __device__ __forceinline__ void ols_block8192(const OlsBlock ob, const float4 *b
   dft16<false>(v);
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], ob.wc[j + q * 512]);
-  // inverse radix-2 pass (Ls = 1) on bins j + 512q, j + 512q + 4096
+  // inverse radix-2 pass (Ls = 1) on bins j + 512q, j + 512q + 4096; its
+  // outputs 2 (j + 512 q) and the next sit at pad(2 j) + 1088 q and + 1
   t0 = pA[0];
   t1 = pA[1];
+  const int e0 = lds_pad(2 * j);
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const int e = 2 * (j + 512 * q);
-    lds[LY::addr(0, e)] = cadd(v[q], v[q + 8]);
-    lds[LY::addr(0, e + 1)] = csub(v[q], v[q + 8]);
+    lds[e0 + 1088 * q] = cadd(v[q], v[q + 8]);
+    lds[e0 + 1088 * q + 1] = csub(v[q], v[q + 8]);
   }
   wg_barrier();
 #pragma unroll
-  for (int m = 0; m < 16; ++m) v[m] = lds[LY::addr(0, j + m * NT)];
+  for (int m = 0; m < 16; ++m) v[m] = lds[rbase + 544 * m];
   wg_barrier();
   tw_from_base(v, t0, t1, true);  // inverse pass Ls = 2
   dft16<true>(v);
@@ -534,13 +567,12 @@ __device__ __forceinline__ void ols_block8192(const OlsBlock ob, const float4 *b
   xchg(32);
   tw_from_base(v, t0, t1, true);  // inverse pass Ls = 512: lags j + 512q
   dft16<true>(v);
+  // lags lo + j + 512 q below lo + lim are kept (stores past lim dropped by
+  // the range check; the spectrum carries the 1/M)
   const int lim = (ob.nout - lo) < ob.V ? (ob.nout - lo) : ob.V;
-  const int qn = lim >> 9, rem = lim & 511;
-  float2 *yb = yr + lo + j;
+  const buf_rsrc ry = make_rsrc(ob.yr + lo, lim * 8);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    if (q < qn || (q == qn && j < rem)) yb[q * 512] = v[q];  // the spectrum carries the 1/M
-  }
+  for (int q = 0; q < 16; ++q) buf_store2(ry, (j + q * 512) * 8, v[q]);
 }
 
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4)))

@@ -71,6 +71,19 @@ int device_id() {
   return d;
 }
 
+int cu_count() {
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  std::lock_guard<std::mutex> g(mu);
+  int d = device_id();
+  auto it = cache.find(d);
+  if (it != cache.end()) return it->second;
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess || v < 1) v = 256;
+  cache[d] = v;
+  return v;
+}
+
 int max_lds_per_block() {
   static std::mutex mu;
   static std::map<int, int> cache;
