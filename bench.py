@@ -115,9 +115,11 @@ def cfar_mask(win, guard):
     return m
 
 
-def cpu_baseline_leg(x0, w, pow0, dets0, nd, budget_s, C, mask):
+def cpu_baseline_leg(x, w, pw, dets, nd, budget_s, C, mask, all_channels=True):
     """Times the FP64 oracle (C, OpenMP) on channel 0 of the cube, and checks
-    the GPU's channel 0 against it."""
+    the GPU's power map and detections against it: every channel of the cube
+    when all_channels (one more oracle pass over the whole cube), else
+    channel 0."""
     import numpy as np
     sys.path.insert(0, ROOT)
     from oracle import oracle as orc
@@ -126,25 +128,29 @@ def cpu_baseline_leg(x0, w, pow0, dets0, nd, budget_s, C, mask):
     reps, t0 = 0, time.perf_counter()
     st = None
     while True:
-        st = orc.radar_pipeline(x0[None], w, Nd=nd, mode="model", mask=mask, stages=True)
+        st = orc.radar_pipeline(x[0][None], w, Nd=nd, mode="model", mask=mask, stages=True)
         reps += 1
         if time.perf_counter() - t0 >= budget_s or reps >= 2000:
             break
     el = time.perf_counter() - t0
-    pw_ref = st["pow"][0]
-    rel = float(np.linalg.norm(pow0 - pw_ref) / np.linalg.norm(pw_ref))
-    near = np.abs(st["margin"][0]) < 1e-5
-    mism = int(((dets0 != st["dets"][0]) & ~near).sum())
     cpu = {"value": reps / C / el, "unit": "cubes/s", "cores": cores, "kind": "port",
            "sample": f"FP64 oracle (oracle/mxr_oracle.c, OpenMP) on channel 0 of the "
                      f"same cube (1/{C} cube), {reps} runs in {el:.1f} s"}
+    chans = C if all_channels else 1
+    if chans > 1:
+        st = orc.radar_pipeline(x[:chans], w, Nd=nd, mode="model", mask=mask, stages=True)
+    pw_ref = st["pow"]
+    pw_g = pw[:chans].astype(np.float64)
+    rel = float(np.linalg.norm(pw_g - pw_ref) / np.linalg.norm(pw_ref))
+    near = np.abs(st["margin"]) < 1e-5
+    mism = int(((dets[:chans] != st["dets"]) & ~near).sum())
     # the CFAR stage alone: the FP64 CFAR applied to the GPU's own power map
-    cf = orc.cfar(pow0[None].astype(np.float64), mask)
-    near_cf = np.abs(cf["margin"][0]) < 1e-5
-    mism_cf = int(((dets0 != cf["dets"][0]) & ~near_cf).sum())
-    parity = {"channel": 0, "pow_rel_l2_vs_fp64": rel, "det_mismatch_outside_1e-5": mism,
+    cf = orc.cfar(pw_g, mask)
+    near_cf = np.abs(cf["margin"]) < 1e-5
+    mism_cf = int(((dets[:chans] != cf["dets"]) & ~near_cf).sum())
+    parity = {"channels": chans, "pow_rel_l2_vs_fp64": rel, "det_mismatch_outside_1e-5": mism,
               "det_mismatch_cfar_stage": mism_cf,
-              "dets_gpu": int(dets0.sum()), "dets_oracle": int(st["dets"][0].sum())}
+              "dets_gpu": int(dets[:chans].sum()), "dets_oracle": int(st["dets"].sum())}
     return cpu, parity
 
 
@@ -1050,9 +1056,12 @@ def bench_pipeline(args, cfg):
 
     oracle, parity = None, None
     if cpu_s > 0:
-        oracle, parity = cpu_baseline_leg(x[0].cpu().numpy(), w.cpu().numpy(),
-                                          pipe.pw[0].cpu().numpy(), dets[0].cpu().numpy(), nd,
-                                          cpu_s, C, cfar_mask(win, guard))
+        # every channel of the headline cube; channel 0 of the 8.6 GB cube
+        full = cfg["tag"] == "radar16x128x9000"
+        nch = C if full else 1
+        oracle, parity = cpu_baseline_leg(x[:nch].cpu().numpy(), w.cpu().numpy(),
+                                          pipe.pw[:nch].cpu().numpy(), dets[:nch].cpu().numpy(),
+                                          nd, cpu_s, C, cfar_mask(win, guard), all_channels=full)
     standin = None
     if cpu_s > 0 and cfg["tag"] == "radar16x128x9000":
         standin = cpu_standin_leg(x[0].cpu().numpy(), w.cpu().numpy(), nd, win, guard, cpu_s, C)
