@@ -69,6 +69,10 @@ def parse():
                          "MatX's stage calls on the C++ front end; conv: conv1d by the FFT "
                          "method on 16 x 1048576 c64 with 209 taps and 1000-point Bluestein "
                          "FFTs; ew: fused elementwise expressions at the radar cube's size")
+    ap.add_argument("--streams", type=int, default=None,
+                    help="pipelines replaying their own graphs on their own streams in the "
+                         "timed region of the radar workloads (default 3 for 16x128x9000, 1 for "
+                         "64x512x32768)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 512 MiB device copies (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -923,14 +927,16 @@ def main():
 # window with 2 x 1 guard cells around the cell under test; large: 21 x 7.
 WORKLOADS = {
     "radar": dict(C=16, P=128, S=9000, L=1000, nd=128, bw=0.4, win=(5, 13), guard=(3, 5),
-                  targets=20, snr=(0.0, 20.0), steps=None, cubes=4, tag="radar16x128x9000",
+                  targets=20, snr=(0.0, 20.0), steps=None, cubes=4, streams=3,
+                  tag="radar16x128x9000",
                   metric="cubes/s (16x128x9000 c64 radar pipeline)",
                   desc="radar pipeline 16ch x 128 pulses x 9000 samples c64, L=1000 LFM (0.4 fs), "
                        "N=16384 pulse compression, [1,-2,1] canceller (126 rows), Hamming + "
                        "128-pt Doppler FFT, CA-CFAR 13x5 window / 2x1 guard cells (5x3 hole), "
                        "Pfa 1e-5; waveform spectrum prepared once per waveform"),
     "large": dict(C=64, P=512, S=32768, L=4096, nd=512, bw=0.5, win=(7, 21), guard=(3, 5),
-                  targets=200, snr=(0.0, 25.0), steps=20, cubes=1, tag="radar64x512x32768",
+                  targets=200, snr=(0.0, 25.0), steps=20, cubes=1, streams=1,
+                  tag="radar64x512x32768",
                   metric="cubes/s (64x512x32768 c64 radar pipeline)",
                   desc="radar pipeline 64ch x 512 pulses x 32768 samples c64 (8.6 GB), L=4096 "
                        "LFM (0.5 fs), N=32768 pulse compression, [1,-2,1] canceller (510 rows), "
@@ -970,28 +976,55 @@ def bench_pipeline(args, cfg):
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    # throughput: the cubes replayed from a captured hipGraph (G cubes per
+    # throughput: the cubes replayed from captured hipGraphs (G cubes per
     # graph, cycling through the resident cubes), with no per-kernel
-    # instrumentation in the timed region
+    # instrumentation in the timed region.  As in MatX's radar example
+    # (examples/simple_radar_pipeline.cu:44-137, num_streams pipelines, one
+    # graph per stream, all launched back to back), `streams` pipelines each
+    # replay their own graph on their own stream, so one cube's pulse
+    # compression can run beside another's pulse-axis + CFAR pass
+    ns = cfg["streams"] if args.streams is None else max(1, args.streams)
+    pipes = [pipe]
+    for _ in range(ns - 1):
+        q = matx_amd.FusedRadarPipeline(C, P, S, L, nd=nd, tpc="valid", win=win, guard=guard,
+                                        want_pow=not fused)
+        q.set_waveform(w)
+        for i in range(min(warmup, ncubes)):
+            q.step(xs[i % ncubes])
+        q.sync()
+        pipes.append(q)
     G = max(ncubes, min(steps, 48) // ncubes * ncubes)
-    reps = max(1, steps // G)
-    gs = torch.cuda.Stream()
-    gs.wait_stream(stream)
-    graph = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(graph, stream=gs):
-        for i in range(G):
-            pipe.step(xs[i % ncubes])
+    reps = max(1, round(steps / (G * ns)))
+    gss = [torch.cuda.Stream() for _ in pipes]
+    graphs = []
+    for k, (p, gs) in enumerate(zip(pipes, gss)):
+        gs.wait_stream(stream)
+        gr = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(gr, stream=gs):
+            for i in range(G):
+                p.step(xs[(i + k) % ncubes])
+        graphs.append(gr)
     torch.cuda.synchronize()
-    graph.replay()
+
+    def replay_all(n):
+        for _ in range(n):
+            for gr, gs in zip(graphs, gss):
+                with torch.cuda.stream(gs):
+                    gr.replay()
+
+    replay_all(1)
     torch.cuda.synchronize()
     e0.record(stream)
-    for _ in range(reps):
-        graph.replay()
+    for gs in gss:
+        gs.wait_event(e0)
+    replay_all(reps)
+    for gs in gss:
+        stream.wait_stream(gs)
     e1.record(stream)
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / (reps * G)
-    steps_timed = reps * G
-    del graph
+    steps_timed = reps * G * ns
+    ms = e0.elapsed_time(e1) / steps_timed
+    del graphs, pipes[1:]
     # eager loop (instrumented when profiling): per-kernel durations from
     # HIP events recorded around every launch on its own stream
     prof = not args.no_profile
@@ -1088,10 +1121,14 @@ def bench_pipeline(args, cfg):
         "config": {"workload": cfg["desc"],
                    "channels": C, "pulses": P, "samples": S, "waveform": L, "doppler": nd,
                    "resident_cubes": ncubes,
-                   "timing": (f"value: {steps_timed} cubes replayed from a hipGraph of {G} "
-                              f"captured cubes cycling over {ncubes} resident cube(s), no "
-                              f"instrumentation; eager loop of {steps} cubes with per-launch HIP "
-                              f"events: {round(ms_eager, 5)} ms per cube"),
+                   "streams": ns,
+                   "timing": (f"value: {steps_timed} cubes from {ns} pipeline(s) on {ns} "
+                              f"stream(s), each replaying its own hipGraph of {G} captured cubes "
+                              f"{reps} time(s), cycling over {ncubes} resident cube(s), all graphs "
+                              f"launched back to back, wall time between events on the launching "
+                              f"stream, no instrumentation; eager loop of {steps} cubes on one "
+                              f"stream with per-launch HIP events: {round(ms_eager, 5)} ms per "
+                              f"cube"),
                    "ms_per_step_eager_instrumented": round(ms_eager, 5),
                    "algorithmic_bytes_per_cube": B,
                    "pipeline_hbm_gbs": round(B * value / 1e9, 1),
