@@ -70,13 +70,38 @@ def parse():
                          "method on 16 x 1048576 c64 with 209 taps and 1000-point Bluestein "
                          "FFTs; ew: fused elementwise expressions at the radar cube's size")
     ap.add_argument("--streams", type=int, default=None,
-                    help="pipelines replaying their own graphs on their own streams in the "
-                         "timed region of the radar workloads (default 3 for 16x128x9000, 1 for "
-                         "64x512x32768)")
+                    help="pipelines run side by side as branches of the timed graph of the "
+                         "radar workloads, each on its own HIP stream (default 3 for "
+                         "16x128x9000, 1 for 64x512x32768)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 512 MiB device copies (scale reference for the "
                          "FETCH_SIZE / WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+class HipStream:
+    """A non-blocking HIP stream created for this process (not one of
+    torch's pooled streams, which share the device's hardware queues
+    round-robin), usable as a torch stream."""
+
+    def __init__(self):
+        import ctypes
+        import torch
+        try:
+            self._hip = ctypes.CDLL("libamdhip64.so")
+        except OSError:
+            self._hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+        h = ctypes.c_void_p()
+        rc = self._hip.hipStreamCreateWithFlags(ctypes.byref(h), 1)
+        if rc != 0:
+            raise RuntimeError(f"hipStreamCreateWithFlags failed: {rc}")
+        self._h = h
+        self.stream = torch.cuda.ExternalStream(h.value)
+
+    def close(self):
+        if self._h is not None:
+            self._hip.hipStreamDestroy(self._h)
+            self._h = None
 
 
 def synth_cube(C, P, S, L, nd, n_targets, snr_db, seed, device, bw=0.4):
@@ -976,13 +1001,14 @@ def bench_pipeline(args, cfg):
     stream = torch.cuda.current_stream()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    # throughput: the cubes replayed from captured hipGraphs (G cubes per
-    # graph, cycling through the resident cubes), with no per-kernel
-    # instrumentation in the timed region.  As in MatX's radar example
-    # (examples/simple_radar_pipeline.cu:44-137, num_streams pipelines, one
-    # graph per stream, all launched back to back), `streams` pipelines each
-    # replay their own graph on their own stream, so one cube's pulse
-    # compression can run beside another's pulse-axis + CFAR pass
+    # throughput: cubes replayed from one captured hipGraph, with no
+    # per-kernel instrumentation in the timed region.  As in MatX's radar
+    # example (examples/simple_radar_pipeline.cu:44-137: num_streams
+    # pipelines, each with its own stream), `streams` pipelines run side by
+    # side: the graph forks into one branch per pipeline (captured on its own
+    # HIP stream, G cubes cycling through the resident cubes) and joins at
+    # the end, so one cube's pulse compression can run beside another's
+    # pulse-axis + CFAR pass
     ns = cfg["streams"] if args.streams is None else max(1, args.streams)
     pipes = [pipe]
     for _ in range(ns - 1):
@@ -995,36 +1021,36 @@ def bench_pipeline(args, cfg):
         pipes.append(q)
     G = max(ncubes, min(steps, 48) // ncubes * ncubes)
     reps = max(1, round(steps / (G * ns)))
-    gss = [torch.cuda.Stream() for _ in pipes]
-    graphs = []
-    for k, (p, gs) in enumerate(zip(pipes, gss)):
-        gs.wait_stream(stream)
-        gr = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(gr, stream=gs):
-            for i in range(G):
-                p.step(xs[(i + k) % ncubes])
-        graphs.append(gr)
+    branches = [HipStream() for _ in pipes]
+    gs = torch.cuda.Stream()
+    gs.wait_stream(stream)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=gs):
+        fork = torch.cuda.Event()
+        fork.record(gs)
+        for br in branches:
+            br.stream.wait_event(fork)
+        for k, (p, br) in enumerate(zip(pipes, branches)):
+            with torch.cuda.stream(br.stream):
+                for i in range(G):
+                    p.step(xs[(i + k) % ncubes])
+        for br in branches:
+            join = torch.cuda.Event()
+            join.record(br.stream)
+            gs.wait_event(join)
     torch.cuda.synchronize()
-
-    def replay_all(n):
-        for _ in range(n):
-            for gr, gs in zip(graphs, gss):
-                with torch.cuda.stream(gs):
-                    gr.replay()
-
-    replay_all(1)
+    graph.replay()
     torch.cuda.synchronize()
     e0.record(stream)
-    for gs in gss:
-        gs.wait_event(e0)
-    replay_all(reps)
-    for gs in gss:
-        stream.wait_stream(gs)
+    for _ in range(reps):
+        graph.replay()
     e1.record(stream)
     torch.cuda.synchronize()
     steps_timed = reps * G * ns
     ms = e0.elapsed_time(e1) / steps_timed
-    del graphs, pipes[1:]
+    del graph, pipes[1:]
+    for br in branches:
+        br.close()
     # eager loop (instrumented when profiling): per-kernel durations from
     # HIP events recorded around every launch on its own stream
     prof = not args.no_profile
@@ -1122,13 +1148,12 @@ def bench_pipeline(args, cfg):
                    "channels": C, "pulses": P, "samples": S, "waveform": L, "doppler": nd,
                    "resident_cubes": ncubes,
                    "streams": ns,
-                   "timing": (f"value: {steps_timed} cubes from {ns} pipeline(s) on {ns} "
-                              f"stream(s), each replaying its own hipGraph of {G} captured cubes "
-                              f"{reps} time(s), cycling over {ncubes} resident cube(s), all graphs "
-                              f"launched back to back, wall time between events on the launching "
-                              f"stream, no instrumentation; eager loop of {steps} cubes on one "
-                              f"stream with per-launch HIP events: {round(ms_eager, 5)} ms per "
-                              f"cube"),
+                   "timing": (f"value: {steps_timed} cubes, {reps} replay(s) of one hipGraph "
+                              f"that forks into {ns} branch(es), one pipeline per branch on its "
+                              f"own HIP stream, {G} captured cubes per branch cycling over "
+                              f"{ncubes} resident cube(s); HIP events around the replays, no "
+                              f"instrumentation; eager loop of {steps} cubes on one stream with "
+                              f"per-launch HIP events: {round(ms_eager, 5)} ms per cube"),
                    "ms_per_step_eager_instrumented": round(ms_eager, 5),
                    "algorithmic_bytes_per_cube": B,
                    "pipeline_hbm_gbs": round(B * value / 1e9, 1),

@@ -22,13 +22,15 @@
 //     t) at i*TB + t).
 //  2. Stockham passes down the LDS columns (lane = column: unit-stride LDS);
 //     the last pass writes |X|^2 over the first half of the tile.
-//  3. CFAR: a thread owns 4 adjacent columns and a band of Doppler rows,
-//     reads each power row with 16-byte LDS loads, forms the positive
-//     partial sums L (left of the hole), M (hole columns), Rt (right of the
-//     hole) of its 4 columns, keeps 2HY+1 rows of them in registers and
-//     writes one 16-byte detection quad per row.  Sums never subtract, so a
-//     strong cell in the hole cannot cancel (the same decomposition as the
-//     stand-alone CFAR kernel).
+//  3. CFAR: a thread owns 4 adjacent output columns on every KS-th Doppler
+//     row.  It reads each of its power rows with 16-byte LDS loads and forms
+//     the positive partial sums L (left of the hole), M (hole columns), Rt
+//     (right of the hole) of its 4 columns once per row; lr = L + Rt and
+//     full = lr + M go back to LDS, and each cell's noise sum adds full over
+//     the window rows outside the hole's Doppler extent and lr over the rows
+//     inside it.  One 16-byte detection quad per row.  Sums never subtract,
+//     so a strong cell in the hole cannot cancel (the same decomposition as
+//     the stand-alone CFAR kernel).
 #include <cmath>
 #include <cstdlib>
 
@@ -124,14 +126,19 @@ void dopcfar_kernel(DcArgs a) {
   static_assert(OUT >= 16, "window too wide for a 64-column tile");
   static_assert(ND >= NW, "Doppler size below one row per wave");
   constexpr int PS = DC_PS;
-  static_assert((ND + 2 * HY) * PS <= 2 * ND * TB, "power tile and its zero rows must fit the complex tile");
   constexpr int SR = ND / NW;  // Doppler-input rows per wave in the load phase
   using LY = Layout<ND, TB, true>;
   using SQ = Seq<ND, false>;
   constexpr int K = SQ::n;
   constexpr int NCELL = (2 * HY + 1) * (2 * HX + 1);
   static_assert(NCELL <= DC_MAXCELL, "window area");
-  __shared__ __align__(16) float2 lds[ND * TB];
+  // the complex tile; after the transform it holds the power rows (stride
+  // PS, HY zero rows either side) followed by the CFAR's full-sum rows
+  // (stride OUT)
+  constexpr int LDS2 = ND * TB > ((ND + 2 * HY) * (PS + OUT) + 1) / 2 ? ND * TB
+                                                                      : ((ND + 2 * HY) * (PS + OUT) + 1) / 2;
+  __shared__ __align__(16) float2 lds[LDS2];
+  static_assert((ND + 2 * HY) * (PS + OUT) <= 2 * LDS2, "power and full-sum rows must fit the tile");
   __shared__ float2 coef[ND];  // Doppler-input row i = coef.x*(y[p-1]+y[p+1]) + coef.y*y[p]
   __shared__ float thr_tab[NCELL + 1];
 
@@ -202,21 +209,73 @@ void dopcfar_kernel(DcArgs a) {
   }
   wg_barrier();
 
-  // ---- 3. CFAR over the inner columns: 16 column quads x NB row bands
-  constexpr int GR = TB / 4;                  // column quads per row
-  constexpr int NB = ND < 32 ? ND : 32;       // row bands
-  constexpr int RB = ND / NB;                 // rows per band
-  constexpr int RING = 2 * HY + 1;
-  constexpr int NQ = 1 + 2 * HXA / 4;         // quads read per row
+  // ---- 3. CFAR over the inner columns, in two steps over the same cells.
+  // A thread owns one output column quad (tile columns tc0..tc0+3) on the
+  // rows k = kb + KS*it.
+  //  3a. per row: the positive partial sums of the row segment, lr = left +
+  //      right of the hole and full = lr + hole columns (each row's sums are
+  //      formed once; full goes to the free upper part of the tile, lr
+  //      replaces the power row once every thread has read its row);
+  //  3b. per cell: noise = sum over the window rows of full (rows outside the
+  //      hole's Doppler extent) or lr (rows inside it), then the threshold.
+  // The cell under test and the row's own lr stay in registers in between.
+  constexpr int NQO = OUT / 4;                // output column quads per row
+  constexpr int KS = NT / NQO;                // row step between a thread's rows
+  constexpr int QIT = (ND + KS - 1) / KS;     // rows per thread (at most)
+  constexpr int NQ = 1 + 2 * HXA / 4;         // power quads read per row segment
   constexpr int NV = 4 * NQ;
-  const int grp = threadIdx.x % GR, band = threadIdx.x / GR;
-  const int tc0 = HXA + 4 * grp;  // first tile column of the quad
-  const int k0 = band * RB;
-  if (band >= NB || grp >= OUT / 4) return;  // no barrier follows
-  const int gr0 = r0 + tc0;  // its range bin (a multiple of 4, >= 0)
-  if (gr0 >= a.R) return;
-  const int R = a.R;
+  constexpr int FS = OUT;                     // full-sum row stride (floats)
+  float *const fsum = reinterpret_cast<float *>(lds) + (ND + 2 * HY) * PS + HY * FS;  // row 0
+  const int q = threadIdx.x % NQO, kb = threadIdx.x / NQO;
+  const bool act = kb < KS;                   // the last NT % NQO threads idle
+  const int tc0 = HXA + 4 * q;                // first tile column of the quad
+  float cut[QIT][4], lrk[QIT][4];
+  // zero rows of the full sums (the CFAR's zero padding along Doppler)
+  for (int i = threadIdx.x; i < 2 * HY * NQO; i += NT) {
+    const int row = i / NQO < HY ? i / NQO - HY : ND + i / NQO - HY;
+    *reinterpret_cast<float4 *>(fsum + row * FS + 4 * (i % NQO)) = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+#pragma unroll
+  for (int it = 0; it < QIT; ++it) {
+    const int k = kb + KS * it;
+    if (!act || k >= ND) continue;
+    float u[NV];
+    const float *row = pwt + k * PS + tc0 - HXA;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) {
+      const float4 z = *reinterpret_cast<const float4 *>(row + 4 * m);
+      u[4 * m] = z.x;
+      u[4 * m + 1] = z.y;
+      u[4 * m + 2] = z.z;
+      u[4 * m + 3] = z.w;
+    }
+    // u[HXA + j + d] is tile column tc0 + j + d
+    float L[4], Mh[4], Rt[4], fl[4];
+    dc_win4<HX - EX>(u + HXA - HX, L);
+    dc_win4<2 * EX + 1>(u + HXA - EX, Mh);
+    dc_win4<HX - EX>(u + HXA + EX + 1, Rt);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      lrk[it][j] = L[j] + Rt[j];
+      fl[j] = lrk[it][j] + Mh[j];
+      cut[it][j] = u[HXA + j];
+    }
+    *reinterpret_cast<float4 *>(fsum + k * FS + 4 * q) = make_float4(fl[0], fl[1], fl[2], fl[3]);
+  }
+  wg_barrier();  // every power row has been read
+  // lr in place of the power rows (columns 0..OUT-1; the zero rows stay zero)
+#pragma unroll
+  for (int it = 0; it < QIT; ++it) {
+    const int k = kb + KS * it;
+    if (!act || k >= ND) continue;
+    *reinterpret_cast<float4 *>(pwt + k * PS + 4 * q) =
+        make_float4(lrk[it][0], lrk[it][1], lrk[it][2], lrk[it][3]);
+  }
+  wg_barrier();
 
+  const int gr0 = r0 + tc0;  // range bin of the quad (a multiple of 4, >= 0)
+  if (!act || gr0 >= a.R) return;  // no barrier follows
+  const int R = a.R;
   // in-bounds cell count = ro*co - rh*chh; away from the first and last HY
   // Doppler rows ro = 2HY+1 and rh = 2EY+1, so it depends on the column only
   int co[4], chh[4], nrm_int[4];
@@ -234,39 +293,31 @@ void dopcfar_kernel(DcArgs a) {
   float *const ba_c = EXACT ? a.ba + (int64_t)c * a.bs_c : nullptr;
   float *const pw_c = a.pw ? a.pw + (int64_t)c * a.ps_c : nullptr;
 
-  float full[RING][4], lr[RING][4];
-  constexpr int NLY = RB + 2 * HY;  // rows walked by the band
-  // one ring period unrolled per trip (ring slots stay compile-time, live
-  // ranges stay within a period)
-#pragma unroll 1
-  for (int ly0 = 0; ly0 < NLY; ly0 += RING)
 #pragma unroll
-  for (int sl = 0; sl < RING; ++sl) {
-    const int ly = ly0 + sl;
-    if (ly >= NLY) break;
-    const int k = k0 - HY + ly;  // in [-HY, ND + HY): zero rows outside the map
-    float u[NV];
-    const float *row = pwt + k * PS + tc0 - HXA;
+  for (int it = 0; it < QIT; ++it) {
+    const int kc = kb + KS * it;  // Doppler row of the cells under test
+    if (kc >= ND) continue;
+    float sm[4];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const float4 z = *reinterpret_cast<const float4 *>(row + 4 * q);
-      u[4 * q] = z.x;
-      u[4 * q + 1] = z.y;
-      u[4 * q + 2] = z.z;
-      u[4 * q + 3] = z.w;
+    for (int j = 0; j < 4; ++j) sm[j] = 0.f;
+#pragma unroll
+    for (int dy = -HY; dy <= HY; ++dy) {
+      float t[4];
+      if (dy == 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[j] = lrk[it][j];
+      } else {
+        const bool inner = dy >= -EY && dy <= EY;
+        const float *src = inner ? pwt + (kc + dy) * PS + 4 * q : fsum + (kc + dy) * FS + 4 * q;
+        const float4 z = *reinterpret_cast<const float4 *>(src);
+        t[0] = z.x;
+        t[1] = z.y;
+        t[2] = z.z;
+        t[3] = z.w;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sm[j] = dy == -HY ? t[j] : sm[j] + t[j];
     }
-    // u[HXA + j + d] is tile column tc0 + j + d
-    float L[4], Mh[4], Rt[4];
-    dc_win4<HX - EX>(u + HXA - HX, L);
-    dc_win4<2 * EX + 1>(u + HXA - EX, Mh);
-    dc_win4<HX - EX>(u + HXA + EX + 1, Rt);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      lr[sl][j] = L[j] + Rt[j];
-      full[sl][j] = lr[sl][j] + Mh[j];
-    }
-    if (ly < 2 * HY) continue;
-    const int kc = k0 + ly - 2 * HY;  // Doppler row of the cells under test
     float thr[4];
     int nrm[4];
 #pragma unroll
@@ -283,31 +334,16 @@ void dopcfar_kernel(DcArgs a) {
         thr[j] = thr_tab[min(max(nrm[j], 0), NCELL)];
       }
     }
-    float4 cut;
-    if constexpr (HXA % 4 == 0) {
-      cut = *reinterpret_cast<const float4 *>(pwt + kc * PS + tc0);
-    } else {  // 8-byte aligned: two 64-bit LDS reads
-      const float2 c01 = *reinterpret_cast<const float2 *>(pwt + kc * PS + tc0);
-      const float2 c23 = *reinterpret_cast<const float2 *>(pwt + kc * PS + tc0 + 2);
-      cut = make_float4(c01.x, c01.y, c23.x, c23.y);
-    }
-    const float cv[4] = {cut.x, cut.y, cut.z, cut.w};
+    const float *cv = cut[it];
     int det[4];
     float bav[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      float sm = 0.f;
-#pragma unroll
-      for (int dy = -HY; dy <= HY; ++dy) {
-        const int kk = (sl - HY + dy + RING) % RING;  // ring slot of row kc + dy
-        const float t = (dy >= -EY && dy <= EY) ? lr[kk][j] : full[kk][j];
-        sm = dy == -HY ? t : sm + t;
-      }
       if constexpr (EXACT) {
-        bav[j] = sm / (float)nrm[j];
+        bav[j] = sm[j] / (float)nrm[j];
         det[j] = cv[j] > thr[j] * bav[j] ? 1 : 0;
       } else {
-        det[j] = cv[j] > thr[j] * sm ? 1 : 0;
+        det[j] = cv[j] > thr[j] * sm[j] ? 1 : 0;
       }
     }
     const uint32_t od = (uint32_t)(kc * a.ds_k + gr0) * 4u;
@@ -319,7 +355,9 @@ void dopcfar_kernel(DcArgs a) {
       if constexpr (EXACT)
         *reinterpret_cast<float4 *>(reinterpret_cast<char *>(ba_c) + ob) =
             make_float4(bav[0], bav[1], bav[2], bav[3]);
-      if (pw_c) *reinterpret_cast<float4 *>(reinterpret_cast<char *>(pw_c) + op) = cut;
+      if (pw_c)
+        *reinterpret_cast<float4 *>(reinterpret_cast<char *>(pw_c) + op) =
+            make_float4(cv[0], cv[1], cv[2], cv[3]);
     } else {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
