@@ -326,11 +326,16 @@ __device__ __forceinline__ void filter_block4096(const OlsBlock &ob, float2 (&v)
   dft16<false>(v);
   t0 = b2p[0];
   t1 = b2p[1];
+  // the spectrum bins this thread multiplies, fetched before the exchange
+  // so that their cache latency hides behind it and the last forward pass
+  float2 sp[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) sp[q] = spec(j + q * 256);
   xchg(16);
   tw_from_base(v, t0, t1, false);  // forward pass 2 (Ls = 256)
   dft16<false>(v);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], spec(j + q * 256));
+  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], sp[q]);
   dft16<true>(v);            // inverse pass 0 (Ls = 1)
   t0 = b1p[0];
   t1 = b1p[1];
@@ -536,11 +541,15 @@ __device__ __forceinline__ void ols_block8192(const OlsBlock ob, const float4 *b
   dft16<false>(v);
   t0 = pC[0];
   t1 = pC[1];
+  // the spectrum bins this thread multiplies, fetched before the exchange
+  float2 sp[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) sp[q] = ob.wc[j + q * 512];
   xchg(32);
   tw_from_base(v, t0, t1, false);  // forward pass Ls = 512: bins j + 512q
   dft16<false>(v);
 #pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], ob.wc[j + q * 512]);
+  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], sp[q]);
   // inverse radix-2 pass (Ls = 1) on bins j + 512q, j + 512q + 4096; its
   // outputs 2 (j + 512 q) and the next sit at pad(2 j) + 1088 q and + 1
   t0 = pA[0];
