@@ -278,11 +278,19 @@ __global__ __launch_bounds__(CL_NT) void cfar_lds_kernel(CfarArgs a) {
   }
   const float *in_c = reinterpret_cast<const float *>(a.in) + (int64_t)c * a.is_c;
   const int isy = (int)a.is_y;
-  for (int qi = threadIdx.x; qi < ROWS * QPR; qi += CL_NT) {
+  // every quad of the staged tile is loaded before any is stored, so all of
+  // a thread's loads are in flight together (a load-store loop would wait
+  // out one memory latency per quad)
+  constexpr int NSQ = ROWS * QPR;                 // staged quads
+  constexpr int NIT = (NSQ + CL_NT - 1) / CL_NT;  // per thread
+  float4 zq[NIT];
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int qi = threadIdx.x + it * CL_NT;
     const int r = qi / QPR, qc = qi - r * QPR;
     const int gy = y0 - HY + r, gx = x0 - HALO + 4 * qc;
     float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (gy >= 0 && gy < Ny) {
+    if (qi < NSQ && gy >= 0 && gy < Ny) {
       const float *rp = in_c + gy * isy;
       if (gx >= 0 && gx + 3 < Nx) {
         z = *reinterpret_cast<const float4 *>(rp + gx);
@@ -293,7 +301,13 @@ __global__ __launch_bounds__(CL_NT) void cfar_lds_kernel(CfarArgs a) {
         z.w = (gx + 3 >= 0 && gx + 3 < Nx) ? rp[gx + 3] : 0.f;
       }
     }
-    *reinterpret_cast<float4 *>(tile + r * PS + 4 * qc) = z;
+    zq[it] = z;
+  }
+#pragma unroll
+  for (int it = 0; it < NIT; ++it) {
+    const int qi = threadIdx.x + it * CL_NT;
+    const int r = qi / QPR, qc = qi - r * QPR;
+    if (qi < NSQ) *reinterpret_cast<float4 *>(tile + r * PS + 4 * qc) = zq[it];
   }
   __syncthreads();
 

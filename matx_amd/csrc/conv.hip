@@ -132,12 +132,34 @@ __global__ __launch_bounds__(256) void conv2d_tile_kernel(Conv2TileArgs a) {
   }
   const int rows = C2_TH + a.fh - 1, cols = C2_TW + a.fw - 1;
   const int iy0 = oy0 - a.dy, ix0 = ox0 - a.dx;
-  for (int e = threadIdx.x; e < rows * C2_LW; e += 256) {
-    const int r = e / C2_LW, c = e % C2_LW;
-    const int iy = iy0 + r, ix = ix0 + c;
-    float v = 0.f;
-    if (c < cols && iy >= 0 && iy < a.H && ix >= 0 && ix < a.W) v = a.in[ib + iy * a.is0 + ix * a.is1];
-    tile[e] = v;
+  // staged in chunks of 8 loads per thread, all issued before any is
+  // stored; positions outside the map read the batch's first element and
+  // are zeroed with a mask (a select against a load would become a branch
+  // around it and serialise the loads)
+  const int nst = rows * C2_LW;
+  if (a.H > 0 && a.W > 0) {
+    constexpr int NST = (C2_TH + C2_FHMAX) * C2_LW / 256;
+    static_assert(NST % 8 == 0, "whole chunks");
+#pragma unroll 1
+    for (int m0 = 0; m0 < NST && m0 * 256 < nst; m0 += 8) {
+      float vv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int e = threadIdx.x + (m0 + j) * 256;
+        const int r = e / C2_LW, c = e % C2_LW;
+        const int iy = iy0 + r, ix = ix0 + c;
+        const bool in = e < nst && c < cols && iy >= 0 && iy < a.H && ix >= 0 && ix < a.W;
+        const float v = a.in[in ? ib + iy * a.is0 + ix * a.is1 : ib];
+        vv[j] = __int_as_float(__float_as_int(v) & (in ? -1 : 0));
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int e = threadIdx.x + (m0 + j) * 256;
+        if (e < nst) tile[e] = vv[j];
+      }
+    }
+  } else {
+    for (int e = threadIdx.x; e < nst; e += 256) tile[e] = 0.f;
   }
   __syncthreads();
   float acc[8];

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <complex>
 #include <map>
+#include <type_traits>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -62,6 +63,44 @@ __device__ __forceinline__ void fft_scatter(const FftArgs &a, int64_t base, int6
   else a.out[base + k * a.ostride] = v;
 }
 
+// fft_gather for loops that keep many loads in flight: every load is issued
+// unconditionally (positions past the input's extent, and ok = false, read
+// element 0 of the row and give zero), so a thread's loads carry no control
+// flow between them and issue back to back.  KIND is a.kind, dispatched
+// once outside the loop by with_kind.  Pre: a.in_len >= 1.
+template <int KIND>
+__device__ __forceinline__ float2 fft_load(const FftArgs &a, int64_t base, int64_t i, bool ok) {
+  // the zero fill and the conjugation are bit operations on the loaded
+  // words, not selects: a select between a load and a constant is turned
+  // into a branch around the load, which serialises the loads again
+  auto keep = [](float x, int mask) { return __int_as_float(__float_as_int(x) & mask); };
+  if constexpr (KIND == FFT_C2R) {
+    const bool mirror = i > a.n / 2;
+    const int64_t h = mirror ? a.n - i : i;
+    const bool in = ok && h < a.in_len;
+    const int mk = in ? -1 : 0;
+    const float2 v = a.in[base + (in ? h : 0) * a.istride];
+    const float y = __int_as_float(__float_as_int(v.y) ^ (mirror ? (int)0x80000000 : 0));
+    return make_float2(keep(v.x, mk), keep(y, mk));
+  } else if constexpr (KIND == FFT_R2C) {
+    const bool in = ok && i < a.in_len;
+    const float r = reinterpret_cast<const float *>(a.in)[base + (in ? i : 0) * a.istride];
+    return make_float2(keep(r, in ? -1 : 0), 0.f);
+  } else {
+    const bool in = ok && i < a.in_len;
+    const int mk = in ? -1 : 0;
+    const float2 v = a.in[base + (in ? i : 0) * a.istride];
+    return make_float2(keep(v.x, mk), keep(v.y, mk));
+  }
+}
+
+template <class F>
+__device__ __forceinline__ void with_kind(int kind, F &&body) {
+  if (kind == FFT_C2R) body(std::integral_constant<int, FFT_C2R>{});
+  else if (kind == FFT_R2C) body(std::integral_constant<int, FFT_R2C>{});
+  else body(std::integral_constant<int, FFT_C2C>{});
+}
+
 template <int N>
 struct GenericCfg {
   static constexpr int TB = N >= 4096 ? 1 : 4096 / N;
@@ -86,19 +125,32 @@ __global__ __launch_bounds__(GenericCfg<N>::NT) void fft_generic_kernel(FftArgs 
   }
   __syncthreads();
 
-  // gather
-  for (int e = threadIdx.x; e < TB * N; e += NT) {
-    int t, i;
-    if (a.batch_fast) {
-      t = e % TB;
-      i = e / TB;
-    } else {
-      t = e / N;
-      i = e % N;
+  // gather: each thread's PER elements are loaded before any is stored, so
+  // its loads are in flight together
+  constexpr int PER = TB * N / NT;
+  static_assert(PER * NT == TB * N, "whole elements per thread");
+  {
+    float2 gv[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) gv[m] = make_float2(0.f, 0.f);
+    if (a.in_len > 0)
+      with_kind(a.kind, [&](auto kt) {
+        constexpr int KIND = decltype(kt)::value;
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+          const int e = threadIdx.x + m * NT;
+          const int t = a.batch_fast ? e % TB : e / N;
+          const int i = a.batch_fast ? e / TB : e % N;
+          gv[m] = fft_load<KIND>(a, base_in[t], i, t0 + t < a.nbatch);
+        }
+      });
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int e = threadIdx.x + m * NT;
+      const int t = a.batch_fast ? e % TB : e / N;
+      const int i = a.batch_fast ? e / TB : e % N;
+      lds[LY::addr(t, i)] = gv[m];
     }
-    float2 v = make_float2(0.f, 0.f);
-    if (t0 + t < a.nbatch) v = fft_gather(a, base_in[t], i);
-    lds[LY::addr(t, i)] = v;
   }
   __syncthreads();
 
@@ -201,28 +253,56 @@ __global__ __launch_bounds__(GenericCfg<M>::NT) void czt_kernel(CztArgs z) {
 
   // gather x.c into the first n slots (x conjugated for the inverse; C2R
   // input extended to n first), zeros to M
-  for (int e = threadIdx.x; e < TB * M; e += NT) {
-    int t, i;
-    if (a.batch_fast) {
-      t = e % TB;
-      i = e / TB;
-    } else {
-      t = e / M;
-      i = e % M;
+  // (each thread's loads are issued together, in chunks of CH elements,
+  // before the values are combined and stored)
+  constexpr int PER = TB * M / NT;
+  constexpr int CH = PER < 8 ? PER : 8;
+  static_assert(PER * NT == TB * M && PER % CH == 0, "whole chunks per thread");
+#pragma unroll 1
+  for (int m0 = 0; m0 < PER; m0 += CH) {
+    float2 xv[CH], cv[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) xv[j] = make_float2(0.f, 0.f);
+    if (a.in_len > 0)
+      with_kind(a.kind, [&](auto kt) {
+        constexpr int KIND = decltype(kt)::value;
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+          const int e = threadIdx.x + (m0 + j) * NT;
+          const int t = a.batch_fast ? e % TB : e / M;
+          const int i = a.batch_fast ? e / TB : e % M;
+          xv[j] = fft_load<KIND>(a, base_in[t], i, t0 + t < a.nbatch && i < n);
+        }
+      });
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int e = threadIdx.x + (m0 + j) * NT;
+      const int i = a.batch_fast ? e / TB : e % M;
+      cv[j] = z.chirp[i < n ? i : 0];
     }
-    float2 v = make_float2(0.f, 0.f);
-    if (t0 + t < a.nbatch && i < n) {
-      float2 x = fft_gather(a, base_in[t], i);
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int e = threadIdx.x + (m0 + j) * NT;
+      const int t = a.batch_fast ? e % TB : e / M;
+      const int i = a.batch_fast ? e / TB : e % M;
+      float2 x = xv[j];
       if (INV) x.y = -x.y;
-      v = cmul(x, z.chirp[i]);
+      lds[LY::addr(t, i)] = cmul(x, cv[j]);
     }
-    lds[LY::addr(t, i)] = v;
   }
   __syncthreads();
   lds_passes<LY, SQ, M, NT, TB, false, 0, SQ::n>(lds, a.tw);
-  for (int e = threadIdx.x; e < TB * M; e += NT) {
-    const int t = e / M, k = e % M;
-    lds[LY::addr(t, k)] = cmul(lds[LY::addr(t, k)], z.spec[k]);
+#pragma unroll 1
+  for (int m0 = 0; m0 < PER; m0 += CH) {
+    float2 sv[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) sv[j] = z.spec[(threadIdx.x + (m0 + j) * NT) % M];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int e = threadIdx.x + (m0 + j) * NT;
+      const int t = e / M, k = e % M;
+      lds[LY::addr(t, k)] = cmul(lds[LY::addr(t, k)], sv[j]);
+    }
   }
   __syncthreads();
   lds_passes<LY, SQ, M, NT, TB, true, 0, SQ::n>(lds, a.tw);
@@ -437,57 +517,116 @@ __global__ __launch_bounds__(256) void fft_big_kernel(BigArgs a) {
   batch_offsets(f.bi, b, &base_in, &base_out);
   const int64_t wbase = b * a.n;
 
-  // gather: element i of column c0 + t, lanes over t (consecutive addresses)
-  for (int e = threadIdx.x; e < TB * N; e += NT) {
-    const int t = e % TB, i = e / TB;
-    const int64_t c = c0 + t;
-    float2 v;
-    if constexpr (MODE == BIG_VIEW_TO_WORK) {
-      v = fft_gather(f, base_in, c + a.N1 * i);
-    } else if constexpr (MODE == BIG_CHIRP_TO_WORK) {
-      const int64_t g = c + a.N1 * i;
-      v = make_float2(0.f, 0.f);
-      if (g < a.czt_n) {
-        float2 x = fft_gather(f, base_in, g);
-        if (a.conj_io) x.y = -x.y;
-        v = cmul(x, a.chirp[g]);
+  // gather: element i of column c0 + t, lanes over t (consecutive
+  // addresses); each thread's loads are issued in chunks of CH before the
+  // values are stored
+  constexpr int PER = TB * N / NT;
+  constexpr int CH = PER < 8 ? PER : 8;
+  static_assert(PER * NT == TB * N && PER % CH == 0, "whole chunks per thread");
+#pragma unroll 1
+  for (int m0 = 0; m0 < PER; m0 += CH) {
+    float2 gv[CH];
+    if constexpr (MODE == BIG_VIEW_TO_WORK || MODE == BIG_CHIRP_TO_WORK) {
+      float2 cv[CH];
+#pragma unroll
+      for (int j = 0; j < CH; ++j) gv[j] = make_float2(0.f, 0.f);
+      if (f.in_len > 0)
+        with_kind(f.kind, [&](auto kt) {
+          constexpr int KIND = decltype(kt)::value;
+#pragma unroll
+          for (int j = 0; j < CH; ++j) {
+            const int e = threadIdx.x + (m0 + j) * NT;
+            const int64_t g = c0 + e % TB + a.N1 * (e / TB);
+            gv[j] = fft_load<KIND>(f, base_in, g, MODE == BIG_VIEW_TO_WORK || g < a.czt_n);
+          }
+        });
+      if constexpr (MODE == BIG_CHIRP_TO_WORK) {
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+          const int e = threadIdx.x + (m0 + j) * NT;
+          const int64_t g = c0 + e % TB + a.N1 * (e / TB);
+          cv[j] = a.chirp[g < a.czt_n ? g : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+          float2 x = gv[j];
+          if (a.conj_io) x.y = -x.y;
+          gv[j] = cmul(x, cv[j]);
+        }
       }
-    } else if constexpr (MODE == BIG_WORK_TO_WORK) {
-      v = a.src[wbase + c + a.N1 * i];
-    } else {  // pass 2 reads the work matrix [N1][N2]: row i, column c
-      v = a.src[wbase + i * a.N2 + c];
+    } else {
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        const int e = threadIdx.x + (m0 + j) * NT;
+        const int64_t c = c0 + e % TB;
+        const int i = e / TB;
+        if constexpr (MODE == BIG_WORK_TO_WORK) gv[j] = a.src[wbase + c + a.N1 * i];
+        else gv[j] = a.src[wbase + i * a.N2 + c];  // pass 2: row i, column c of [N1][N2]
+      }
     }
-    lds[LY::addr(t, i)] = v;
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+      const int e = threadIdx.x + (m0 + j) * NT;
+      lds[LY::addr(e % TB, e / TB)] = gv[j];
+    }
   }
   __syncthreads();
   lds_passes<LY, SQ, N, NT, TB, INV, 0, SQ::n>(lds, a.tw_pass);
 
   if constexpr (PASS1) {
-    // row c of the work matrix, times W_n^(c k); lanes over k
-    for (int e = threadIdx.x; e < TB * N; e += NT) {
-      const int t = e / N, k = e % N;
-      const int64_t c = c0 + t;
-      const int64_t m = c * k;
-      float2 w = cmul(a.tw_lo[m & ((1LL << a.lo_bits) - 1)], a.tw_hi[m >> a.lo_bits]);
-      if (INV) w.y = -w.y;
-      a.dst[wbase + c * a.N2 + k] = cmul(lds[LY::addr(t, k)], w);
+    // row c of the work matrix, times W_n^(c k); lanes over k (the table
+    // entries of a chunk are loaded before its products are stored)
+  #pragma unroll 1
+  for (int m0 = 0; m0 < PER; m0 += CH) {
+      float2 lo[CH], hi[CH];
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        const int e = threadIdx.x + (m0 + j) * NT;
+        const int64_t m = (c0 + e / N) * (e % N);
+        lo[j] = a.tw_lo[m & ((1LL << a.lo_bits) - 1)];
+        hi[j] = a.tw_hi[m >> a.lo_bits];
+      }
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        const int e = threadIdx.x + (m0 + j) * NT;
+        const int t = e / N, k = e % N;
+        float2 w = cmul(lo[j], hi[j]);
+        if (INV) w.y = -w.y;
+        a.dst[wbase + (c0 + t) * a.N2 + k] = cmul(lds[LY::addr(t, k)], w);
+      }
     }
   } else {
-    // output index N2 k + c; lanes over t
-    for (int e = threadIdx.x; e < TB * N; e += NT) {
-      const int t = e % TB, k = e / TB;
-      const int64_t kg = a.N2 * k + c0 + t;
-      float2 v = lds[LY::addr(t, k)];
-      if constexpr (MODE == BIG_WORK_SPEC) {
-        a.dst[wbase + kg] = cmul(v, a.spec[kg]);
-      } else if constexpr (MODE == BIG_WORK_CHIRP) {
-        if (kg < f.out_len) {
-          float2 y = cscale(cmul(v, a.chirp[kg]), f.scale);
-          if (a.conj_io) y.y = -y.y;
-          fft_scatter(f, base_out, kg, y);
+    // output index N2 k + c; lanes over t (a chunk's spectrum or chirp
+    // entries are loaded before its outputs are stored)
+  #pragma unroll 1
+  for (int m0 = 0; m0 < PER; m0 += CH) {
+      float2 fac[CH];
+      if constexpr (MODE == BIG_WORK_SPEC || MODE == BIG_WORK_CHIRP) {
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+          const int e = threadIdx.x + (m0 + j) * NT;
+          const int64_t kg = a.N2 * (e / TB) + c0 + e % TB;
+          if constexpr (MODE == BIG_WORK_SPEC) fac[j] = a.spec[kg];
+          else fac[j] = kg < f.out_len ? a.chirp[kg] : make_float2(0.f, 0.f);
         }
-      } else {
-        if (kg < f.out_len) fft_scatter(f, base_out, kg, cscale(v, f.scale));
+      }
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        const int e = threadIdx.x + (m0 + j) * NT;
+        const int t = e % TB, k = e / TB;
+        const int64_t kg = a.N2 * k + c0 + t;
+        float2 v = lds[LY::addr(t, k)];
+        if constexpr (MODE == BIG_WORK_SPEC) {
+          a.dst[wbase + kg] = cmul(v, fac[j]);
+        } else if constexpr (MODE == BIG_WORK_CHIRP) {
+          if (kg < f.out_len) {
+            float2 y = cscale(cmul(v, fac[j]), f.scale);
+            if (a.conj_io) y.y = -y.y;
+            fft_scatter(f, base_out, kg, y);
+          }
+        } else {
+          if (kg < f.out_len) fft_scatter(f, base_out, kg, cscale(v, f.scale));
+        }
       }
     }
   }
