@@ -390,7 +390,9 @@ __global__ __launch_bounds__(DopCfg<ND>::NT) void doppler_adj_kernel(DopGradArgs
 #pragma unroll 8
     for (int idx = threadIdx.x; idx < ND * TR; idx += NT) {
       const int k = idx / TR, r = r0 + idx % TR;
-      if (r < a.R && gpc[(int64_t)k * a.gs_k + (int64_t)r * a.gs_r] != 0.f) nz = 1;
+      // unconditional loads (clamped column) so they issue back to back
+      const float g = gpc[(int64_t)k * a.gs_k + (int64_t)min(r, a.R - 1) * a.gs_r];
+      nz |= (r < a.R) & (g != 0.f);
     }
     if (!__syncthreads_or(nz)) {
       if (gyc)
@@ -413,14 +415,28 @@ __global__ __launch_bounds__(DopCfg<ND>::NT) void doppler_adj_kernel(DopGradArgs
   wg_barrier();
 
   // forward: first pass fed by the windowed canceller, the rest in LDS; the
-  // last forward pass stores gD = 2 gpow D
-  auto store_gd = [&](int t, int k, float2 v) {
-    const int r = r0 + t;
-    const float g = r < a.R ? gpc[(int64_t)k * a.gs_k + (int64_t)r * a.gs_r] : 0.f;
-    lds[LY::addr(t, k)] = cscale(v, 2.0f * g);
+  // last forward pass stores gD = 2 gpow D.  The gpow values of the last
+  // pass's outputs are fetched before its LDS reads (column clamped, zeroed
+  // past R by a mask, so the loads issue back to back).
+  constexpr int RL = SQ::template R<K - 1>;
+  constexpr int LSL = SQ::template LS<K - 1>;
+  using PL = Pass<LY, RL, ND, NT, TR>;
+  float gpv[PL::BPT][RL];
+  auto fetch_gp = [&]() {
+    PL::template outputs<LSL>([&](int u, int q, int t, int k) {
+      const int r = r0 + t;
+      const float g = gpc[(int64_t)k * a.gs_k + (int64_t)min(r, a.R - 1) * a.gs_r];
+      gpv[u][q] = __int_as_float(__float_as_int(g) & (r < a.R ? -1 : 0));
+    });
+  };
+  auto store_gd = [&](PL &pl) {
+    PL::template outputs<LSL>([&](int u, int q, int t, int k) {
+      lds[LY::addr(t, k)] = cscale(pl.v[u][q], 2.0f * gpv[u][q]);
+    });
   };
   constexpr int R0 = SQ::template R<0>;
   Pass<LY, R0, ND, NT, TR> p0;
+  if constexpr (K == 1) fetch_gp();
   p0.load([&](int t, int i) -> float2 {
     if (i >= a.rows_t) return make_float2(0.f, 0.f);
     const int p = i + a.p_off;
@@ -433,18 +449,17 @@ __global__ __launch_bounds__(DopCfg<ND>::NT) void doppler_adj_kernel(DopGradArgs
   wg_barrier();  // every canceller read is done before the tile is overwritten
   p0.template compute<1, false>(nullptr);
   if constexpr (K == 1) {
-    p0.template store<1>(store_gd);
+    store_gd(p0);
   } else {
     p0.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
     wg_barrier();
     lds_passes<LY, SQ, ND, NT, TR, false, 1, K - 1>(lds, a.tw);
-    constexpr int RL = SQ::template R<K - 1>;
-    constexpr int LSL = SQ::template LS<K - 1>;
-    Pass<LY, RL, ND, NT, TR> pl;
+    PL pl;
+    fetch_gp();
     pl.load([&](int t, int i) { return lds[LY::addr(t, i)]; });
     wg_barrier();
     pl.template compute<LSL, false>(a.tw + SQ::template TWOFF<K - 1>);
-    pl.template store<LSL>(store_gd);
+    store_gd(pl);
   }
   wg_barrier();
   // gu = F^H gD: the same passes with conjugate twiddles, unscaled

@@ -395,6 +395,22 @@ struct Pass {
       dft<R, INV>(v[u]);
     }
   }
+  // f(u, q, t, i) for the output positions store<LS> visits, value
+  // v[u][q] going to element i of transform t (to fetch per-output data
+  // ahead of the store, or to store with it)
+  template <int LS, class F>
+  __device__ __forceinline__ static void outputs(F &&f) {
+#pragma unroll
+    for (int u = 0; u < BPT; ++u) {
+      if (!active(u)) continue;
+      int t, j;
+      LY::template bfly<R>(threadIdx.x + u * NT, t, j);
+      const int k = j & (LS - 1);
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) f(u, q, t, base + q * LS);
+    }
+  }
   // store(t, i, value) for the Stockham output positions
   template <int LS, class StoreF>
   __device__ __forceinline__ void store(StoreF &&f) {

@@ -104,6 +104,14 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
 }
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
+// v if keep, else +0: a bit mask rather than a select, so that a load
+// feeding it stays unconditional (the compiler turns a select between a
+// load and a constant into a branch around the load, which serialises a
+// thread's loads)
+__device__ __forceinline__ float2 keep_if(float2 v, bool keep) {
+  const int m = keep ? -1 : 0;
+  return make_float2(__int_as_float(__float_as_int(v.x) & m), __int_as_float(__float_as_int(v.y) & m));
+}
 
 // Hamming window evaluated in float as MatX's generator does
 // (include/matx/generators/hamming.h:96-99).

@@ -83,7 +83,7 @@ __device__ __forceinline__ void ols_block(const OlsBlock &ob, const float2 *twf,
     ps.load([&](int, int i) {
       int gi = lo + i;
       if (gi >= n32) gi -= n32;
-      return gi < seff32 ? xr[gi] : make_float2(0.f, 0.f);
+      return keep_if(xr[gi < seff32 ? gi : 0], gi < seff32);
     });
     ps.template compute<1, false>(nullptr);
     ps.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
@@ -147,7 +147,9 @@ __device__ __forceinline__ void ols_multi(const OlsBlock *obs, const float2 *twf
       const OlsBlock &ob = obs[t];
       int gi = ob.lo + i;
       if (gi >= ob.n) gi -= ob.n;
-      return gi < ob.seff ? ob.xr[gi] : make_float2(0.f, 0.f);
+      // sample 0 of the row (an idle transform's row is a real one) and a
+      // mask past seff: unconditional loads
+      return keep_if(ob.xr[gi < ob.seff ? gi : 0], gi < ob.seff);
     });
     ps.template compute<1, false>(nullptr);
     ps.template store<1>([&](int t, int i, float2 v) { lds[LY::addr(t, i)] = v; });
@@ -620,7 +622,7 @@ __device__ __forceinline__ void wfspec_body(const WfArgs &a, float2 *lds, double
 #pragma unroll
   for (int m = 0; m < 16; ++m) {
     const int i = tid + m * NT;
-    p0.v[0][m] = i < a.L ? a.w[(int64_t)i * a.wstride] : make_float2(0.f, 0.f);
+    p0.v[0][m] = keep_if(a.w[(int64_t)(i < a.L ? i : 0) * a.wstride], i < a.L);
   }
 #pragma unroll
   for (int m = 0; m < 16; ++m) {
