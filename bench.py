@@ -71,7 +71,7 @@ def parse():
                          "FFTs; ew: fused elementwise expressions at the radar cube's size")
     ap.add_argument("--streams", type=int, default=None,
                     help="pipelines run side by side as branches of the timed graph of the "
-                         "radar workloads, each on its own HIP stream (default 3 for "
+                         "radar workloads, each on its own HIP stream (default 6 for "
                          "16x128x9000, 1 for 64x512x32768)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 5 plain 512 MiB device copies (scale reference for the "
@@ -952,7 +952,7 @@ def main():
 # window with 2 x 1 guard cells around the cell under test; large: 21 x 7.
 WORKLOADS = {
     "radar": dict(C=16, P=128, S=9000, L=1000, nd=128, bw=0.4, win=(5, 13), guard=(3, 5),
-                  targets=20, snr=(0.0, 20.0), steps=None, cubes=4, streams=3,
+                  targets=20, snr=(0.0, 20.0), steps=None, cubes=4, streams=6,
                   tag="radar16x128x9000",
                   metric="cubes/s (16x128x9000 c64 radar pipeline)",
                   desc="radar pipeline 16ch x 128 pulses x 9000 samples c64, L=1000 LFM (0.4 fs), "
