@@ -58,8 +58,9 @@ class RaggedBatch:
         return self.pw
 
     def graph(self, stream=None):
-        """Captures one execute() into a CUDA graph (run execute() once on
-        the same stream first); returns the graph, replayed with .replay()."""
+        """Captures one execute() into a CUDA graph; returns the graph,
+        replayed with .replay().  The plan builds its tables and streams when
+        it is created, so the first execute can be the captured one."""
         s = stream or torch.cuda.current_stream()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=s):

@@ -101,6 +101,55 @@ def test_ragged_graph_replays(mx):
     rb.close()
 
 
+def test_ragged_capture_without_warmup(mx):
+    """The plan builds its tables, spectra buffers and branch streams when it
+    is created, so its very first execute can be captured into a graph with
+    no eager run before it; every replay matches the per-job entries."""
+    jobs = make_jobs(4, 5)
+    s = torch.cuda.Stream()
+    rb = mx.ragged.RaggedBatch(jobs, autotune=False, stream=s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        rb.execute()
+    refs = [plain(mx, x, w) for x, w in jobs]
+    torch.cuda.synchronize()
+    for k in range(2):
+        for p in rb.pw:
+            p.fill_(-1.0)
+        with torch.cuda.stream(s):
+            g.replay()
+        s.synchronize()
+        for i, ref in enumerate(refs):
+            r = float((rb.pw[i] - ref).norm() / ref.norm())
+            assert r < RTOL, (k, i, r)
+    rb.close()
+
+
+def test_plans_survive_clear_caches(mx):
+    """mxr_clear_caches frees the library's cached tables, spectra and
+    workspaces but keeps what live plans point at: a pulse-compression plan
+    and a ragged plan give the same result before and after the clear."""
+    jobs = make_jobs(6, 3)
+    x, w = jobs[0]
+    C, P, S = x.shape
+    plan = mx.ops.PulseCompressionPlan(S, w.shape[0], mx.ragged.next_pow2(S))
+    plan.set_waveform(w)
+    rb = mx.ragged.RaggedBatch(jobs, autotune=False)
+    y0 = plan(x).clone()
+    p0 = [p.clone() for p in rb.execute()]
+    torch.cuda.synchronize()
+    mx.lib().mxr_clear_caches()
+    y1 = plan(x)
+    p1 = rb.execute()
+    torch.cuda.synchronize()
+    assert torch.equal(y0, y1)
+    for a, b in zip(p0, p1):
+        assert torch.equal(a, b)
+    plan.close()
+    rb.close()
+
+
 def config3_jobs(seed, n=256):
     """n jobs from the ragged workload's distribution: C uniform in {1..8},
     P in {16, 32, 64, 128, 256}, S uniform in [1024, 16384], L uniform in
