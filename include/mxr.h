@@ -359,10 +359,13 @@ typedef struct mxr_ragged_plan_s *mxr_ragged_plan;
 mxr_status mxr_ragged_plan_create(mxr_ragged_plan *plan, const mxr_ragged_job *jobs, int njobs,
                                   int flags, hipStream_t stream);
 /* Enqueues every job: waveform spectra, pulse compression, Doppler.  The
- * size classes of each stage run concurrently on the plan's own helper
- * streams, forked from and joined back into `stream` with events (parallel
- * branches when captured).  No allocation and no host synchronisation:
- * capturable in a hipGraph.  A plan executes on one stream at a time. */
+ * jobs of one Doppler size form a chain (their block-size classes' spectrum
+ * and pulse-compression launches, then their Doppler launch); chains run
+ * concurrently on the plan's own helper streams, forked from and joined back
+ * into `stream` with events (parallel branches when captured), so one
+ * chain's Doppler pass overlaps another's pulse compression.  No allocation
+ * and no host synchronisation: capturable in a hipGraph, also on the first
+ * execute.  A plan executes on one stream at a time. */
 mxr_status mxr_ragged_execute(mxr_ragged_plan plan, hipStream_t stream);
 /* The block size chosen for job `job`. */
 mxr_status mxr_ragged_plan_block_size(mxr_ragged_plan plan, int job, int *M);

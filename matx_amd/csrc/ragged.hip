@@ -37,6 +37,7 @@ int64_t next_pow2(int64_t v) {
 
 struct PcClass {
   int M = 0;
+  int nd = 0;  // Doppler size of its jobs (the chain it belongs to)
   int njobs = 0;
   int64_t items = 0;
   int grid = 0;
@@ -54,11 +55,14 @@ struct DopClass {
   const int *d_item_job = nullptr;
 };
 
-// Classes of one stage run concurrently on helper streams: the caller's
-// stream records a fork event, each class's launches wait on it from their
-// own stream and record a join event, and the caller's stream waits on every
-// join (the same fork/join, captured into a graph, becomes parallel
-// branches).  All at default priority.
+// The jobs of one Doppler size form a chain: the spectrum and pulse-
+// compression launches of their block-size classes, then their Doppler
+// launch.  Chains are independent, so they run concurrently on helper
+// streams: the caller's stream records a fork event, each helper stream
+// waits on it, runs its chains in turn and records a join event, and the
+// caller's stream waits on every join (the same fork/join, captured into a
+// graph, becomes parallel branches).  One chain's Doppler pass thus overlaps
+// another chain's pulse compression.  All at default priority.
 constexpr int RAGGED_BRANCHES = 4;
 
 struct RaggedPlanObj {
@@ -70,6 +74,8 @@ struct RaggedPlanObj {
   void *tables = nullptr;  // device copies of the job tables
   std::vector<PcClass> pc;
   std::vector<DopClass> dop;
+  std::vector<int> dop_order;   // chains by decreasing work
+  std::vector<int> dop_branch;  // helper stream of each chain
   hipStream_t branch[RAGGED_BRANCHES] = {};
   hipEvent_t fork[2] = {}, join[2][RAGGED_BRANCHES] = {};
 };
@@ -260,15 +266,20 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
     const int nd = (int)next_pow2(jobs[i].P);
     if (std::find(nds.begin(), nds.end(), nd) == nds.end()) nds.push_back(nd);
   }
+  for (int nd : nds)
   for (int M : Ms) {
     if (st) break;
+    bool any = false;
+    for (int i = 0; i < njobs && !any; ++i) any = p->M[i] == M && (int)next_pow2(jobs[i].P) == nd;
+    if (!any) continue;
     PcClass cl;
     cl.M = M;
+    cl.nd = nd;
     const size_t first = hpc.size();
     if ((st = pc_tables_prepare(M, fn))) break;
     const float2 *tw = twiddles_device(M, false);
     for (int i = 0; i < njobs; ++i) {
-      if (p->M[i] != M) continue;
+      if (p->M[i] != M || (int)next_pow2(jobs[i].P) != nd) continue;
       const mxr_ragged_job &j = jobs[i];
       const int64_t n = next_pow2(j.S), R = j.S - j.L + 1;
       int Mp;
@@ -355,6 +366,29 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
     dop_ranges.push_back({first, hdop.size()});
     p->dop.push_back(cl);
   }
+  // chains to helper streams, largest first onto the least loaded stream
+  // (work ~ samples read by the chain's pulse compression)
+  if (!st) {
+    std::vector<double> cost(p->dop.size(), 0.0);
+    for (int i = 0; i < njobs; ++i) {
+      const int nd = (int)next_pow2(jobs[i].P);
+      for (size_t d = 0; d < p->dop.size(); ++d)
+        if (p->dop[d].nd == nd) cost[d] += (double)jobs[i].C * jobs[i].P * jobs[i].S;
+    }
+    p->dop_order.resize(p->dop.size());
+    for (size_t d = 0; d < p->dop.size(); ++d) p->dop_order[d] = (int)d;
+    std::stable_sort(p->dop_order.begin(), p->dop_order.end(),
+                     [&](int a, int b) { return cost[a] > cost[b]; });
+    p->dop_branch.assign(p->dop.size(), 0);
+    double load[RAGGED_BRANCHES] = {};
+    for (int d : p->dop_order) {
+      int b = 0;
+      for (int k = 1; k < RAGGED_BRANCHES; ++k)
+        if (load[k] < load[b]) b = k;
+      p->dop_branch[d] = b;
+      load[b] += cost[d];
+    }
+  }
   // item -> job tables (job index within its class), classes back to back
   std::vector<int> hitem;
   std::vector<size_t> pc_item0, dop_item0;
@@ -432,32 +466,29 @@ static mxr_status ragged_execute(mxr_ragged_plan h, hipStream_t s) {
   MXR_REQUIRE(device_id() == p->dev, MXR_INVALID_PARAMETER,
               "ragged_execute: plan was made on device %d, current device is %d", p->dev,
               device_id());
+  // fork into the helper streams that hold a chain, run every chain on its
+  // stream (its pulse-compression classes, then its Doppler pass), join
+  int nb = 0;
+  for (int b : p->dop_branch) nb = std::max(nb, b + 1);
+  MXR_HIP_TRY(hipEventRecord(p->fork[0], s));
+  for (int b = 0; b < nb; ++b) MXR_HIP_TRY(hipStreamWaitEvent(p->branch[b], p->fork[0], 0));
   mxr_status st = MXR_SUCCESS;
-  // stage k: its classes spread over the branches (class i on branch
-  // i mod RAGGED_BRANCHES), then joined back into s
-  auto stage = [&](int k, size_t nclass, auto &&launch) -> mxr_status {
-    if (nclass == 0) return MXR_SUCCESS;
-    const int nb = (int)std::min<size_t>(nclass, RAGGED_BRANCHES);
-    MXR_HIP_TRY(hipEventRecord(p->fork[k], s));
-    for (int b = 0; b < nb; ++b) MXR_HIP_TRY(hipStreamWaitEvent(p->branch[b], p->fork[k], 0));
-    mxr_status r = MXR_SUCCESS;
-    for (size_t i = 0; i < nclass && !r; ++i) r = launch(i, p->branch[i % RAGGED_BRANCHES]);
-    for (int b = 0; b < nb; ++b) {
-      MXR_HIP_TRY(hipEventRecord(p->join[k][b], p->branch[b]));
-      MXR_HIP_TRY(hipStreamWaitEvent(s, p->join[k][b], 0));
+  for (size_t k = 0; k < p->dop_order.size() && !st; ++k) {
+    const int d = p->dop_order[k];
+    const DopClass &dc = p->dop[d];
+    hipStream_t bs = p->branch[p->dop_branch[d]];
+    for (const PcClass &c : p->pc) {
+      if (c.nd != dc.nd || st) continue;
+      st = launch_wfspec_batch(c.M, c.d_wf, c.njobs, bs);
+      if (!st) st = launch_pc_batch(c.M, c.d_jobs, c.d_item_job, c.items, c.grid, bs);
     }
-    return r;
-  };
-  st = stage(0, p->pc.size(), [&](size_t i, hipStream_t bs) -> mxr_status {
-    const PcClass &c = p->pc[i];
-    mxr_status r = launch_wfspec_batch(c.M, c.d_wf, c.njobs, bs);
-    return r ? r : launch_pc_batch(c.M, c.d_jobs, c.d_item_job, c.items, c.grid, bs);
-  });
-  if (st) return st;
-  return stage(1, p->dop.size(), [&](size_t i, hipStream_t bs) {
-    const DopClass &c = p->dop[i];
-    return launch_doppler_batch(c.nd, c.d_jobs, c.d_item_job, c.items, c.grid, bs);
-  });
+    if (!st) st = launch_doppler_batch(dc.nd, dc.d_jobs, dc.d_item_job, dc.items, dc.grid, bs);
+  }
+  for (int b = 0; b < nb; ++b) {
+    MXR_HIP_TRY(hipEventRecord(p->join[0][b], p->branch[b]));
+    MXR_HIP_TRY(hipStreamWaitEvent(s, p->join[0][b], 0));
+  }
+  return st;
 }
 
 static mxr_status ragged_block_size(mxr_ragged_plan h, int job, int *M) {
