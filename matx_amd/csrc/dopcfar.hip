@@ -22,8 +22,8 @@
 //     t) at i*TB + t).
 //  2. Stockham passes down the LDS columns (lane = column: unit-stride LDS);
 //     the last pass writes |X|^2 over the first half of the tile.
-//  3. CFAR: a thread owns 4 adjacent output columns on every KS-th Doppler
-//     row.  It reads each of its power rows with 16-byte LDS loads and forms
+//  3. CFAR: a thread owns 4 adjacent output columns on every 32nd Doppler
+//     row (16 lanes per row).  It reads each of its power rows with 16-byte LDS loads and forms
 //     the positive partial sums L (left of the hole), M (hole columns), Rt
 //     (right of the hole) of its 4 columns once per row; lr = L + Rt and
 //     full = lr + M go back to LDS, and each cell's noise sum adds full over
@@ -220,14 +220,19 @@ void dopcfar_kernel(DcArgs a) {
   //      hole's Doppler extent) or lr (rows inside it), then the threshold.
   // The cell under test and the row's own lr stay in registers in between.
   constexpr int NQO = OUT / 4;                // output column quads per row
-  constexpr int KS = NT / NQO;                // row step between a thread's rows
+  // 16 lanes per row (quads q < NQO active): the lanes of one 16-lane LDS
+  // access group read one row, so their 16-byte reads never collide in a
+  // bank (rows of different lanes 256 bytes apart would)
+  constexpr int GQ = 16;
+  static_assert(NQO <= GQ, "output quads per row");
+  constexpr int KS = NT / GQ;                 // row step between a thread's rows
   constexpr int QIT = (ND + KS - 1) / KS;     // rows per thread (at most)
   constexpr int NQ = 1 + 2 * HXA / 4;         // power quads read per row segment
   constexpr int NV = 4 * NQ;
   constexpr int FS = OUT;                     // full-sum row stride (floats)
   float *const fsum = reinterpret_cast<float *>(lds) + (ND + 2 * HY) * PS + HY * FS;  // row 0
-  const int q = threadIdx.x % NQO, kb = threadIdx.x / NQO;
-  const bool act = kb < KS;                   // the last NT % NQO threads idle
+  const int q = threadIdx.x % GQ, kb = threadIdx.x / GQ;
+  const bool act = q < NQO;
   const int tc0 = HXA + 4 * q;                // first tile column of the quad
   float cut[QIT][4], lrk[QIT][4];
   // zero rows of the full sums (the CFAR's zero padding along Doppler)
