@@ -49,14 +49,28 @@ __device__ __forceinline__ void row_fft(float2 *buf, const float2 *tw) {
   lds_passes<LY, SQ, M, XcCfg<M>::NT, 1, INV, 0, SQ::n>(buf, tw);
 }
 
+// Row b of the operand into LDS, zero past len.  A thread's 16 loads are
+// issued together: unconditional loads (element 0 past len, zeroed by a mask)
+// before any LDS store.
 template <int M>
 __device__ __forceinline__ void load_row(float2 *buf, const RowOp &op, int64_t row) {
   using LY = Layout<M, 1, false>;
+  constexpr int NT = XcCfg<M>::NT, PER = M / NT;
   int64_t off, unused;
   batch_offsets(op.bi, row, &off, &unused);
   const float2 *src = op.p + off;
-  for (int i = threadIdx.x; i < M; i += XcCfg<M>::NT)
-    buf[LY::addr(0, i)] = i < op.len ? src[(int64_t)i * op.s] : make_float2(0.f, 0.f);
+  float2 v[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) v[m] = make_float2(0.f, 0.f);
+  if (op.len > 0) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int i = threadIdx.x + m * NT;
+      v[m] = keep_if(src[(int64_t)(i < op.len ? i : 0) * op.s], i < op.len);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < PER; ++m) buf[LY::addr(0, threadIdx.x + m * NT)] = v[m];
 }
 
 template <int M>
