@@ -77,7 +77,7 @@ struct RaggedPlanObj {
   std::vector<int> dop_order;   // chains by decreasing work
   std::vector<int> dop_branch;  // helper stream of each chain
   hipStream_t branch[RAGGED_BRANCHES] = {};
-  hipEvent_t fork[2] = {}, join[2][RAGGED_BRANCHES] = {};
+  hipEvent_t fork = {}, join[RAGGED_BRANCHES] = {};
 };
 
 std::mutex g_tune_mu;
@@ -189,11 +189,9 @@ void free_plan(RaggedPlanObj *p) {
   (void)hipFree(p->tables);
   for (int b = 0; b < RAGGED_BRANCHES; ++b) {
     if (p->branch[b]) (void)hipStreamDestroy(p->branch[b]);
-    for (int k = 0; k < 2; ++k)
-      if (p->join[k][b]) (void)hipEventDestroy(p->join[k][b]);
+    if (p->join[b]) (void)hipEventDestroy(p->join[b]);
   }
-  for (int k = 0; k < 2; ++k)
-    if (p->fork[k]) (void)hipEventDestroy(p->fork[k]);
+  if (p->fork) (void)hipEventDestroy(p->fork);
   (void)hipSetDevice(cur);
   delete p;
   plan_tables_release();
@@ -440,17 +438,14 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
   }
   for (int b = 0; b < RAGGED_BRANCHES && !st; ++b) {
     if (hipStreamCreateWithFlags(&p->branch[b], hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&p->join[0][b], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&p->join[1][b], hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&p->join[b], hipEventDisableTiming) != hipSuccess) {
       set_error(MXR_HIP_ERROR, "%s: branch stream creation failed", fn);
       st = MXR_HIP_ERROR;
     }
   }
-  for (int k = 0; k < 2 && !st; ++k) {
-    if (hipEventCreateWithFlags(&p->fork[k], hipEventDisableTiming) != hipSuccess) {
-      set_error(MXR_HIP_ERROR, "%s: event creation failed", fn);
-      st = MXR_HIP_ERROR;
-    }
+  if (!st && hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) != hipSuccess) {
+    set_error(MXR_HIP_ERROR, "%s: event creation failed", fn);
+    st = MXR_HIP_ERROR;
   }
   if (st) {
     free_plan(p);
@@ -470,8 +465,8 @@ static mxr_status ragged_execute(mxr_ragged_plan h, hipStream_t s) {
   // stream (its pulse-compression classes, then its Doppler pass), join
   int nb = 0;
   for (int b : p->dop_branch) nb = std::max(nb, b + 1);
-  MXR_HIP_TRY(hipEventRecord(p->fork[0], s));
-  for (int b = 0; b < nb; ++b) MXR_HIP_TRY(hipStreamWaitEvent(p->branch[b], p->fork[0], 0));
+  MXR_HIP_TRY(hipEventRecord(p->fork, s));
+  for (int b = 0; b < nb; ++b) MXR_HIP_TRY(hipStreamWaitEvent(p->branch[b], p->fork, 0));
   mxr_status st = MXR_SUCCESS;
   for (size_t k = 0; k < p->dop_order.size() && !st; ++k) {
     const int d = p->dop_order[k];
@@ -485,8 +480,8 @@ static mxr_status ragged_execute(mxr_ragged_plan h, hipStream_t s) {
     if (!st) st = launch_doppler_batch(dc.nd, dc.d_jobs, dc.d_item_job, dc.items, dc.grid, bs);
   }
   for (int b = 0; b < nb; ++b) {
-    MXR_HIP_TRY(hipEventRecord(p->join[0][b], p->branch[b]));
-    MXR_HIP_TRY(hipStreamWaitEvent(s, p->join[0][b], 0));
+    MXR_HIP_TRY(hipEventRecord(p->join[b], p->branch[b]));
+    MXR_HIP_TRY(hipStreamWaitEvent(s, p->join[b], 0));
   }
   return st;
 }
