@@ -100,8 +100,13 @@ __global__ __launch_bounds__(256) void conv_kernel(ConvArgs a) {
 // tile of one batch index.  It stages the input rows oy0 - dy ..
 // oy0 + 31 + fh - 1 - dy and columns ox0 - dx .. ox0 + 63 + fw - 1 - dx
 // (zero outside the map) in LDS once; thread (tx, ty) then accumulates the
-// 8 outputs of rows 8 ty .. 8 ty + 7 of column tx, reading each tap's input
-// from LDS and the tap weight (uniform) through the scalar cache.
+// 8 outputs of rows 8 ty .. 8 ty + 7 of column tx.  The filter height FH is
+// a template parameter so that, per filter column, each of the 8 + FH - 1
+// staged rows the thread touches is read from LDS once and feeds every
+// (output row, filter row) pair it belongs to: 19.5 LDS reads per output for
+// the 5 x 13 CFAR mask instead of one per tap (65), which leaves the kernel
+// bound by its FMAs.  Tap weights are uniform and come through the scalar
+// cache.
 constexpr int C2_TH = 32, C2_TW = 64, C2_FHMAX = 16, C2_FWMAX = 64;
 constexpr int C2_LW = C2_TW + C2_FWMAX;  // staged row pitch (floats)
 
@@ -118,6 +123,7 @@ struct Conv2TileArgs {
   int tiles_x;
 };
 
+template <int FH>
 __global__ __launch_bounds__(256) void conv2d_tile_kernel(Conv2TileArgs a) {
   __shared__ float tile[(C2_TH + C2_FHMAX) * C2_LW];
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -132,47 +138,56 @@ __global__ __launch_bounds__(256) void conv2d_tile_kernel(Conv2TileArgs a) {
   }
   const int rows = C2_TH + a.fh - 1, cols = C2_TW + a.fw - 1;
   const int iy0 = oy0 - a.dy, ix0 = ox0 - a.dx;
-  // staged in chunks of 8 loads per thread, all issued before any is
-  // stored; positions outside the map read the batch's first element and
-  // are zeroed with a mask (a select against a load would become a branch
-  // around it and serialise the loads)
-  const int nst = rows * C2_LW;
-  if (a.H > 0 && a.W > 0) {
-    constexpr int NST = (C2_TH + C2_FHMAX) * C2_LW / 256;
-    static_assert(NST % 8 == 0, "whole chunks");
+  // staged column by column: thread t loads column t % 128 of rows
+  // t / 128, t / 128 + 2, ..., in chunks of 8 loads issued before any is
+  // stored; rows outside the map read the column's first row and are zeroed
+  // with a mask (a select against a load would become a branch around it
+  // and serialise the loads); lanes past the staged width stay idle
+  const int sc = threadIdx.x & (C2_LW - 1), sr0 = threadIdx.x / C2_LW;
+  constexpr int RSTEP = 256 / C2_LW;
+  const int ix = ix0 + sc;
+  if (sc < cols && (a.H == 0 || a.W == 0)) {
+    // empty map: nothing to load, the window sums are zero
+    for (int r = sr0; r < rows; r += RSTEP) tile[r * C2_LW + sc] = 0.f;
+  } else if (sc < cols) {
+    const bool col_in = ix >= 0 && ix < a.W;
+    const float *colp = a.in + ib + (col_in ? (int64_t)ix * a.is1 : 0);
+    constexpr int NR = (C2_TH + C2_FHMAX + RSTEP - 1) / RSTEP;
 #pragma unroll 1
-    for (int m0 = 0; m0 < NST && m0 * 256 < nst; m0 += 8) {
+    for (int k0 = 0; k0 < NR && sr0 + k0 * RSTEP < rows; k0 += 8) {
       float vv[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int e = threadIdx.x + (m0 + j) * 256;
-        const int r = e / C2_LW, c = e % C2_LW;
-        const int iy = iy0 + r, ix = ix0 + c;
-        const bool in = e < nst && c < cols && iy >= 0 && iy < a.H && ix >= 0 && ix < a.W;
-        const float v = a.in[in ? ib + iy * a.is0 + ix * a.is1 : ib];
+        const int iy = iy0 + sr0 + (k0 + j) * RSTEP;
+        const bool in = col_in && iy >= 0 && iy < a.H;
+        const float v = colp[in ? (int64_t)iy * a.is0 : 0];
         vv[j] = __int_as_float(__float_as_int(v) & (in ? -1 : 0));
       }
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        const int e = threadIdx.x + (m0 + j) * 256;
-        if (e < nst) tile[e] = vv[j];
+        const int r = sr0 + (k0 + j) * RSTEP;
+        if (r < rows) tile[r * C2_LW + sc] = vv[j];
       }
     }
-  } else {
-    for (int e = threadIdx.x; e < nst; e += 256) tile[e] = 0.f;
   }
   __syncthreads();
   float acc[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) acc[r] = 0.f;
   const float *tb = tile + (8 * ty) * C2_LW + tx;
-  for (int fy = 0; fy < a.fh; ++fy) {
-    for (int fx = 0; fx < a.fw; ++fx) {
-      const float w = a.f[(a.fh - 1 - fy) * a.fs0 + (a.fw - 1 - fx) * a.fs1];
-      const float *t = tb + fy * C2_LW + fx;
+  // out[r] += in[r + fy][fx] * f[FH-1-fy][fw-1-fx]: staged row ry = r + fy
+  for (int fx = 0; fx < a.fw; ++fx) {
+    float w[FH];
 #pragma unroll
-      for (int r = 0; r < 8; ++r) acc[r] = fmaf(t[r * C2_LW], w, acc[r]);
-    }
+    for (int fy = 0; fy < FH; ++fy) w[fy] = a.f[(FH - 1 - fy) * a.fs0 + (a.fw - 1 - fx) * a.fs1];
+    float v[8 + FH - 1];
+#pragma unroll
+    for (int ry = 0; ry < 8 + FH - 1; ++ry) v[ry] = tb[ry * C2_LW + fx];
+#pragma unroll
+    for (int ry = 0; ry < 8 + FH - 1; ++ry)
+#pragma unroll
+      for (int fy = 0; fy < FH; ++fy)
+        if (ry - fy >= 0 && ry - fy < 8) acc[ry - fy] = fmaf(v[ry], w[fy], acc[ry - fy]);
   }
   const int ox = ox0 + tx;
   if (ox >= a.Wo) return;
@@ -181,6 +196,13 @@ __global__ __launch_bounds__(256) void conv2d_tile_kernel(Conv2TileArgs a) {
     const int oy = oy0 + 8 * ty + r;
     if (oy < a.Ho) a.out[ob + oy * a.os0 + ox * a.os1] = acc[r];
   }
+}
+
+// one instantiation per filter height 1 .. C2_FHMAX
+template <int FH>
+static void launch_conv2d_tile(const Conv2TileArgs &t, dim3 grid, hipStream_t s) {
+  if (t.fh == FH) conv2d_tile_kernel<FH><<<grid, dim3(256), 0, s>>>(t);
+  else if constexpr (FH < C2_FHMAX) launch_conv2d_tile<FH + 1>(t, grid, s);
 }
 
 // --------------------------------------------- 1-D along a strided axis
@@ -234,6 +256,77 @@ __global__ __launch_bounds__(256) void conv1d_strip_kernel(Conv1StripArgs a) {
     if (a.out_c) reinterpret_cast<float2 *>(a.out)[ob + o * a.os_c] = acc;
     else reinterpret_cast<float *>(a.out)[ob + o * a.os_c] = acc.x;
   }
+}
+
+// Short filters (fl <= 16), the canceller's case: the strip's whole input
+// window (16 + fl - 1 samples) is loaded into registers first, every load
+// issued before any arithmetic (positions outside the input read its first
+// sample and are zeroed with a mask), then each output is summed from
+// registers.  FLM bounds fl at compile time; taps u >= fl are skipped by a
+// uniform branch.
+template <int FLM, bool IC, bool FC>
+__global__ __launch_bounds__(256) void conv1d_window_kernel(Conv1StripArgs a) {
+  constexpr int NW = C1_STRIP + FLM - 1;
+  const int64_t l = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t o0 = (int64_t)blockIdx.y * C1_STRIP;
+  int64_t ib = 0, ob = 0, rem = blockIdx.z;
+  for (int d = a.nb - 1; d >= 0; --d) {
+    const int64_t i = rem % a.bshape[d];
+    rem /= a.bshape[d];
+    ib += i * a.bis[d];
+    ob += i * a.bos[d];
+  }
+  if (l >= a.n_lane) return;
+  ib += l;
+  ob += l;
+  const int fl = (int)a.fl;
+  // window sample j is input position s0 + j; output o0 + k takes tap t
+  // from window sample k + (fl - 1 - t)
+  const int64_t s0 = o0 + a.off - (fl - 1);
+  float2 xv[NW];
+  bool ok[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) {
+    const int64_t si = s0 + j;
+    ok[j] = j < C1_STRIP + fl - 1 && si >= 0 && si < a.n_in;
+    const int64_t at = ib + (ok[j] ? si : 0) * a.is_c;
+    if constexpr (IC) xv[j] = reinterpret_cast<const float2 *>(a.in)[at];
+    else xv[j] = make_float2(reinterpret_cast<const float *>(a.in)[at], 0.f);
+  }
+#pragma unroll
+  for (int j = 0; j < NW; ++j) xv[j] = keep_if(xv[j], ok[j]);
+  float2 w[FLM];
+#pragma unroll
+  for (int u = 0; u < FLM; ++u) {
+    // window offset u carries tap t = fl - 1 - u
+    const int64_t t = u < fl ? fl - 1 - u : 0;
+    if constexpr (FC) w[u] = reinterpret_cast<const float2 *>(a.f)[t * a.fs];
+    else w[u] = make_float2(reinterpret_cast<const float *>(a.f)[t * a.fs], 0.f);
+  }
+#pragma unroll
+  for (int k = 0; k < C1_STRIP; ++k) {
+    const int64_t o = o0 + k;
+    if (o >= a.n_out) break;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int u = 0; u < FLM; ++u) {
+      if (u < fl) {
+        const float2 x = xv[k + u];
+        if constexpr (FC) acc = make_float2(acc.x + (x.x * w[u].x - x.y * w[u].y), acc.y + (x.x * w[u].y + x.y * w[u].x));
+        else acc = make_float2(acc.x + x.x * w[u].x, acc.y + x.y * w[u].x);
+      }
+    }
+    if (a.out_c) reinterpret_cast<float2 *>(a.out)[ob + o * a.os_c] = acc;
+    else reinterpret_cast<float *>(a.out)[ob + o * a.os_c] = acc.x;
+  }
+}
+
+template <int FLM>
+static void launch_conv1d_window(const Conv1StripArgs &c, dim3 grid, hipStream_t s) {
+  if (c.in_c && c.f_c) conv1d_window_kernel<FLM, true, true><<<grid, dim3(256), 0, s>>>(c);
+  else if (c.in_c) conv1d_window_kernel<FLM, true, false><<<grid, dim3(256), 0, s>>>(c);
+  else if (c.f_c) conv1d_window_kernel<FLM, false, true><<<grid, dim3(256), 0, s>>>(c);
+  else conv1d_window_kernel<FLM, false, false><<<grid, dim3(256), 0, s>>>(c);
 }
 
 static mxr_status conv_common(const char *fn, const mxr_tensor *out, const mxr_tensor *in,
@@ -347,7 +440,7 @@ static mxr_status conv_common(const char *fn, const mxr_tensor *out, const mxr_t
     const int64_t tiles = (int64_t)t.tiles_x * ((t.Ho + C2_TH - 1) / C2_TH);
     MXR_REQUIRE(tiles <= 0x7fffffff && nbatch <= 65535, MXR_INVALID_SIZE, "%s: too large", fn);
     ProfScope ps("conv2d_tile_kernel", stream);
-    conv2d_tile_kernel<<<dim3((unsigned)tiles, (unsigned)nbatch), dim3(256), 0, stream>>>(t);
+    launch_conv2d_tile<1>(t, dim3((unsigned)tiles, (unsigned)nbatch), stream);
     MXR_HIP_TRY(hipGetLastError());
     return MXR_SUCCESS;
   }
@@ -385,9 +478,16 @@ static mxr_status conv_common(const char *fn, const mxr_tensor *out, const mxr_t
       }
       const int64_t lanes = (c.n_lane + 255) / 256;
       MXR_REQUIRE(lanes <= 0x7fffffff, MXR_INVALID_SIZE, "%s: too large", fn);
-      ProfScope ps("conv1d_strip_kernel", stream);
-      conv1d_strip_kernel<<<dim3((unsigned)lanes, (unsigned)strips, (unsigned)nbatch), dim3(256), 0,
-                            stream>>>(c);
+      const dim3 grid((unsigned)lanes, (unsigned)strips, (unsigned)nbatch);
+      if (c.fl <= 16) {
+        ProfScope ps("conv1d_window_kernel", stream);
+        if (c.fl <= 4) launch_conv1d_window<4>(c, grid, stream);
+        else if (c.fl <= 8) launch_conv1d_window<8>(c, grid, stream);
+        else launch_conv1d_window<16>(c, grid, stream);
+      } else {
+        ProfScope ps("conv1d_strip_kernel", stream);
+        conv1d_strip_kernel<<<grid, dim3(256), 0, stream>>>(c);
+      }
       MXR_HIP_TRY(hipGetLastError());
       return MXR_SUCCESS;
     }

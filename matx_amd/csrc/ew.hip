@@ -344,118 +344,135 @@ std::string ewj_source(const mxr_ew_instr *prog, int nprog, const int *idt, int 
     for (int i = 0; i < nin; ++i)
       k << "  io" << i << " += idx[" << d << "] * a.ist[" << i << "][" << d << "];\n";
   }
-  k << "  const long long ncol = a.shape[" << rank - 1 << "];\n"
-    << "#pragma unroll\n"
-    << "  for (int e = 0; e < " << EWJ_COLS / 256 << "; ++e) {\n"
-    << "    const long long c = col0 + threadIdx.x + e * 256;\n"
-    << "    if (c >= ncol) break;\n";
-  std::vector<EwjVal> st;
-  int nv = 0;
-  auto def = [&](bool real, const std::string &expr) {
-    EwjVal v{"v" + std::to_string(nv++), real};
-    k << "    const " << (real ? "float " : "float2 ") << v.name << " = " << expr << ";\n";
-    st.push_back(v);
-  };
-  auto pos = [&](int d) {
-    return d == rank - 1 ? std::string("c") : "idx[" + std::to_string(d) + "]";
-  };
-  for (int pc = 0; pc < nprog; ++pc) {
-    const mxr_ew_instr &ins = prog[pc];
-    switch (ins.op) {
-      case MXR_EW_LOAD: {
-        const int i = ins.a;
-        const std::string off = "io" + std::to_string(i) + " + c * a.ist[" + std::to_string(i) + "][" +
-                                std::to_string(rank - 1) + "]";
-        const std::string p = "a.in[" + std::to_string(i) + "]";
-        if (idt[i] == MXR_C64) def(false, "((const float2 *)" + p + ")[" + off + "]");
-        else if (idt[i] == MXR_F32) def(true, "((const float *)" + p + ")[" + off + "]");
-        else def(true, "(float)((const int *)" + p + ")[" + off + "]");
-        break;
-      }
-      case MXR_EW_CONST: {
-        const std::string c = std::to_string(2 * ins.a);
-        if (creal >> ins.a & 1u) def(true, "a.cst[" + c + "]");
-        else def(false, "mk(a.cst[" + c + "], a.cst[" + c + " + 1])");
-        break;
-      }
-      case MXR_EW_HAMMING: def(true, "ham(" + pos(ins.a) + ", " + std::to_string(ins.b) + ")"); break;
-      case MXR_EW_INDEX: def(true, "(float)" + pos(ins.a)); break;
-      case MXR_EW_SELECT: {
-        const EwjVal c = st.back(); st.pop_back();
-        const EwjVal y = st.back(); st.pop_back();
-        const EwjVal x = st.back(); st.pop_back();
-        if (x.real && y.real) def(true, "(" + ewj_re(c) + " != 0.f) ? " + x.name + " : " + y.name);
-        else def(false, "(" + ewj_re(c) + " != 0.f) ? " + ewj_c(x) + " : " + ewj_c(y));
-        break;
-      }
-      default: {
-        if (ins.op >= MXR_EW_ADD && ins.op <= MXR_EW_MIN) {
+  // The EWJ_COLS / 256 columns of a thread are unrolled in the source:
+  // every operand load of every column is issued first (a column past the
+  // row's end reads the row's last column instead), then the arithmetic,
+  // then the stores of the columns inside the row.  Loads written after a
+  // per-column exit would wait out one memory latency per column.
+  constexpr int NE = EWJ_COLS / 256;
+  k << "  const long long ncol = a.shape[" << rank - 1 << "];\n";
+  for (int e = 0; e < NE; ++e)
+    k << "  const long long c" << e << "r = col0 + threadIdx.x + " << e * 256 << ";\n"
+      << "  const bool ok" << e << " = c" << e << "r < ncol;\n"
+      << "  const long long c" << e << " = ok" << e << " ? c" << e << "r : ncol - 1;\n";
+  auto load_name = [](int e, int pc) { return "l" + std::to_string(e) + "_" + std::to_string(pc); };
+  for (int e = 0; e < NE; ++e)
+    for (int pc = 0; pc < nprog; ++pc) {
+      if (prog[pc].op != MXR_EW_LOAD) continue;
+      const int i = prog[pc].a;
+      const std::string off = "io" + std::to_string(i) + " + c" + std::to_string(e) + " * a.ist[" +
+                              std::to_string(i) + "][" + std::to_string(rank - 1) + "]";
+      const std::string p = "a.in[" + std::to_string(i) + "]", n = load_name(e, pc);
+      if (idt[i] == MXR_C64) k << "  const float2 " << n << " = ((const float2 *)" << p << ")[" << off << "];\n";
+      else if (idt[i] == MXR_F32) k << "  const float " << n << " = ((const float *)" << p << ")[" << off << "];\n";
+      else k << "  const float " << n << " = (float)((const int *)" << p << ")[" << off << "];\n";
+    }
+  for (int e = 0; e < NE; ++e) {
+    const std::string c = "c" + std::to_string(e);
+    std::vector<EwjVal> st;
+    int nv = 0;
+    auto def = [&](bool real, const std::string &expr) {
+      EwjVal v{"v" + std::to_string(e) + "_" + std::to_string(nv++), real};
+      k << "  const " << (real ? "float " : "float2 ") << v.name << " = " << expr << ";\n";
+      st.push_back(v);
+    };
+    auto pos = [&](int d) {
+      return d == rank - 1 ? c : "idx[" + std::to_string(d) + "]";
+    };
+    for (int pc = 0; pc < nprog; ++pc) {
+      const mxr_ew_instr &ins = prog[pc];
+      switch (ins.op) {
+        case MXR_EW_LOAD: {
+          const int i = ins.a;
+          st.push_back(EwjVal{load_name(e, pc), idt[i] != MXR_C64});
+          break;
+        }
+        case MXR_EW_CONST: {
+          const std::string c = std::to_string(2 * ins.a);
+          if (creal >> ins.a & 1u) def(true, "a.cst[" + c + "]");
+          else def(false, "mk(a.cst[" + c + "], a.cst[" + c + " + 1])");
+          break;
+        }
+        case MXR_EW_HAMMING: def(true, "ham(" + pos(ins.a) + ", " + std::to_string(ins.b) + ")"); break;
+        case MXR_EW_INDEX: def(true, "(float)" + pos(ins.a)); break;
+        case MXR_EW_SELECT: {
+          const EwjVal c = st.back(); st.pop_back();
           const EwjVal y = st.back(); st.pop_back();
           const EwjVal x = st.back(); st.pop_back();
-          const bool rr = x.real && y.real;
-          const std::string X = ewj_c(x), Y = ewj_c(y), xr = ewj_re(x), yr = ewj_re(y);
-          switch (ins.op) {
-            case MXR_EW_ADD: rr ? def(true, x.name + " + " + y.name) : def(false, "cadd(" + X + ", " + Y + ")"); break;
-            case MXR_EW_SUB: rr ? def(true, x.name + " - " + y.name) : def(false, "csub(" + X + ", " + Y + ")"); break;
-            case MXR_EW_MUL:
-              if (rr) def(true, x.name + " * " + y.name);
-              else if (x.real) def(false, "mk(" + x.name + " * " + y.name + ".x, " + x.name + " * " + y.name + ".y)");
-              else if (y.real) def(false, "mk(" + x.name + ".x * " + y.name + ", " + x.name + ".y * " + y.name + ")");
-              else def(false, "cmul(" + x.name + ", " + y.name + ")");
-              break;
-            case MXR_EW_DIV:
-              if (rr) def(true, x.name + " / " + y.name);
-              else if (y.real) def(false, "mk(" + X + ".x / " + y.name + ", " + X + ".y / " + y.name + ")");
-              else def(false, "(" + y.name + ".y == 0.f) ? mk(" + X + ".x / " + y.name + ".x, " + X + ".y / " +
-                                   y.name + ".x) : cdiv(" + X + ", " + y.name + ")");
-              break;
-            case MXR_EW_POW: def(true, "powf(" + xr + ", " + yr + ")"); break;
-            case MXR_EW_GT: def(true, "(" + xr + " > " + yr + ") ? 1.f : 0.f"); break;
-            case MXR_EW_LT: def(true, "(" + xr + " < " + yr + ") ? 1.f : 0.f"); break;
-            case MXR_EW_GE: def(true, "(" + xr + " >= " + yr + ") ? 1.f : 0.f"); break;
-            case MXR_EW_LE: def(true, "(" + xr + " <= " + yr + ") ? 1.f : 0.f"); break;
-            case MXR_EW_EQ:
-              if (rr) def(true, "(" + x.name + " == " + y.name + ") ? 1.f : 0.f");
-              else def(true, "(" + X + ".x == " + Y + ".x && " + X + ".y == " + Y + ".y) ? 1.f : 0.f");
-              break;
-            case MXR_EW_MAX: def(true, "fmaxf(" + xr + ", " + yr + ")"); break;
-            default: def(true, "fminf(" + xr + ", " + yr + ")"); break;
-          }
-        } else {
-          const EwjVal x = st.back(); st.pop_back();
-          const std::string X = ewj_c(x), xr = ewj_re(x), n = x.name;
-          switch (ins.op) {
-            case MXR_EW_CONJ: x.real ? def(true, n) : def(false, "mk(" + n + ".x, -" + n + ".y)"); break;
-            case MXR_EW_ABS2: x.real ? def(true, n + " * " + n) : def(true, n + ".x * " + n + ".x + " + n + ".y * " + n + ".y"); break;
-            case MXR_EW_ABS:
-              x.real ? def(true, "sqrtf(" + n + " * " + n + ")")
-                     : def(true, "sqrtf(" + n + ".x * " + n + ".x + " + n + ".y * " + n + ".y)");
-              break;
-            case MXR_EW_SQRT: def(true, "sqrtf(" + xr + ")"); break;
-            case MXR_EW_NEG: x.real ? def(true, "-" + n) : def(false, "mk(-" + n + ".x, -" + n + ".y)"); break;
-            case MXR_EW_REAL: def(true, xr); break;
-            case MXR_EW_IMAG: x.real ? def(true, "0.f") : def(true, n + ".y"); break;
-            case MXR_EW_EXP:
-              if (x.real) def(true, "expf(" + n + ")");
-              else def(false, "mk(expf(" + n + ".x) * cosf(" + n + ".y), expf(" + n + ".x) * sinf(" + n + ".y))");
-              break;
-            case MXR_EW_LOG:
-              def(false, "mk(logf(sqrtf(" + X + ".x * " + X + ".x + " + X + ".y * " + X + ".y)), atan2f(" + X +
-                             ".y, " + X + ".x))");
-              break;
-            case MXR_EW_COS: def(true, "cosf(" + xr + ")"); break;
-            default: def(true, "sinf(" + xr + ")"); break;
+          if (x.real && y.real) def(true, "(" + ewj_re(c) + " != 0.f) ? " + x.name + " : " + y.name);
+          else def(false, "(" + ewj_re(c) + " != 0.f) ? " + ewj_c(x) + " : " + ewj_c(y));
+          break;
+        }
+        default: {
+          if (ins.op >= MXR_EW_ADD && ins.op <= MXR_EW_MIN) {
+            const EwjVal y = st.back(); st.pop_back();
+            const EwjVal x = st.back(); st.pop_back();
+            const bool rr = x.real && y.real;
+            const std::string X = ewj_c(x), Y = ewj_c(y), xr = ewj_re(x), yr = ewj_re(y);
+            switch (ins.op) {
+              case MXR_EW_ADD: rr ? def(true, x.name + " + " + y.name) : def(false, "cadd(" + X + ", " + Y + ")"); break;
+              case MXR_EW_SUB: rr ? def(true, x.name + " - " + y.name) : def(false, "csub(" + X + ", " + Y + ")"); break;
+              case MXR_EW_MUL:
+                if (rr) def(true, x.name + " * " + y.name);
+                else if (x.real) def(false, "mk(" + x.name + " * " + y.name + ".x, " + x.name + " * " + y.name + ".y)");
+                else if (y.real) def(false, "mk(" + x.name + ".x * " + y.name + ", " + x.name + ".y * " + y.name + ")");
+                else def(false, "cmul(" + x.name + ", " + y.name + ")");
+                break;
+              case MXR_EW_DIV:
+                if (rr) def(true, x.name + " / " + y.name);
+                else if (y.real) def(false, "mk(" + X + ".x / " + y.name + ", " + X + ".y / " + y.name + ")");
+                else def(false, "(" + y.name + ".y == 0.f) ? mk(" + X + ".x / " + y.name + ".x, " + X + ".y / " +
+                                     y.name + ".x) : cdiv(" + X + ", " + y.name + ")");
+                break;
+              case MXR_EW_POW: def(true, "powf(" + xr + ", " + yr + ")"); break;
+              case MXR_EW_GT: def(true, "(" + xr + " > " + yr + ") ? 1.f : 0.f"); break;
+              case MXR_EW_LT: def(true, "(" + xr + " < " + yr + ") ? 1.f : 0.f"); break;
+              case MXR_EW_GE: def(true, "(" + xr + " >= " + yr + ") ? 1.f : 0.f"); break;
+              case MXR_EW_LE: def(true, "(" + xr + " <= " + yr + ") ? 1.f : 0.f"); break;
+              case MXR_EW_EQ:
+                if (rr) def(true, "(" + x.name + " == " + y.name + ") ? 1.f : 0.f");
+                else def(true, "(" + X + ".x == " + Y + ".x && " + X + ".y == " + Y + ".y) ? 1.f : 0.f");
+                break;
+              case MXR_EW_MAX: def(true, "fmaxf(" + xr + ", " + yr + ")"); break;
+              default: def(true, "fminf(" + xr + ", " + yr + ")"); break;
+            }
+          } else {
+            const EwjVal x = st.back(); st.pop_back();
+            const std::string X = ewj_c(x), xr = ewj_re(x), n = x.name;
+            switch (ins.op) {
+              case MXR_EW_CONJ: x.real ? def(true, n) : def(false, "mk(" + n + ".x, -" + n + ".y)"); break;
+              case MXR_EW_ABS2: x.real ? def(true, n + " * " + n) : def(true, n + ".x * " + n + ".x + " + n + ".y * " + n + ".y"); break;
+              case MXR_EW_ABS:
+                x.real ? def(true, "sqrtf(" + n + " * " + n + ")")
+                       : def(true, "sqrtf(" + n + ".x * " + n + ".x + " + n + ".y * " + n + ".y)");
+                break;
+              case MXR_EW_SQRT: def(true, "sqrtf(" + xr + ")"); break;
+              case MXR_EW_NEG: x.real ? def(true, "-" + n) : def(false, "mk(-" + n + ".x, -" + n + ".y)"); break;
+              case MXR_EW_REAL: def(true, xr); break;
+              case MXR_EW_IMAG: x.real ? def(true, "0.f") : def(true, n + ".y"); break;
+              case MXR_EW_EXP:
+                if (x.real) def(true, "expf(" + n + ")");
+                else def(false, "mk(expf(" + n + ".x) * cosf(" + n + ".y), expf(" + n + ".x) * sinf(" + n + ".y))");
+                break;
+              case MXR_EW_LOG:
+                def(false, "mk(logf(sqrtf(" + X + ".x * " + X + ".x + " + X + ".y * " + X + ".y)), atan2f(" + X +
+                               ".y, " + X + ".x))");
+                break;
+              case MXR_EW_COS: def(true, "cosf(" + xr + ")"); break;
+              default: def(true, "sinf(" + xr + ")"); break;
+            }
           }
         }
       }
     }
+    const EwjVal v = st.back();
+    const std::string o = "oo + " + c + " * a.ost[" + std::to_string(rank - 1) + "]";
+    k << "  if (ok" << e << ") ";
+    if (odt == MXR_C64) k << "((float2 *)a.out)[" << o << "] = " << ewj_c(v) << ";\n";
+    else if (odt == MXR_F32) k << "((float *)a.out)[" << o << "] = " << ewj_re(v) << ";\n";
+    else k << "((int *)a.out)[" << o << "] = (int)" << ewj_re(v) << ";\n";
   }
-  const EwjVal v = st.back();
-  const std::string o = "oo + c * a.ost[" + std::to_string(rank - 1) + "]";
-  if (odt == MXR_C64) k << "    ((float2 *)a.out)[" << o << "] = " << ewj_c(v) << ";\n";
-  else if (odt == MXR_F32) k << "    ((float *)a.out)[" << o << "] = " << ewj_re(v) << ";\n";
-  else k << "    ((int *)a.out)[" << o << "] = (int)" << ewj_re(v) << ";\n";
-  k << "  }\n}\n";
+  k << "}\n";
   return k.str();
 }
 
