@@ -40,6 +40,7 @@
 #include <initializer_list>
 #include <limits>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -75,9 +76,6 @@ using ::std::sqrt;
 #endif
 #ifndef __MATX_DEVICE__
 #define __MATX_DEVICE__ __device__
-#endif
-#ifndef MATX_LOG_DEBUG
-#define MATX_LOG_DEBUG(...) ((void)0)
 #endif
 
 namespace matx {
@@ -134,6 +132,81 @@ struct dtype_of<int> {
   static constexpr int value = MXR_I32;
 };
 }  // namespace detail
+
+// ------------------------------------------------------- trace ranges, log
+// MatX's NVTX levels and macros (include/matx/core/nvtx.h:48-168) over the
+// library's roctx ranges (mxr_trace_*), and its MATX_LOG_* macros
+// (include/matx/core/log.h:389-401) over the library log (mxr_log_*).  The
+// log macros take MatX's "{}" placeholders, filled in order by operator<<.
+enum matx_nvxtLogLevels {
+  MATX_NVTX_LOG_NONE = 0,
+  MATX_NVTX_LOG_USER = 1,
+  MATX_NVTX_LOG_API = 2,
+  MATX_NVTX_LOG_INTERNAL = 3,
+  MATX_NVTX_LOG_ALL = 0xFFFFFF
+};
+
+namespace detail {
+// A nested range that ends with the scope (MATX_NVTX_START).
+class TraceScope {
+ public:
+  TraceScope(const char *function, const std::string &message = "", int level = MATX_NVTX_LOG_USER)
+      : opened_(mxr_trace_push(message.empty() ? function : message.c_str(), level) != 0) {}
+  TraceScope(const TraceScope &) = delete;
+  TraceScope &operator=(const TraceScope &) = delete;
+  ~TraceScope() {
+    if (opened_) mxr_trace_pop();
+  }
+
+ private:
+  bool opened_;
+};
+
+// A range ended by id (MATX_NVTX_START_RANGE); id < 0 picks a fresh one.
+inline int trace_range_start(const char *function, const std::string &message = "",
+                             int level = MATX_NVTX_LOG_USER, int id = -1) {
+  int i = id;
+  check(mxr_trace_range_start(message.empty() ? function : message.c_str(), level, &i));
+  return i;
+}
+
+inline void log_fill(std::ostringstream &o, const char *f) { o << f; }
+template <typename A, typename... R>
+void log_fill(std::ostringstream &o, const char *f, const A &a, const R &...r) {
+  const char *p = std::strstr(f, "{}");
+  if (!p) {
+    o << f;
+    return;
+  }
+  o.write(f, p - f);
+  o << a;
+  log_fill(o, p + 2, r...);
+}
+template <typename... A>
+void log_at(int level, const char *fmt, const A &...a) {
+  const int cur = mxr_log_get_level();
+  if (cur >= 6 || level < cur) return;
+  std::ostringstream o;
+  log_fill(o, fmt, a...);
+  mxr_log_write(level, o.str().c_str());
+}
+}  // namespace detail
+}  // namespace matx
+
+#define MATX_AMD_CAT2(a, b) a##b
+#define MATX_AMD_CAT(a, b) MATX_AMD_CAT2(a, b)
+#define MATX_NVTX_START(...) ::matx::detail::TraceScope MATX_AMD_CAT(matx_trace_, __COUNTER__)(__FUNCTION__, __VA_ARGS__);
+#define MATX_NVTX_START_RANGE(...) ::matx::detail::trace_range_start(__FUNCTION__, __VA_ARGS__);
+#define MATX_NVTX_END_RANGE(id) ::matx::detail::check(mxr_trace_range_end(id));
+#define MATX_NVTX_SET_LOG_LEVEL(level) ::matx::detail::check(mxr_trace_set_level((int)(level)));
+#define MATX_LOG_TRACE(...) ::matx::detail::log_at(0, __VA_ARGS__)
+#define MATX_LOG_DEBUG(...) ::matx::detail::log_at(1, __VA_ARGS__)
+#define MATX_LOG_INFO(...) ::matx::detail::log_at(2, __VA_ARGS__)
+#define MATX_LOG_WARN(...) ::matx::detail::log_at(3, __VA_ARGS__)
+#define MATX_LOG_ERROR(...) ::matx::detail::log_at(4, __VA_ARGS__)
+#define MATX_LOG_FATAL(...) ::matx::detail::log_at(5, __VA_ARGS__)
+
+namespace matx {
 
 // ---------------------------------------------------------------- executor
 class HIPExecutor {

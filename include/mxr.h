@@ -92,6 +92,42 @@ mxr_status mxr_profile_enable(int on);
 mxr_status mxr_profile_reset(void);
 mxr_status mxr_profile_query(const char *name, int64_t *count, double *total_ms);
 
+/* Library log (MatX's logger, include/matx/core/log.h:110-401: MATX_LOG_LEVEL
+ * / MATX_LOG_DEST).  Levels TRACE 0, DEBUG 1, INFO 2, WARN 3, ERROR 4,
+ * FATAL 5, OFF 6; start-up values from MXR_LOG_LEVEL (a name or a number)
+ * and MXR_LOG_DEST ("stderr", "stdout" or a file path, appended).  Default:
+ * OFF, stderr (MatX: ERROR, stdout) -- failures are returned as statuses
+ * already, and stdout belongs to the caller.  When enabled, every failed
+ * call logs its message at ERROR, plan and cache events at DEBUG and every
+ * kernel launch at TRACE.  mxr_log_write adds a caller message. */
+mxr_status mxr_log_set_level(int level);
+int mxr_log_get_level(void);
+mxr_status mxr_log_set_dest(const char *dest);
+void mxr_log_write(int level, const char *message);
+
+/* Trace ranges (MatX's NVTX ranges, include/matx/core/nvtx.h:48-168, emitted
+ * here through roctx for rocprofv3 --marker-trace).  Levels are MatX's
+ * matx_nvxtLogLevels; a range is emitted when its level is at most the trace
+ * level (default API, as nvtx.h:93; MXR_TRACE_LEVEL overrides it: a number
+ * or NONE / USER / API / INTERNAL / ALL).  The library opens an INTERNAL
+ * range around every kernel launch, named after the kernel.  roctx is loaded
+ * at the first range that passes the filter; without it ranges are no-ops
+ * (mxr_trace_available says which).
+ * mxr_trace_push / mxr_trace_pop: a nested range (MATX_NVTX_START); push
+ * returns 1 when it opened one, which the caller then pops.
+ * mxr_trace_range_start / _end: a range ended by id from anywhere
+ * (MATX_NVTX_START_RANGE / MATX_NVTX_END_RANGE, nvtx.h:122-134); *id < 0
+ * asks for a fresh id, written back.  Starting an id that is open is an
+ * error; ending an id that is not open does nothing. */
+enum { MXR_TRACE_NONE = 0, MXR_TRACE_USER = 1, MXR_TRACE_API = 2, MXR_TRACE_INTERNAL = 3, MXR_TRACE_ALL = 0xFFFFFF };
+mxr_status mxr_trace_set_level(int level);
+int mxr_trace_get_level(void);
+int mxr_trace_available(void);
+int mxr_trace_push(const char *name, int level);
+void mxr_trace_pop(void);
+mxr_status mxr_trace_range_start(const char *name, int level, int *id);
+mxr_status mxr_trace_range_end(int id);
+
 /* DLPack interop (MatX: tensor_t::ToDlPack / make_tensor(DLManagedTensor*),
  * include/matx/core/tensor.h:1471-1555, make_tensor.h:1136-1190).
  * mxr_dl_tensor has the memory layout of DLPack's DLTensor (data, device

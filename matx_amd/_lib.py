@@ -31,7 +31,10 @@ EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_ena
            "mxr_fft_conv_adjoint", "mxr_autocorr_adjoint", "mxr_doppler_taps", "mxr_doppler_taps_adjoint",
            "mxr_ragged_plan_create", "mxr_ragged_execute", "mxr_ragged_plan_block_size",
            "mxr_ragged_plan_destroy", "mxr_doppler_adjoint", "mxr_pc_plan_waveform_grad",
-           "mxr_isl", "mxr_isl_adjoint", "mxr_psl", "mxr_psl_adjoint")
+           "mxr_isl", "mxr_isl_adjoint", "mxr_psl", "mxr_psl_adjoint",
+           "mxr_log_set_level", "mxr_log_get_level", "mxr_log_set_dest", "mxr_log_write",
+           "mxr_trace_set_level", "mxr_trace_get_level", "mxr_trace_available", "mxr_trace_push",
+           "mxr_trace_pop", "mxr_trace_range_start", "mxr_trace_range_end")
 
 STATUS_NAMES = {0: "SUCCESS", 1: "NOT_SUPPORTED", 2: "INVALID_PARAMETER",
                 3: "INVALID_DIM", 4: "INVALID_SIZE", 5: "OUT_OF_MEMORY",
@@ -118,6 +121,28 @@ def lib():
     L.mxr_profile_query.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
                                     ctypes.POINTER(ctypes.c_double)]
     L.mxr_profile_query.restype = I
+    L.mxr_log_set_level.argtypes = [I]
+    L.mxr_log_set_level.restype = I
+    L.mxr_log_get_level.argtypes = []
+    L.mxr_log_get_level.restype = I
+    L.mxr_log_set_dest.argtypes = [ctypes.c_char_p]
+    L.mxr_log_set_dest.restype = I
+    L.mxr_log_write.argtypes = [I, ctypes.c_char_p]
+    L.mxr_log_write.restype = None
+    L.mxr_trace_set_level.argtypes = [I]
+    L.mxr_trace_set_level.restype = I
+    L.mxr_trace_get_level.argtypes = []
+    L.mxr_trace_get_level.restype = I
+    L.mxr_trace_available.argtypes = []
+    L.mxr_trace_available.restype = I
+    L.mxr_trace_push.argtypes = [ctypes.c_char_p, I]
+    L.mxr_trace_push.restype = I
+    L.mxr_trace_pop.argtypes = []
+    L.mxr_trace_pop.restype = None
+    L.mxr_trace_range_start.argtypes = [ctypes.c_char_p, I, ctypes.POINTER(I)]
+    L.mxr_trace_range_start.restype = I
+    L.mxr_trace_range_end.argtypes = [I]
+    L.mxr_trace_range_end.restype = I
     sigs = {
         "mxr_tensor_from_dlpack": [T, V],
         "mxr_tensor_to_dlpack": [T, ctypes.c_int32, ctypes.c_int32, RELEASE_FN, V,

@@ -13,6 +13,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <chrono>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -20,6 +21,7 @@
 #include <vector>
 
 #include "mxr_cache.hpp"
+#include "mxr_trace.hpp"
 #include "mxr_internal.hpp"
 
 namespace mxr {
@@ -528,11 +530,16 @@ static hipFunction_t ewj_kernel(const mxr_ew_instr *prog, int nprog, const int *
   std::lock_guard<std::mutex> g(g_ewj_mu);
   EwjKernel &kk = g_ewj[{device_id(), key.str()}];
   if (kk.fn || kk.failed) return kk.fn;
+  const auto t0 = std::chrono::steady_clock::now();
   const std::vector<char> code = ewj_compile(ewj_source(prog, nprog, idt, nin, odt, rank, creal));
   if (code.empty() || hipModuleLoadData(&kk.mod, code.data()) != hipSuccess ||
       hipModuleGetFunction(&kk.fn, kk.mod, "mxr_ew_jit") != hipSuccess) {
     kk.fn = nullptr;
     kk.failed = true;
+    log_msg(LOG_WARN, "ew: generated kernel failed to build; the interpreter runs [%s]", key.str().c_str());
+  } else {
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    log_msg(LOG_DEBUG, "ew: compiled a generated kernel in %.1f ms [%s]", ms, key.str().c_str());
   }
   return kk.fn;
 }

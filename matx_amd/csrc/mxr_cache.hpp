@@ -30,7 +30,9 @@ mxr_status workspace_failure(const char *fn);
 void plan_tables_retain();
 void plan_tables_release();
 
-// Brackets one kernel launch with profiling events when profiling is on.
+// Brackets one kernel launch: profiling events when profiling is on, an
+// INTERNAL trace range named after the kernel when the trace level asks for
+// it, a TRACE log line when the log level does.
 class ProfScope {
  public:
   ProfScope(const char *kernel, hipStream_t s);
@@ -38,6 +40,7 @@ class ProfScope {
 
  private:
   int slot_ = -1;
+  bool traced_ = false;
   hipStream_t s_;
 };
 }  // namespace mxr

@@ -23,6 +23,7 @@
 
 #include "fft_engine.hpp"
 #include "mxr_cache.hpp"
+#include "mxr_trace.hpp"
 #include "pcgrad.hpp"
 #include "ragged.hpp"
 
@@ -1217,6 +1218,8 @@ static mxr_status pc_plan_create(mxr_pc_plan *out, int64_t S, int64_t L, int64_t
   MXR_HIP_TRY(hipMalloc(&wc, sizeof(float2) * plan.M));
   plan_tables_retain();
   *out = reinterpret_cast<mxr_pc_plan>(new PcPlanObj{device_id(), S, L, n, nout, flags, plan, wc, twf, twi});
+  log_msg(LOG_DEBUG, "pc plan: S=%lld L=%lld n=%lld -> %d-point blocks, %lld per row", (long long)S,
+          (long long)L, (long long)n, plan.M, (long long)plan.nblk);
   return MXR_SUCCESS;
 }
 

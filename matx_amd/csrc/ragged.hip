@@ -23,6 +23,7 @@
 
 #include "fft_engine.hpp"
 #include "mxr_cache.hpp"
+#include "mxr_trace.hpp"
 #include "ragged.hpp"
 
 namespace mxr {
@@ -452,6 +453,8 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
     return st;
   }
   *out = reinterpret_cast<mxr_ragged_plan>(p);
+  log_msg(LOG_DEBUG, "ragged plan: %d jobs, %zu pulse-compression classes, %zu Doppler chains", njobs,
+          p->pc.size(), p->dop.size());
   return MXR_SUCCESS;
 }
 

@@ -11,6 +11,7 @@
 
 #include "fft_engine.hpp"
 #include "mxr_cache.hpp"
+#include "mxr_trace.hpp"
 
 namespace mxr {
 
@@ -23,6 +24,7 @@ void set_error(mxr_status st, const char *fmt, ...) {
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
   g_last_error = std::string("[mxr status ") + std::to_string((int)st) + "] " + buf;
+  log_msg(LOG_ERROR, "%s", g_last_error.c_str());
 }
 
 mxr_status check_tensor(const mxr_tensor *t, const char *name, int want_dtype, int min_rank,
@@ -162,6 +164,8 @@ const float2 *twiddles_device(int N, bool rev) {
     return nullptr;
   }
   g_tw[key] = d;
+  log_msg(LOG_DEBUG, "twiddle table: n=%d%s, %lld entries on device %d", N, rev ? " (reverse order)" : "",
+          (long long)sz, dev);
   return d;
 }
 
@@ -179,6 +183,8 @@ bool g_prof_on = false;
 }  // namespace
 
 ProfScope::ProfScope(const char *kernel, hipStream_t s) : s_(s) {
+  traced_ = trace_push(kernel, MXR_TRACE_INTERNAL);
+  if (log_on(LOG_TRACE)) log_msg(LOG_TRACE, "launch %s on stream %p", kernel, (void *)s);
   std::lock_guard<std::mutex> g(g_prof_mu);
   if (!g_prof_on) return;
   if (g_prof_n == g_prof.size()) {
@@ -194,9 +200,11 @@ ProfScope::ProfScope(const char *kernel, hipStream_t s) : s_(s) {
 }
 
 ProfScope::~ProfScope() {
-  if (slot_ < 0) return;
-  std::lock_guard<std::mutex> g(g_prof_mu);
-  (void)hipEventRecord(g_prof[slot_].b, s_);
+  if (slot_ >= 0) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    (void)hipEventRecord(g_prof[slot_].b, s_);
+  }
+  if (traced_) trace_pop();
 }
 
 namespace {
@@ -278,6 +286,7 @@ void *stream_workspace(const char *tag, size_t bytes, hipStream_t s) {
   if (b.p) g_ws_retired.emplace_back(dev, b.p);
   b.p = nullptr;
   b.bytes = 0;
+  log_msg(LOG_DEBUG, "workspace '%s' on stream %p: %zu bytes", tag, (void *)s, bytes);
   if (hipMalloc(&b.p, bytes) != hipSuccess) {
     b.p = nullptr;
     return nullptr;
@@ -392,6 +401,7 @@ mxr_status mxr_clear_caches(void) {
     }
     for (auto fn : hooks) fn();
     mxr::clear_twiddles();
+    mxr::log_msg(mxr::LOG_DEBUG, "caches cleared");
     return MXR_SUCCESS;
   });
 }

@@ -18,19 +18,35 @@ int main(int argc, char **argv) {
   hipStream_t stream;
   detail::hip_check(hipStreamCreate(&stream), "hipStreamCreate");
   try {
+    // user ranges around the set-up and each stage, as MatX's driver
+    // (examples/simple_radar_pipeline.cu:59-97); a profiler shows them with
+    // rocprofv3 --marker-trace
+    MATX_NVTX_SET_LOG_LEVEL(matx_nvxtLogLevels::MATX_NVTX_LOG_ALL)
+    MATX_NVTX_START_RANGE("Pipeline Initialize", matx_nvxtLogLevels::MATX_NVTX_LOG_USER, 1)
     RadarPipeline p(P, S, L, C, stream);
     p.sync();
+    MATX_NVTX_END_RANGE(1)
     HIPExecutor ex(stream);
     auto run = [&] {
+      MATX_NVTX_START_RANGE("PulseCompression", matx_nvxtLogLevels::MATX_NVTX_LOG_USER, 21)
       p.PulseCompression();
+      MATX_NVTX_END_RANGE(21)
+      MATX_NVTX_START_RANGE("ThreePulseCanceller", matx_nvxtLogLevels::MATX_NVTX_LOG_USER, 22)
       p.ThreePulseCanceller();
+      MATX_NVTX_END_RANGE(22)
+      MATX_NVTX_START_RANGE("DopplerProcessing", matx_nvxtLogLevels::MATX_NVTX_LOG_USER, 23)
       p.DopplerProcessing();
+      MATX_NVTX_END_RANGE(23)
+      MATX_NVTX_START_RANGE("CFARDetections", matx_nvxtLogLevels::MATX_NVTX_LOG_USER, 24)
       p.CFARDetections();
+      MATX_NVTX_END_RANGE(24)
     };
     run();
+    MATX_NVTX_START_RANGE("Pipeline Test", matx_nvxtLogLevels::MATX_NVTX_LOG_USER, 2)
     ex.start_timer();
     for (int i = 0; i < iterations; ++i) run();
     ex.stop_timer();
+    MATX_NVTX_END_RANGE(2)
     const float ms = ex.get_time_ms();
     const bool json = argc > 2 && std::strcmp(argv[2], "--json") == 0;
     if (json)
