@@ -19,6 +19,7 @@
 // DEBUG, kernel launches at TRACE.
 #include <dlfcn.h>
 
+#include <atomic>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -37,8 +38,8 @@ namespace {
 
 // ------------------------------------------------------------------ log
 struct LogState {
-  std::mutex mu;
-  int level = LOG_OFF;
+  std::mutex mu;                    // guards dest and the writes
+  std::atomic<int> level{LOG_OFF};  // read without the lock on every launch
   FILE *dest = stderr;
   bool own = false;  // dest is a file the library opened
 };
@@ -92,7 +93,7 @@ using StopFn = void (*)(uint64_t);
 
 struct TraceState {
   std::mutex mu;
-  int level = MXR_TRACE_API;
+  std::atomic<int> level{MXR_TRACE_API};
   int loaded = 0;  // 0 not tried, 1 loaded, -1 unavailable
   PushFn push = nullptr;
   PopFn pop = nullptr;
@@ -145,8 +146,8 @@ bool trace_load_locked(TraceState &st) {
 }  // namespace
 
 bool log_on(int level) {
-  LogState &st = log_state();
-  return level >= st.level && st.level < LOG_OFF;
+  const int cur = log_state().level.load(std::memory_order_relaxed);
+  return level < LOG_OFF && level >= cur && cur < LOG_OFF;
 }
 
 void log_msg(int level, const char *fmt, ...) {
@@ -172,8 +173,9 @@ void log_msg(int level, const char *fmt, ...) {
 
 bool trace_push(const char *name, int level) {
   TraceState &st = trace_state();
+  // the filter without the lock: every kernel launch passes through here
+  if (level > st.level.load(std::memory_order_relaxed) || level <= MXR_TRACE_NONE) return false;
   std::lock_guard<std::mutex> g(st.mu);
-  if (level > st.level || level <= MXR_TRACE_NONE) return false;
   if (!trace_load_locked(st)) return false;
   st.push(name ? name : "");
   return true;
@@ -193,17 +195,11 @@ extern "C" {
 mxr_status mxr_log_set_level(int level) {
   MXR_REQUIRE(level >= LOG_TRACE && level <= LOG_OFF, MXR_INVALID_PARAMETER,
               "mxr_log_set_level: level %d outside [0, 6]", level);
-  LogState &st = log_state();
-  std::lock_guard<std::mutex> g(st.mu);
-  st.level = level;
+  log_state().level.store(level);
   return MXR_SUCCESS;
 }
 
-int mxr_log_get_level(void) {
-  LogState &st = log_state();
-  std::lock_guard<std::mutex> g(st.mu);
-  return st.level;
-}
+int mxr_log_get_level(void) { return log_state().level.load(); }
 
 mxr_status mxr_log_set_dest(const char *dest) {
   LogState &st = log_state();
@@ -220,17 +216,11 @@ void mxr_log_write(int level, const char *message) { log_msg(level, "%s", messag
 
 mxr_status mxr_trace_set_level(int level) {
   MXR_REQUIRE(level >= MXR_TRACE_NONE, MXR_INVALID_PARAMETER, "mxr_trace_set_level: level %d < 0", level);
-  TraceState &st = trace_state();
-  std::lock_guard<std::mutex> g(st.mu);
-  st.level = level;
+  trace_state().level.store(level);
   return MXR_SUCCESS;
 }
 
-int mxr_trace_get_level(void) {
-  TraceState &st = trace_state();
-  std::lock_guard<std::mutex> g(st.mu);
-  return st.level;
-}
+int mxr_trace_get_level(void) { return trace_state().level.load(); }
 
 int mxr_trace_available(void) {
   TraceState &st = trace_state();
