@@ -450,14 +450,16 @@ def test_conv_random_strided(mx, orc):
 def test_conv2d_real_tiles(mx, orc):
     """Real maps and real filters (the CA-CFAR window sums) through the
     staged-tile kernel: several 32 x 64 tiles per map, batch dims, a strided
-    output view, the 5 x 13 CFAR mask, the largest staged window (16 x 64)
-    and one beyond it (the per-output kernel), all three modes."""
+    output view, the 5 x 13 CFAR mask, other filter heights (one kernel
+    instance per height), the largest staged window (16 x 64) and one beyond
+    it (the per-output kernel), all three modes."""
     rng = np.random.default_rng(9)
     x = rng.random((2, 3, 70, 150)).astype(np.float32)
     mask = np.ones((5, 13), np.float32)
     mask[1:4, 5:8] = 0
     for f in (mask, rng.standard_normal((16, 64)).astype(np.float32),
-              rng.standard_normal((17, 3)).astype(np.float32), np.ones((1, 1), np.float32)):
+              rng.standard_normal((17, 3)).astype(np.float32), np.ones((1, 1), np.float32),
+              rng.standard_normal((3, 7)).astype(np.float32), rng.standard_normal((9, 2)).astype(np.float32)):
         for m in ("full", "same", "valid"):
             ref = orc.conv2d(x.reshape(6, 70, 150), f, m).reshape(2, 3, *orc.conv2d(x[0, 0], f, m).shape[-2:])
             got = host(mx.conv2d(dev(x, torch.float32), dev(f, torch.float32), m))
@@ -479,7 +481,11 @@ def test_conv1d_strided_axis_into_permuted_view(mx, orc):
     C, P, R = 3, 130, 300
     y = crandn(rng, C, P, R)
     yd = dev(y)
-    for f in (np.array([1, -2, 1], np.float32), crandn(rng, 5), rng.standard_normal(40).astype(np.float32)):
+    # register-window kernel up to 16 taps (bounds 4, 8, 16), strip kernel
+    # beyond; real and complex taps
+    filters = (np.array([1, -2, 1], np.float32), crandn(rng, 5), rng.standard_normal(8).astype(np.float32),
+               crandn(rng, 12), rng.standard_normal(16).astype(np.float32), rng.standard_normal(40).astype(np.float32))
+    for f in filters:
         fd = dev(f, torch.complex64 if np.iscomplexobj(f) else torch.float32)
         for m in ("full", "same", "valid"):
             n = {"full": P + len(f) - 1, "same": P, "valid": P - len(f) + 1}[m]
@@ -487,6 +493,20 @@ def test_conv1d_strided_axis_into_permuted_view(mx, orc):
             mx.conv1d(yd.permute(0, 2, 1), fd, m, out=outbuf.permute(0, 2, 1))
             ref = orc.conv1d(np.transpose(y, (0, 2, 1)), f, m)
             assert rel(host(outbuf.permute(0, 2, 1)), ref) < RTOL, (len(f), m)
+    # real input (a real output view): every window-kernel bound, short
+    # conv axis (fewer samples than one strip and than the filter)
+    yr = rng.standard_normal((C, 7, R)).astype(np.float32)
+    for f in (np.array([1, -2, 1], np.float32), rng.standard_normal(6).astype(np.float32),
+              crandn(rng, 9), rng.standard_normal(16).astype(np.float32)):
+        fd = dev(f, torch.complex64 if np.iscomplexobj(f) else torch.float32)
+        odt = torch.complex64 if np.iscomplexobj(f) else torch.float32
+        for m in ("full", "same"):
+            n = {"full": 7 + len(f) - 1, "same": 7}[m]
+            outbuf = torch.zeros((C, n, R), dtype=odt, device="cuda")
+            mx.conv1d(dev(yr, torch.float32).permute(0, 2, 1), fd, m, out=outbuf.permute(0, 2, 1))
+            ref = orc.conv1d(np.transpose(yr, (0, 2, 1)).astype(np.complex64), f, m)
+            got = host(outbuf.permute(0, 2, 1))
+            assert rel(got, ref if np.iscomplexobj(f) else ref.real) < RTOL, (len(f), m)
 
 
 # ------------------------------------------------- end to end vs reference
