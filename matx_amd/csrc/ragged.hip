@@ -16,6 +16,7 @@
 // the device for the job's size class (waveform length rounded up to a power
 // of two, transform size), cached per device for the process.
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -365,14 +366,27 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
     dop_ranges.push_back({first, hdop.size()});
     p->dop.push_back(cl);
   }
-  // chains to helper streams, largest first onto the least loaded stream
-  // (work ~ samples read by the chain's pulse compression)
+  // chains to helper streams, largest first onto the least loaded stream.
+  // A chain's cost is the time its jobs' kernels would take at the rates
+  // they reach on their own: pulse compression bound by the larger of its
+  // HBM bytes (x in, y out) at 5 TB/s and its FFT flops (forward + inverse,
+  // 5 M log2 M each per block) at 35 TFLOP/s; the Doppler pass by its bytes
+  // (y in, power out) at 4 TB/s.
   if (!st) {
     std::vector<double> cost(p->dop.size(), 0.0);
     for (int i = 0; i < njobs; ++i) {
-      const int nd = (int)next_pow2(jobs[i].P);
+      const mxr_ragged_job &j = jobs[i];
+      const int nd = (int)next_pow2(j.P);
+      const int64_t n = next_pow2(j.S), R = j.S - j.L + 1;
+      int Mp = 0;
+      int64_t V = 0, nblk = 0;
+      if (pc_plan_blocks(j.S, j.L, n, R, p->M[i], &Mp, &V, &nblk) != MXR_SUCCESS) nblk = (R + p->M[i] - 1) / p->M[i];
+      const double CP = (double)j.C * (double)j.P, M = (double)p->M[i];
+      const double t_pc = std::max(8.0 * CP * (double)(j.S + R) / 5.0e12,
+                                   10.0 * CP * (double)nblk * M * std::log2(M) / 35.0e12);
+      const double t_dop = (8.0 * CP * (double)R + 4.0 * (double)j.C * nd * (double)R) / 4.0e12;
       for (size_t d = 0; d < p->dop.size(); ++d)
-        if (p->dop[d].nd == nd) cost[d] += (double)jobs[i].C * jobs[i].P * jobs[i].S;
+        if (p->dop[d].nd == nd) cost[d] += t_pc + t_dop;
     }
     p->dop_order.resize(p->dop.size());
     for (size_t d = 0; d < p->dop.size(); ++d) p->dop_order[d] = (int)d;
