@@ -248,9 +248,19 @@ def profiled_kernels(fn, calls, names):
     import torch
     from matx_amd import _lib
     lib = _lib.lib()
+    # untimed warm-up: callers often come here after seconds of CPU work
+    # (the oracle), with the GPU idle in between
+    for _ in range(min(calls, 10)):
+        fn()
     torch.cuda.synchronize()
     _lib.check(lib.mxr_profile_reset())
     _lib.check(lib.mxr_profile_enable(1))
+    # the launches queue behind a fixed-length spin kernel on the same
+    # stream, so the GPU runs them back to back: on an idle GPU the events
+    # around a kernel shorter than its host-side launch would also time the
+    # launch (and any descheduling of the host thread)
+    if hasattr(torch.cuda, "_sleep"):
+        torch.cuda._sleep(5_000_000)
     for _ in range(calls):
         fn()
     torch.cuda.synchronize()

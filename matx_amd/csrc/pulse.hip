@@ -290,7 +290,10 @@ __device__ __forceinline__ void load_block4096(const OlsBlock &ob, float2 (&v)[N
 
 // The block's transforms and stores, from its samples in v (loaded by
 // load_block4096).
-template <class Spec>
+// LATE_SPEC: the spectrum is written by other threads of the workgroup
+// while the forward passes run (pc4096_wf_kernel); it is read after one more
+// barrier, following the last forward pass, instead of being prefetched.
+template <class Spec, bool LATE_SPEC = false>
 __device__ __forceinline__ void filter_block4096(const OlsBlock &ob, float2 (&v)[16],
                                                  const float4 *base1, const float4 *base2,
                                                  float2 *lds, Spec spec) {
@@ -332,11 +335,18 @@ __device__ __forceinline__ void filter_block4096(const OlsBlock &ob, float2 (&v)
   // the spectrum bins this thread multiplies, fetched before the exchange
   // so that their cache latency hides behind it and the last forward pass
   float2 sp[16];
+  if constexpr (!LATE_SPEC) {
 #pragma unroll
-  for (int q = 0; q < 16; ++q) sp[q] = spec(j + q * 256);
+    for (int q = 0; q < 16; ++q) sp[q] = spec(j + q * 256);
+  }
   xchg(16);
   tw_from_base(v, t0, t1, false);  // forward pass 2 (Ls = 256)
   dft16<false>(v);
+  if constexpr (LATE_SPEC) {
+    wg_barrier();  // the spectrum is complete
+#pragma unroll
+    for (int q = 0; q < 16; ++q) sp[q] = spec(j + q * 256);
+  }
 #pragma unroll
   for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], sp[q]);
   dft16<true>(v);            // inverse pass 0 (Ls = 1)
@@ -393,11 +403,14 @@ void pc4096_kernel(Pc4096Args pa) {
 }
 
 // Small grids (fft_conv on a few hundred pulses, where one workgroup per
-// pulse leaves most CUs idle): every workgroup first transforms the
-// zero-padded waveform itself with the same passes, instead of waiting for a
-// one-workgroup spectrum launch.  The last forward pass leaves bins j + 256 q
-// in thread j, exactly the bins its multiply step reads, so the spectrum goes
-// to a slice of LDS that only the same thread reads back (no barrier).  No
+// pulse leaves most CUs idle): every workgroup transforms the zero-padded
+// waveform itself instead of waiting for a one-workgroup spectrum launch.
+// The workgroup has 512 threads: threads 0-255 load and filter the block
+// (the forward passes, then the multiply and the inverse passes), threads
+// 256-511 transform the waveform at the same time through their own LDS
+// buffer and leave the spectrum there (bins j + 256 q of their thread j,
+// where the block's thread j reads them after one more barrier).  The
+// waveform threads then take part in the block's remaining barriers.  No
 // window, normalisation or write-back (the fft_conv form).
 struct Pc4096WfArgs {
   Pc4096Args p;
@@ -407,58 +420,66 @@ struct Pc4096WfArgs {
   float out_scale;
 };
 
-__global__ __launch_bounds__(256) void pc4096_wf_kernel(Pc4096WfArgs q) {
+__global__ __launch_bounds__(512) void pc4096_wf_kernel(Pc4096WfArgs q) {
   using LY = Layout<4096, 1, false>;
-  __shared__ float2 lds[LY::LDS_ELEMS];
-  __shared__ float2 wsp[4096];
+  __shared__ float2 lds[LY::LDS_ELEMS];   // the block's exchanges
+  __shared__ float2 wsp[LY::LDS_ELEMS];   // the waveform's exchanges, then its spectrum
   constexpr int NT = 256;
-  const int j = threadIdx.x;
-  {
+  if (threadIdx.x < NT) {
+    const PcArgs &a = q.p.a;
+    const int64_t g = xcd_remap(blockIdx.x, a.groups);
+    const int64_t row = g / a.nblk;
+    int64_t ox, oy;
+    batch_offsets(a.bi, row, &ox, &oy);
+    const OlsBlock ob{a.x + ox, a.y + oy, (int)((g % a.nblk) * a.V), (int)a.n, (int)a.seff,
+                      (int)a.nout, (int)a.V, nullptr};
     float2 v[16];
-    auto xchg = [&](int ls_out) {
-      const int k = j & (ls_out - 1), base = (j - k) * 16 + k;
-#pragma unroll
-      for (int m = 0; m < 16; ++m) lds[LY::addr(0, base + m * ls_out)] = v[m];
-      wg_barrier();
-#pragma unroll
-      for (int m = 0; m < 16; ++m) v[m] = lds[LY::addr(0, j + m * NT)];
-      wg_barrier();
-    };
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int i = j + m * NT;
-      v[m] = i < q.L ? q.w[(int64_t)i * q.wstride] : make_float2(0.f, 0.f);
-    }
-    const float4 *b1p = q.p.base1 + (j & 15) * 2;
-    const float4 *b2p = q.p.base2 + j * 2;
-    dft16<false>(v);
-    float4 t0 = b1p[0], t1 = b1p[1];
-    xchg(1);
-    tw_from_base(v, t0, t1, false);
-    dft16<false>(v);
-    t0 = b2p[0];
-    t1 = b2p[1];
-    xchg(16);
-    tw_from_base(v, t0, t1, false);
-    dft16<false>(v);
-#pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      float2 x = cscale(v[m], q.out_scale);
-      if (q.conj) x.y = -x.y;
-      wsp[j + m * NT] = x;
-    }
+    load_block4096(ob, v, 0);
+    // barriers: 4 in the forward exchanges, 1 for the spectrum, 4 in the
+    // inverse exchanges
+    auto spec = [&](int i) { return wsp[i]; };
+    filter_block4096<decltype(spec), true>(ob, v, q.p.base1, q.p.base2, lds, spec);
+    return;
   }
-  // the exchanges above end with a barrier: lds is free for the block
-  const PcArgs &a = q.p.a;
-  const int64_t g = xcd_remap(blockIdx.x, a.groups);
-  const int64_t row = g / a.nblk;
-  int64_t ox, oy;
-  batch_offsets(a.bi, row, &ox, &oy);
-  const OlsBlock ob{a.x + ox, a.y + oy, (int)((g % a.nblk) * a.V), (int)a.n, (int)a.seff,
-                    (int)a.nout, (int)a.V, nullptr};
-  ols_block4096(ob, q.p.base1, q.p.base2, lds, [&](int i) { return wsp[i]; });
+  // the waveform: the same three forward passes, exchanged through wsp
+  const int j = threadIdx.x - NT;
+  float2 v[16];
+  auto xchg = [&](int ls_out) {
+    const int k = j & (ls_out - 1), base = (j - k) * 16 + k;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) wsp[LY::addr(0, base + m * ls_out)] = v[m];
+    wg_barrier();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) v[m] = wsp[LY::addr(0, j + m * NT)];
+    wg_barrier();
+  };
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int i = j + m * NT;
+    v[m] = keep_if(q.w[(int64_t)(i < q.L ? i : 0) * q.wstride], i < q.L);
+  }
+  const float4 *b1p = q.p.base1 + (j & 15) * 2;
+  const float4 *b2p = q.p.base2 + j * 2;
+  dft16<false>(v);
+  float4 t0 = b1p[0], t1 = b1p[1];
+  xchg(1);
+  tw_from_base(v, t0, t1, false);
+  dft16<false>(v);
+  t0 = b2p[0];
+  t1 = b2p[1];
+  xchg(16);
+  tw_from_base(v, t0, t1, false);
+  dft16<false>(v);
+  // the exchange above ended with a barrier after every read of wsp
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    float2 x = cscale(v[m], q.out_scale);
+    if (q.conj) x.y = -x.y;
+    wsp[j + m * NT] = x;
+  }
+  wg_barrier();  // the spectrum is complete (the block's fifth barrier)
+  for (int b = 0; b < 4; ++b) wg_barrier();  // the block's inverse exchanges
 }
-
 
 // Dedicated M = 8192 block kernel: radix sequence 2 x 16 x 16 x 16 over 512
 // threads (one radix-16 butterfly per thread per pass) with the power-base
@@ -988,13 +1009,13 @@ template <int M>
 mxr_status launch_pc(const WfArgs *wa, const PcArgs *pa, hipStream_t s) {
   if constexpr (M == 4096) {
     if (wa && pa && pa->groups > 0 && pa->bpw == 1 && pa->groups <= PC_WF_FUSE_MAX_GROUPS &&
-        !wa->window && !wa->normalize && !wa->writeback && wa->L <= 4096) {
+        !wa->window && !wa->normalize && !wa->writeback && wa->L >= 1 && wa->L <= 4096) {
       Pc4096WfArgs q{{*pa, pow_base_device(4096, 16), pow_base_device(4096, 256)}, wa->w, wa->wstride,
                      (int)wa->L, wa->conj, wa->out_scale};
       MXR_REQUIRE(q.p.base1 && q.p.base2, MXR_OUT_OF_MEMORY,
                   "pulse compression: twiddle table allocation failed");
       ProfScope ps("pc4096_wf_kernel", s);
-      pc4096_wf_kernel<<<dim3((unsigned)pa->groups), dim3(256), 0, s>>>(q);
+      pc4096_wf_kernel<<<dim3((unsigned)pa->groups), dim3(512), 0, s>>>(q);
       MXR_HIP_TRY(hipGetLastError());
       return MXR_SUCCESS;
     }
