@@ -20,11 +20,17 @@
 
 namespace mxr {
 
-// One row operand: element i of row b at p[off(b) + i*s], i < len.
+// One row operand: element i of row b at p[off(b) + i*s], i < len.  With
+// wrap_n > 0 the operand is a lag sequence of a wrap_n-point circular
+// correlation gathered into a shorter transform: buffer slot i < pos holds
+// lag i, slot M - j (j <= neg) holds lag -j (element wrap_n - j), the rest
+// zeros.
 struct RowOp {
   const float2 *p;
   int64_t len, s;
   BatchIdx bi;  // sa: offsets of this operand per batch dim
+  int64_t wrap_n;
+  int64_t pos, neg;
 };
 
 struct XcArgs {
@@ -32,6 +38,10 @@ struct XcArgs {
   int mode;          // 0: A.*conj(B), 1: A.*B, 3: |A|^2 (b unused), 4: A.*2Re(B)
   float2 *out;
   int64_t nout, os;  // kept lags, out element stride
+  // full_n > 0: the M-point result stands for a full_n-point circular one
+  // whose nonzero lags are 0..npos-1 and -1..-nneg (at full_n - j); lag i
+  // of the output reads slot i, slot M - (full_n - i) or zero
+  int64_t full_n, npos, nneg;
   BatchIdx obi;      // sa: out offsets per batch dim
   const float2 *tw;
   float scale;
@@ -66,7 +76,10 @@ __device__ __forceinline__ void load_row(float2 *buf, const RowOp &op, int64_t r
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
       const int i = threadIdx.x + m * NT;
-      v[m] = keep_if(src[(int64_t)(i < op.len ? i : 0) * op.s], i < op.len);
+      int64_t e = i;  // element of the operand row in slot i
+      if (op.wrap_n > 0) e = i < op.pos ? i : (i >= M - op.neg ? op.wrap_n - (M - i) : -1);
+      const bool ok = e >= 0 && e < op.len;
+      v[m] = keep_if(src[(ok ? e : 0) * op.s], ok);
     }
   }
 #pragma unroll
@@ -102,6 +115,15 @@ __global__ __launch_bounds__(XcCfg<M>::NT) void xcorr_rows_kernel(XcArgs a) {
   int64_t off, unused;
   batch_offsets(a.obi, row, &off, &unused);
   float2 *dst = a.out + off;
+  if (a.full_n > 0) {
+    for (int64_t i = threadIdx.x; i < a.nout; i += NT) {
+      float2 r = make_float2(0.f, 0.f);
+      if (i < a.npos) r = A[LY::addr(0, (int)i)];
+      else if (i >= a.full_n - a.nneg) r = A[LY::addr(0, (int)(M - (a.full_n - i)))];
+      dst[i * a.os] = cscale(r, a.scale);
+    }
+    return;
+  }
   for (int i = threadIdx.x; i < a.nout; i += NT) dst[(int64_t)i * a.os] = cscale(A[LY::addr(0, i)], a.scale);
 }
 
@@ -156,6 +178,16 @@ static int64_t batch_rows(const mxr_tensor *t) {
   return r;
 }
 
+// The transform size a row correlation needs: an n-point circular
+// correlation (or convolution) of rows whose linear result has at most
+// `support` nonzero lags equals the linear result, so the smallest power of
+// two >= support (at least 1024, the smallest kernel) gives the same lags.
+static int64_t xc_size(int64_t support) {
+  int64_t m = 1024;
+  while (m < support) m <<= 1;
+  return m;
+}
+
 static mxr_status xc_run(const char *fn, const mxr_tensor *out, const RowOp &A, const RowOp &B,
                          int mode, int64_t n, hipStream_t s) {
   MXR_REQUIRE(n >= 1024 && n <= 8192 && (n & (n - 1)) == 0, MXR_NOT_SUPPORTED,
@@ -167,6 +199,25 @@ static mxr_status xc_run(const char *fn, const mxr_tensor *out, const RowOp &A, 
   a.mode = mode;
   a.out = (float2 *)out->data;
   a.nout = out->shape[out->rank - 1];
+  // Short rows: transform at the support instead of n (the same lags,
+  // exactly; the lags outside the support are written as zeros).
+  //  3 (autocorrelation): lags -(La-1) .. La-1;  0 (correlation): -(Lb-1) ..
+  //  La-1;  1 (convolution): 0 .. La+Lb-2.  4 (the autocorrelation's
+  //  adjoint, kept lags < La): only the lags -(La-1) .. La-1 of b reach the
+  //  kept outputs, and M >= 2La - 1 keeps them from aliasing.
+  const int64_t La = A.len, Lb = B.len;
+  int64_t M = n;
+  if (mode == 3 && La >= 1 && xc_size(2 * La - 1) < n) {
+    M = xc_size(2 * La - 1);
+    a.full_n = n, a.npos = La, a.nneg = La - 1;
+  } else if ((mode == 0 || mode == 1) && La >= 1 && Lb >= 1 && xc_size(La + Lb - 1) < n) {
+    M = xc_size(La + Lb - 1);
+    a.full_n = n, a.npos = mode == 0 ? La : La + Lb - 1, a.nneg = mode == 0 ? Lb - 1 : 0;
+  } else if (mode == 4 && La >= 1 && a.nout <= La && xc_size(2 * La - 1) < n) {
+    M = xc_size(2 * La - 1);
+    a.b.wrap_n = n, a.b.pos = La, a.b.neg = La - 1;
+  }
+  n = M;
   a.os = out->stride[out->rank - 1];
   a.obi.nd = out->rank - 1;
   for (int d = 0; d < out->rank - 1; ++d) {

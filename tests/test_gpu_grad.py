@@ -49,9 +49,14 @@ def np_corr(x, w, n, conj=True):
     return np.fft.ifft(X * (np.conj(W) if conj else W), axis=-1)
 
 
+# the last four transform at the rows' support (a smaller power of two than
+# n) and write the other lags as zeros: convolution, a kernel row longer
+# than the signal row, fewer kept lags than n
 @pytest.mark.parametrize("B,S,L,n,nout,conj", [
     (5, 1000, 1000, 2048, 2048, True), (3, 700, 300, 1024, 1000, True),
-    (4, 2048, 256, 4096, 4096, False), (2, 1024, 1024, 8192, 8192, True)])
+    (4, 2048, 256, 4096, 4096, False), (2, 1024, 1024, 8192, 8192, True),
+    (3, 500, 200, 4096, 4096, False), (4, 100, 900, 4096, 4096, True),
+    (2, 600, 300, 8192, 5000, True), (2, 1500, 500, 8192, 8192, False)])
 def test_fft_conv_rows(mx, B, S, L, n, nout, conj):
     rng = np.random.default_rng(n + S)
     x = crandn(rng, B, S)
