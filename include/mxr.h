@@ -353,6 +353,14 @@ mxr_status mxr_doppler_taps_adjoint(const mxr_tensor *gtaps, const mxr_tensor *g
  *   zero outside the target columns, columns shared by targets summed in
  *   target order. */
 mxr_status mxr_isl(const mxr_tensor *isl, const mxr_tensor *r, hipStream_t stream);
+/* The phase-only waveform's ISL in one kernel each way (no MatX counterpart):
+ * mxr_phase_isl: phi [B, L] f32 -> isl [B] f32, the ISL of the n-point
+ *   autocorrelation fft_conv(s, s, n) of s = exp(j phi) (1 <= L <= n, n a
+ *   power of two in [1024, 8192]); equal to mxr_isl of that autocorrelation.
+ * mxr_phase_isl_adjoint: gphi [B, L] f32 = d(sum_b gisl_b isl_b)/dphi. */
+mxr_status mxr_phase_isl(const mxr_tensor *isl, const mxr_tensor *phi, int64_t n, hipStream_t stream);
+mxr_status mxr_phase_isl_adjoint(const mxr_tensor *gphi, const mxr_tensor *gisl, const mxr_tensor *phi,
+                                 int64_t n, hipStream_t stream);
 mxr_status mxr_isl_adjoint(const mxr_tensor *gr, const mxr_tensor *gisl, const mxr_tensor *r,
                            hipStream_t stream);
 mxr_status mxr_psl(const mxr_tensor *psl, const mxr_tensor *pw, const mxr_tensor *targets, int ml,

@@ -20,6 +20,10 @@ WeightedPowerSum: L = sum_{c,k,r} wmap[k, r] pow[c, k, r]; its backward hands
               on wmap expanded over channels (a stride-0 view, never
               materialised), which the pulse-axis adjoint reads as is.
 
+PhaseIsl:     isl = ISL(fft_conv(s, s, n)) of s = exp(j phi) per row, one kernel
+              each way (mxr_phase_isl / mxr_phase_isl_adjoint); the same value
+              and gradient as Isl(autocorr(polar(1, phi))).
+
 `design_loss` is the waveform/window design objective: integrated sidelobe
 level of every waveform's matched-filter output plus a peak-sidelobe penalty
 on the Doppler spectra of a target cube.
@@ -167,6 +171,24 @@ class Isl(torch.autograd.Function):
         return ops.isl_adjoint(g.contiguous(), r)
 
 
+class PhaseIsl(torch.autograd.Function):
+    """Per-row ISL of the n-point autocorrelation of exp(j phi)
+    (mxr_phase_isl / mxr_phase_isl_adjoint): Isl(autocorr(polar(1, phi)))
+    in one kernel each way."""
+
+    @staticmethod
+    def forward(ctx, phi, n: int):
+        phi = phi.contiguous()
+        ctx.save_for_backward(phi)
+        ctx.n = n
+        return ops.phase_isl(phi, n)
+
+    @staticmethod
+    def backward(ctx, g):
+        (phi,) = ctx.saved_tensors
+        return ops.phase_isl_adjoint(g.contiguous(), phi, ctx.n), None
+
+
 class Psl(torch.autograd.Function):
     """Peak-sidelobe ratio of Doppler columns at targets (mxr_psl /
     mxr_psl_adjoint); the gradient is zero outside the target columns."""
@@ -207,9 +229,10 @@ def design_loss(phi, taps, cube, targets, n: int, nd: int, lam: float = 1.0, ml:
     """phi: [B, Lw] phases; taps: [P] Doppler window; cube: [C, P, R] c64;
     targets: [T, 3] int (channel, range bin, Doppler bin; int32 avoids a
     conversion per call).  Returns mean ISL + lam * mean PSL (the
-    definitions of oracle/grad.py); both heads are single HIP kernels."""
-    s = torch.polar(torch.ones_like(phi), phi)
-    isl = Isl.apply(autocorr(s, n))
+    definitions of oracle/grad.py); both heads are single HIP kernels, and
+    the waveform term (phases -> samples -> autocorrelation -> ISL) is one
+    kernel forward and one backward."""
+    isl = PhaseIsl.apply(phi, n)
     pw = doppler_power(cube, taps, nd)
     tg = targets if targets.dtype == torch.int32 else targets.to(torch.int32)
     psl = Psl.apply(pw, tg, ml, q)

@@ -13,6 +13,7 @@
 // serves the forward and both adjoints: a workgroup per row keeps two
 // n-point spectra in LDS (n <= 8192: 2 x 68 KiB), runs the Stockham passes of
 // the shared FFT engine, multiplies, inverts and stores the kept lags.
+#include <algorithm>
 #include <cmath>
 
 #include "fft_engine.hpp"
@@ -316,6 +317,211 @@ static mxr_status autocorr_adjoint_impl(const mxr_tensor *gs, const mxr_tensor *
   return xc_run(fn, gs, S, GY, 4, n, st_);
 }
 
+// ---------------------------------------------------- phase-only ISL head
+// The waveform term of the design loss in two kernels: ISL_b of the n-point
+// autocorrelation of s_b = exp(j phi_b), and its gradient w.r.t. phi.  The
+// phases become samples on load and the autocorrelation never leaves LDS;
+// the transform runs at the support (xc_size(2L - 1)) when that is shorter
+// than n, which gives the same lags (the others are zero and add nothing to
+// either sum).  Sums run in double, as in the stand-alone ISL head.  The
+// adjoint forms gr = dISL/dr in place of r, then gs = IFFT(S .* 2Re(GR)) / M
+// (the autocorrelation's adjoint) and gphi = Im(conj(s) gs) = cos(phi)
+// Im(gs) - sin(phi) Re(gs) (the chain rule through s = exp(j phi), torch's
+// g = dL/dRe + i dL/dIm convention).
+struct PhIslArgs {
+  const float *phi;
+  int64_t ps_b, ps_l;  // phi [B, L]
+  int L;
+  float *isl;          // forward output [B]
+  int64_t is_b;
+  const float *gisl;   // adjoint input [B]
+  int64_t gs_b;
+  float *gphi;         // adjoint output [B, L]
+  int64_t gps_b, gps_l;
+  const float2 *tw;
+};
+
+template <int NT>
+__device__ __forceinline__ double xc_block_sum(double v, double *red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();  // red is free
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) t += red[w];
+  return t;
+}
+
+// s = exp(j phi) of row `row` into buf, zero past L.  Pre: L >= 1.
+template <int M>
+__device__ __forceinline__ void load_phases(float2 *buf, const PhIslArgs &a, int64_t row) {
+  using LY = Layout<M, 1, false>;
+  constexpr int NT = XcCfg<M>::NT, PER = M / NT;
+  const float *src = a.phi + row * a.ps_b;
+  float ph[PER];
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const int i = threadIdx.x + m * NT;
+    ph[m] = src[(int64_t)(i < a.L ? i : 0) * a.ps_l];
+  }
+#pragma unroll
+  for (int m = 0; m < PER; ++m) {
+    const int i = threadIdx.x + m * NT;
+    float sn, cs;
+    sincosf(ph[m], &sn, &cs);
+    buf[LY::addr(0, i)] = i < a.L ? make_float2(cs, sn) : make_float2(0.f, 0.f);
+  }
+}
+
+// |FFT(s)|^2 -> inverse: the unscaled autocorrelation in buf (spec, when
+// given, keeps the spectrum S).  Returns sum |r|^2 and |r[0]|^2 (unscaled).
+template <int M>
+__device__ __forceinline__ void phase_autocorr(float2 *buf, float2 *spec, const PhIslArgs &a,
+                                               int64_t row, double *red, double *S, double *P0) {
+  using LY = Layout<M, 1, false>;
+  constexpr int NT = XcCfg<M>::NT;
+  load_phases<M>(buf, a, row);
+  wg_barrier();
+  row_fft<M, false>(buf, a.tw);
+  for (int i = threadIdx.x; i < M; i += NT) {
+    const float2 x = buf[LY::addr(0, i)];
+    if (spec) spec[LY::addr(0, i)] = x;
+    buf[LY::addr(0, i)] = make_float2(cabs2(x), 0.f);
+  }
+  wg_barrier();
+  row_fft<M, true>(buf, a.tw);
+  const float2 v0 = buf[LY::addr(0, 0)];
+  double t = 0.0;
+  for (int i = threadIdx.x; i < M; i += NT) {
+    const float2 v = buf[LY::addr(0, i)];
+    t += (double)v.x * v.x + (double)v.y * v.y;
+  }
+  *S = xc_block_sum<NT>(t, red);  // its barriers also order the reads of r[0] before any write
+  *P0 = (double)v0.x * v0.x + (double)v0.y * v0.y;
+}
+
+template <int M>
+__global__ __launch_bounds__(XcCfg<M>::NT) void phase_isl_kernel(PhIslArgs a) {
+  using LY = Layout<M, 1, false>;
+  constexpr int NT = XcCfg<M>::NT;
+  __shared__ float2 A[LY::LDS_ELEMS];
+  __shared__ double red[NT / 64];
+  const int64_t row = blockIdx.x;
+  double S, P0;
+  phase_autocorr<M>(A, nullptr, a, row, red, &S, &P0);
+  if (threadIdx.x == 0) a.isl[row * a.is_b] = (float)((S - P0) / P0);
+}
+
+template <int M>
+__global__ __launch_bounds__(XcCfg<M>::NT) void phase_isl_adj_kernel(PhIslArgs a) {
+  using LY = Layout<M, 1, false>;
+  constexpr int NT = XcCfg<M>::NT;
+  __shared__ float2 A[LY::LDS_ELEMS];
+  __shared__ float2 B[LY::LDS_ELEMS];
+  __shared__ double red[NT / 64];
+  const int64_t row = blockIdx.x;
+  double S, P0;
+  phase_autocorr<M>(A, B, a, row, red, &S, &P0);
+  // with r = r_raw / M: dISL/dr = 2 g r / P0 (l != 0), -2 g (S - P0) / P0^2 r
+  // (l = 0), in terms of the unscaled sums: factors M / P0_raw and
+  // M (S_raw - P0_raw) / P0_raw^2
+  const double g = (double)a.gisl[row * a.gs_b];
+  const float side = (float)(2.0 * g * M / P0);
+  const float lag0 = (float)(-2.0 * g * (S - P0) * M / (P0 * P0));
+  for (int i = threadIdx.x; i < M; i += NT) A[LY::addr(0, i)] = cscale(A[LY::addr(0, i)], i == 0 ? lag0 : side);
+  wg_barrier();
+  row_fft<M, false>(A, a.tw);
+  for (int i = threadIdx.x; i < M; i += NT)
+    A[LY::addr(0, i)] = cscale(B[LY::addr(0, i)], 2.0f * A[LY::addr(0, i)].x);
+  wg_barrier();
+  row_fft<M, true>(A, a.tw);
+  const float inv = 1.0f / (float)M;
+  const float *src = a.phi + row * a.ps_b;
+  float *dst = a.gphi + row * a.gps_b;
+  for (int i = threadIdx.x; i < a.L; i += NT) {
+    const float2 gs = cscale(A[LY::addr(0, i)], inv);
+    float sn, cs;
+    sincosf(src[(int64_t)i * a.ps_l], &sn, &cs);
+    dst[(int64_t)i * a.gps_l] = cs * gs.y - sn * gs.x;
+  }
+}
+
+// [B, L] f32 phases: a row view with its strides.
+static mxr_status phase_isl_args(const char *fn, const mxr_tensor *phi, int64_t n, PhIslArgs *a,
+                                 int64_t *M) {
+  mxr_status st;
+  if ((st = check_tensor(phi, "phi", MXR_F32, 2, 2))) return st;
+  MXR_REQUIRE(n >= 1024 && n <= 8192 && (n & (n - 1)) == 0, MXR_NOT_SUPPORTED,
+              "%s: transform size %lld must be a power of two in [1024, 8192]", fn, (long long)n);
+  MXR_REQUIRE(phi->shape[1] >= 1 && phi->shape[1] <= n, MXR_INVALID_SIZE,
+              "%s: need 1 <= L <= n (L = %lld)", fn, (long long)phi->shape[1]);
+  MXR_REQUIRE(phi->shape[0] <= 0x7fffffff, MXR_INVALID_SIZE, "%s: too many rows", fn);
+  a->phi = (const float *)phi->data;
+  a->ps_b = phi->stride[0];
+  a->ps_l = phi->stride[1];
+  a->L = (int)phi->shape[1];
+  *M = std::min<int64_t>(n, xc_size(2 * phi->shape[1] - 1));
+  a->tw = twiddles_device((int)*M, false);
+  MXR_REQUIRE(a->tw != nullptr, MXR_OUT_OF_MEMORY, "%s: twiddle table allocation failed", fn);
+  return MXR_SUCCESS;
+}
+
+#define MXR_PH_DISPATCH(KERNEL, M, rows, a, s)                                                    \
+  switch (M) {                                                                                   \
+    case 1024: KERNEL<1024><<<dim3((unsigned)(rows)), dim3(XcCfg<1024>::NT), 0, s>>>(a); break; \
+    case 2048: KERNEL<2048><<<dim3((unsigned)(rows)), dim3(XcCfg<2048>::NT), 0, s>>>(a); break; \
+    case 4096: KERNEL<4096><<<dim3((unsigned)(rows)), dim3(XcCfg<4096>::NT), 0, s>>>(a); break; \
+    default: KERNEL<8192><<<dim3((unsigned)(rows)), dim3(XcCfg<8192>::NT), 0, s>>>(a); break;   \
+  }
+
+static mxr_status phase_isl_impl(const mxr_tensor *isl, const mxr_tensor *phi, int64_t n,
+                                 hipStream_t s) {
+  const char *fn = "phase_isl";
+  PhIslArgs a{};
+  int64_t M;
+  mxr_status st;
+  if ((st = phase_isl_args(fn, phi, n, &a, &M))) return st;
+  if ((st = check_tensor(isl, "isl", MXR_F32, 1, 1))) return st;
+  MXR_REQUIRE(isl->shape[0] == phi->shape[0], MXR_INVALID_SIZE, "%s: isl must be [B]", fn);
+  MXR_REQUIRE(!views_overlap(*isl, *phi), MXR_ALIAS_ERROR, "%s: isl overlaps phi", fn);
+  if (phi->shape[0] == 0) return MXR_SUCCESS;
+  a.isl = (float *)isl->data;
+  a.is_b = isl->stride[0];
+  ProfScope ps("phase_isl_kernel", s);
+  MXR_PH_DISPATCH(phase_isl_kernel, M, phi->shape[0], a, s)
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
+static mxr_status phase_isl_adjoint_impl(const mxr_tensor *gphi, const mxr_tensor *gisl,
+                                         const mxr_tensor *phi, int64_t n, hipStream_t s) {
+  const char *fn = "phase_isl_adjoint";
+  PhIslArgs a{};
+  int64_t M;
+  mxr_status st;
+  if ((st = phase_isl_args(fn, phi, n, &a, &M))) return st;
+  if ((st = check_tensor(gisl, "gisl", MXR_F32, 1, 1))) return st;
+  if ((st = check_tensor(gphi, "gphi", MXR_F32, 2, 2))) return st;
+  MXR_REQUIRE(gisl->shape[0] == phi->shape[0] && gphi->shape[0] == phi->shape[0] &&
+                  gphi->shape[1] == phi->shape[1],
+              MXR_INVALID_SIZE, "%s: gisl [B] and gphi [B, L] must match phi", fn);
+  MXR_REQUIRE(!views_overlap(*gphi, *phi) && !views_overlap(*gphi, *gisl), MXR_ALIAS_ERROR,
+              "%s: gphi overlaps an input", fn);
+  if (phi->shape[0] == 0) return MXR_SUCCESS;
+  a.gisl = (const float *)gisl->data;
+  a.gs_b = gisl->stride[0];
+  a.gphi = (float *)gphi->data;
+  a.gps_b = gphi->stride[0];
+  a.gps_l = gphi->stride[1];
+  ProfScope ps("phase_isl_adj_kernel", s);
+  MXR_PH_DISPATCH(phase_isl_adj_kernel, M, phi->shape[0], a, s)
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+#undef MXR_PH_DISPATCH
+
 }  // namespace mxr
 
 extern "C" mxr_status mxr_autocorr_adjoint(const mxr_tensor *gs, const mxr_tensor *gy,
@@ -331,4 +537,15 @@ extern "C" mxr_status mxr_fft_conv_adjoint(const mxr_tensor *gx, const mxr_tenso
   return mxr::guarded("mxr_fft_conv_adjoint", [&] {
     return mxr::fft_conv_adjoint_impl(gx, gw, gy, x, w, n, conj_w, stream);
   });
+}
+
+extern "C" mxr_status mxr_phase_isl(const mxr_tensor *isl, const mxr_tensor *phi, int64_t n,
+                                    hipStream_t stream) {
+  return mxr::guarded("mxr_phase_isl", [&] { return mxr::phase_isl_impl(isl, phi, n, stream); });
+}
+
+extern "C" mxr_status mxr_phase_isl_adjoint(const mxr_tensor *gphi, const mxr_tensor *gisl,
+                                            const mxr_tensor *phi, int64_t n, hipStream_t stream) {
+  return mxr::guarded("mxr_phase_isl_adjoint",
+                      [&] { return mxr::phase_isl_adjoint_impl(gphi, gisl, phi, n, stream); });
 }

@@ -320,6 +320,22 @@ def isl(r, stream=None):
     return out
 
 
+def phase_isl(phi, n: int, stream=None):
+    """ISL of the n-point autocorrelation of exp(j phi) per row (phi: [B, L]
+    f32), one kernel: equal to isl(fft_conv(s, s, n)) with s = polar(1, phi)."""
+    out = torch.empty((phi.shape[0],), dtype=torch.float32, device=phi.device)
+    L.check(L.lib().mxr_phase_isl(L.ref(L.desc(out)), L.ref(L.desc(phi)), int(n),
+                                  L.stream_handle(stream)))
+    return out
+
+
+def phase_isl_adjoint(gisl, phi, n: int, stream=None):
+    gphi = torch.empty_like(phi)
+    L.check(L.lib().mxr_phase_isl_adjoint(L.ref(L.desc(gphi)), L.ref(L.desc(gisl)), L.ref(L.desc(phi)),
+                                          int(n), L.stream_handle(stream)))
+    return gphi
+
+
 def isl_adjoint(gisl, r, stream=None):
     gr = torch.empty_like(r)
     L.check(L.lib().mxr_isl_adjoint(L.ref(L.desc(gr)), L.ref(L.desc(gisl)), L.ref(L.desc(r)),

@@ -145,6 +145,54 @@ def test_autocorr_adjoint_is_sum_of_both_terms(mx):
     assert rel(host(gs), ref) < RTOL
 
 
+def _phase_isl_fp64(phi, n, g):
+    """ISL of the n-point autocorrelation of exp(j phi) per row and its
+    gradient w.r.t. phi for upstream g, in FP64 (closed forms: r = IFFT(|S|^2),
+    dISL/dr = 2 r / P0 off lag 0 and -2 (S - P0) / P0^2 r at lag 0, the
+    autocorrelation's adjoint IFFT(S .* 2Re(FFT(gr))), then the chain rule
+    through s = exp(j phi))."""
+    s = np.exp(1j * phi.astype(np.float64))
+    S = np.fft.fft(s, n, axis=-1)
+    r = np.fft.ifft(np.abs(S) ** 2, axis=-1)
+    pr = np.abs(r) ** 2
+    tot, p0 = pr.sum(-1), pr[:, 0]
+    isl = (tot - p0) / p0
+    gr = (2.0 * g / p0)[:, None] * r
+    gr[:, 0] = -2.0 * g * (tot - p0) / p0 ** 2 * r[:, 0]
+    gs = np.fft.ifft(S * 2.0 * np.real(np.fft.fft(gr, axis=-1)), axis=-1)[:, :phi.shape[1]]
+    gphi = np.cos(phi) * gs.imag - np.sin(phi) * gs.real
+    return isl, gphi
+
+
+# the transform runs at the support (2048 for L = 1024 or 700) or at n
+@pytest.mark.parametrize("B,L,n", [(4, 1024, 8192), (5, 700, 4096), (3, 300, 1024), (2, 3000, 8192)])
+def test_phase_isl_head(mx, B, L, n):
+    from matx_amd import autograd as ag
+    rng = np.random.default_rng(L + n)
+    phi = rng.uniform(0, 2 * np.pi, (B, L)).astype(np.float32)
+    g = rng.standard_normal(B).astype(np.float32)
+    isl64, gphi64 = _phase_isl_fp64(phi, n, g.astype(np.float64))
+    pd = dev(phi, torch.float32)
+    assert rel(host(mx.ops.phase_isl(pd, n)), isl64) < RTOL
+    assert rel(host(mx.ops.phase_isl_adjoint(dev(g, torch.float32), pd, n)), gphi64) < GTOL
+    # the same value and gradient as the composition it replaces
+    p1 = pd.clone().requires_grad_()
+    p2 = pd.clone().requires_grad_()
+    gt = dev(g, torch.float32)
+    (ag.PhaseIsl.apply(p1, n) * gt).sum().backward()
+    s = torch.polar(torch.ones_like(p2), p2)
+    (ag.Isl.apply(ag.autocorr(s, n)) * gt).sum().backward()
+    assert rel(host(p1.grad), host(p2.grad)) < GTOL
+
+
+def test_phase_isl_errors(mx):
+    phi = torch.zeros((2, 100), device="cuda")
+    with pytest.raises(Exception, match="power of two"):
+        mx.ops.phase_isl(phi, 3000)
+    with pytest.raises(Exception, match="L <= n"):
+        mx.ops.phase_isl(torch.zeros((2, 2000), device="cuda"), 1024)
+
+
 def test_isl_head_and_adjoint(mx):
     rng = np.random.default_rng(31)
     B, n = 5, 4096
