@@ -367,6 +367,15 @@ mxr_status mxr_psl(const mxr_tensor *psl, const mxr_tensor *pw, const mxr_tensor
                    int q, hipStream_t stream);
 mxr_status mxr_psl_adjoint(const mxr_tensor *gpow, const mxr_tensor *gpsl, const mxr_tensor *pw,
                            const mxr_tensor *targets, int ml, int q, hipStream_t stream);
+/* The taps gradient of sum_t gpsl_t psl_t for pw = |FFT_nd(taps .* x)|^2
+ * along pulses (mxr_doppler_taps' power), taken from the target columns
+ * alone (no MatX counterpart): gtaps [P] f32 from gpsl [T], the saved pw
+ * [C, nd, R], x [C, P, R] c64 and taps [P]; equal to mxr_psl_adjoint
+ * followed by mxr_doppler_taps_adjoint's gtaps, without the map-sized
+ * gradient.  Shares per target are summed in target order. */
+mxr_status mxr_psl_taps_adjoint(const mxr_tensor *gtaps, const mxr_tensor *gpsl, const mxr_tensor *pw,
+                                const mxr_tensor *x, const mxr_tensor *taps, const mxr_tensor *targets,
+                                int ml, int q, hipStream_t stream);
 
 /* --------------------------------------------------------- ragged batch */
 /* Many independent pulse-compression + Doppler jobs of uneven shapes in a

@@ -32,7 +32,7 @@ EXPORTS = ("mxr_last_error", "mxr_version", "mxr_clear_caches", "mxr_profile_ena
            "mxr_ragged_plan_create", "mxr_ragged_execute", "mxr_ragged_plan_block_size",
            "mxr_ragged_plan_destroy", "mxr_doppler_adjoint", "mxr_pc_plan_waveform_grad",
            "mxr_isl", "mxr_isl_adjoint", "mxr_psl", "mxr_psl_adjoint", "mxr_phase_isl",
-           "mxr_phase_isl_adjoint",
+           "mxr_phase_isl_adjoint", "mxr_psl_taps_adjoint",
            "mxr_log_set_level", "mxr_log_get_level", "mxr_log_set_dest", "mxr_log_write",
            "mxr_trace_set_level", "mxr_trace_get_level", "mxr_trace_available", "mxr_trace_push",
            "mxr_trace_pop", "mxr_trace_range_start", "mxr_trace_range_end")
@@ -182,6 +182,7 @@ def lib():
         "mxr_phase_isl_adjoint": [T, T, T, I64, V],
         "mxr_psl": [T, T, T, I, I, V],
         "mxr_psl_adjoint": [T, T, T, T, I, I, V],
+        "mxr_psl_taps_adjoint": [T, T, T, T, T, T, I, I, V],
     }
     for name, argt in sigs.items():
         fn = getattr(L, name)

@@ -20,6 +20,9 @@ WeightedPowerSum: L = sum_{c,k,r} wmap[k, r] pow[c, k, r]; its backward hands
               on wmap expanded over channels (a stride-0 view, never
               materialised), which the pulse-axis adjoint reads as is.
 
+DopplerPsl:   psl = Psl(DopplerPower(x, taps, nd)); the backward takes the taps
+              gradient from the target columns (mxr_psl_taps_adjoint) when x
+              needs none.
 PhaseIsl:     isl = ISL(fft_conv(s, s, n)) of s = exp(j phi) per row, one kernel
               each way (mxr_phase_isl / mxr_phase_isl_adjoint); the same value
               and gradient as Isl(autocorr(polar(1, phi))).
@@ -205,6 +208,34 @@ class Psl(torch.autograd.Function):
         return ops.psl_adjoint(g.contiguous(), pw, targets, ctx.ml, ctx.q), None, None, None
 
 
+class DopplerPsl(torch.autograd.Function):
+    """psl = Psl(DopplerPower(x, taps, nd)) at the targets.  The forward
+    computes the whole power map; when x needs no gradient the backward takes
+    the taps gradient from the target columns alone (mxr_psl_taps_adjoint)
+    instead of a map-sized power gradient and the pulse-axis adjoint."""
+
+    @staticmethod
+    def forward(ctx, x, taps, targets, nd: int, ml: int, q: int):
+        x = x.contiguous()
+        taps = taps.contiguous()
+        pw, _ = ops.doppler_taps(x, taps, nd)
+        ctx.save_for_backward(x, taps, pw, targets)
+        ctx.nd, ctx.ml, ctx.q = nd, ml, q
+        return ops.psl(pw, targets, ml, q)
+
+    @staticmethod
+    def backward(ctx, g):
+        x, taps, pw, targets = ctx.saved_tensors
+        g = g.contiguous()
+        if ctx.needs_input_grad[0]:
+            gp = ops.psl_adjoint(g, pw, targets, ctx.ml, ctx.q)
+            gx, gt = ops.doppler_taps_adjoint(gp, x, taps, ctx.nd, want_gx=True,
+                                              want_gtaps=ctx.needs_input_grad[1])
+            return gx, gt, None, None, None, None
+        gt = ops.psl_taps_adjoint(g, pw, x, taps, targets, ctx.ml, ctx.q) if ctx.needs_input_grad[1] else None
+        return None, gt, None, None, None, None
+
+
 def fft_conv(x, w, n: int, conj: bool = True):
     return FFTConv.apply(x, w, n, conj)
 
@@ -233,9 +264,8 @@ def design_loss(phi, taps, cube, targets, n: int, nd: int, lam: float = 1.0, ml:
     the waveform term (phases -> samples -> autocorrelation -> ISL) is one
     kernel forward and one backward."""
     isl = PhaseIsl.apply(phi, n)
-    pw = doppler_power(cube, taps, nd)
     tg = targets if targets.dtype == torch.int32 else targets.to(torch.int32)
-    psl = Psl.apply(pw, tg, ml, q)
+    psl = DopplerPsl.apply(cube, taps, tg, nd, ml, q)
     return isl.mean() + lam * psl.mean()
 
 

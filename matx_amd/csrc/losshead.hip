@@ -187,6 +187,94 @@ __global__ __launch_bounds__(LH_NT) void psl_adj_kernel(PslArgs a) {
   }
 }
 
+// The taps gradient of sum_t gpsl_t PSL_t over pw = |FFT_nd(taps .* x)|^2
+// along pulses, taken straight from the target columns (the loss touches
+// nothing else): one workgroup per target forms the column's d PSL/d pw
+// (the rule of psl_adj_kernel, from the saved power map), recomputes the
+// column's windowed spectrum D_k = sum_p taps_p x_p e^(-2 pi i k p / nd),
+// forms gu_p = sum_k 2 gp_k D_k e^(2 pi i k p / nd) (the pulse-axis adjoint
+// of |.|^2 and the transform) and writes the target's share
+// Re(conj(gu_p) x_p) of d/dtaps_p; a second kernel sums the shares in target
+// order.  Equal to psl_adjoint followed by doppler_taps_adjoint, without
+// the map-sized gradient.  The column transforms are direct sums over an
+// LDS table of the nd roots of unity (P nd complex products per target).
+struct PslTapsArgs {
+  const float2 *x;
+  int64_t xs_c, xs_p, xs_r;
+  const float *taps;
+  int64_t ts;
+  int P;
+  float *part;  // [T, P] shares
+  float *gt;    // [P]
+  int64_t gts;
+};
+
+__global__ __launch_bounds__(LH_NT) void psl_taps_adj_kernel(PslArgs a, PslTapsArgs b) {
+  __shared__ double red[LH_NT / 64];
+  __shared__ float2 W[4 * LH_NT];   // e^(-2 pi i m / nd)
+  __shared__ float2 hx[4 * LH_NT];  // taps_p x_p of the column
+  __shared__ float2 D[4 * LH_NT];
+  __shared__ float gp[4 * LH_NT];
+  const int t = blockIdx.x;
+  const float *col;
+  int kd;
+  double A, M;
+  const bool ok = psl_terms(a, t, red, &col, &kd, &A, &M);
+  float *share = b.part + (int64_t)t * b.P;
+  if (!ok) {  // NaN target: no gradient (uniform over the workgroup)
+    for (int p = threadIdx.x; p < b.P; p += LH_NT) share[p] = 0.f;
+    return;
+  }
+  const int c = a.tg[(int64_t)t * a.ts_t], r = a.tg[(int64_t)t * a.ts_t + a.ts_e];
+  const float2 *xc = b.x + (int64_t)c * b.xs_c + (int64_t)r * b.xs_r;
+  const double g = (double)a.gpsl[(int64_t)t * a.gls_t];
+  const double sc = g * pow(A, 1.0 - a.q) / M;
+  const int nd = a.nd, P = b.P;
+  for (int k = threadIdx.x; k < nd; k += LH_NT) {
+    double v = 0.0;
+    if (k == kd) v = -g * A / (M * M);
+    else if (!in_mainlobe(k, kd, nd, a.ml)) v = sc * powq((double)col[(int64_t)k * a.ps_k], a.q - 1);
+    gp[k] = (float)v;
+    float sn, cs;
+    sincospif(2.0f * (float)k / (float)nd, &sn, &cs);
+    W[k] = make_float2(cs, -sn);
+  }
+  for (int p = threadIdx.x; p < P; p += LH_NT) hx[p] = cscale(xc[(int64_t)p * b.xs_p], b.taps[(int64_t)p * b.ts]);
+  __syncthreads();
+  // D_k = sum_p hx_p W^(kp)
+  for (int k = threadIdx.x; k < nd; k += LH_NT) {
+    float2 d = make_float2(0.f, 0.f);
+    int m = 0;  // k p mod nd
+    for (int p = 0; p < P; ++p) {
+      d = cadd(d, cmul(hx[p], W[m]));
+      m += k;
+      if (m >= nd) m -= nd;
+    }
+    D[k] = d;
+  }
+  __syncthreads();
+  // gu_p = sum_k 2 gp_k D_k conj(W^(kp)); the share Re(conj(gu_p) x_p)
+  for (int p = threadIdx.x; p < P; p += LH_NT) {
+    float2 gu = make_float2(0.f, 0.f);
+    int m = 0;  // k p mod nd
+    for (int k = 0; k < nd; ++k) {
+      gu = cadd(gu, cscale(cmulc(D[k], W[m]), 2.0f * gp[k]));
+      m += p;
+      if (m >= nd) m -= nd;
+    }
+    const float2 xv = xc[(int64_t)p * b.xs_p];
+    share[p] = gu.x * xv.x + gu.y * xv.y;
+  }
+}
+
+__global__ __launch_bounds__(LH_NT) void psl_taps_sum_kernel(PslTapsArgs b, int T) {
+  const int p = blockIdx.x * LH_NT + threadIdx.x;
+  if (p >= b.P) return;
+  float acc = 0.f;
+  for (int t = 0; t < T; ++t) acc += b.part[(int64_t)t * b.P + p];
+  b.gt[(int64_t)p * b.gts] = acc;
+}
+
 static mxr_status isl_args(const char *fn, const mxr_tensor *r, IslArgs *a) {
   mxr_status st;
   if ((st = check_tensor(r, "r", MXR_C64, 2, 2))) return st;
@@ -325,6 +413,59 @@ static mxr_status psl_adjoint_impl(const mxr_tensor *gp, const mxr_tensor *gpsl,
   return MXR_SUCCESS;
 }
 
+static mxr_status psl_taps_adjoint_impl(const mxr_tensor *gtaps, const mxr_tensor *gpsl,
+                                        const mxr_tensor *pw, const mxr_tensor *x,
+                                        const mxr_tensor *taps, const mxr_tensor *tg, int ml, int q,
+                                        hipStream_t s) {
+  const char *fn = "psl_taps_adjoint";
+  PslArgs a{};
+  PslTapsArgs b{};
+  mxr_status st;
+  if ((st = psl_args(fn, pw, tg, ml, q, &a))) return st;
+  if ((st = check_tensor(gpsl, "gpsl", MXR_F32, 1, 1))) return st;
+  if ((st = check_tensor(x, "x", MXR_C64, 3, 3))) return st;
+  if ((st = check_tensor(taps, "taps", MXR_F32, 1, 1))) return st;
+  if ((st = check_tensor(gtaps, "gtaps", MXR_F32, 1, 1))) return st;
+  MXR_REQUIRE(gpsl->shape[0] == a.T, MXR_INVALID_SIZE, "%s: gpsl must be [%d]", fn, a.T);
+  MXR_REQUIRE(x->shape[0] == pw->shape[0] && x->shape[2] == pw->shape[2] && x->shape[1] >= 1 &&
+                  x->shape[1] <= pw->shape[1],
+              MXR_INVALID_SIZE, "%s: x must be [C, P, R] with P <= nd for pw [C, nd, R]", fn);
+  MXR_REQUIRE(taps->shape[0] == x->shape[1] && gtaps->shape[0] == x->shape[1], MXR_INVALID_SIZE,
+              "%s: taps and gtaps must be [P]", fn);
+  for (const mxr_tensor *i : {gpsl, pw, x, taps, tg})
+    MXR_REQUIRE(!views_overlap(*gtaps, *i), MXR_ALIAS_ERROR, "%s: gtaps overlaps an input", fn);
+  b.x = (const float2 *)x->data;
+  b.xs_c = x->stride[0];
+  b.xs_p = x->stride[1];
+  b.xs_r = x->stride[2];
+  b.taps = (const float *)taps->data;
+  b.ts = taps->stride[0];
+  b.P = (int)x->shape[1];
+  b.gt = (float *)gtaps->data;
+  b.gts = gtaps->stride[0];
+  a.gpsl = (const float *)gpsl->data;
+  a.gls_t = gpsl->stride[0];
+  if (a.T == 0) {
+    ProfScope ps("psl_taps_sum_kernel", s);
+    psl_taps_sum_kernel<<<dim3((unsigned)((b.P + LH_NT - 1) / LH_NT)), dim3(LH_NT), 0, s>>>(b, 0);
+    MXR_HIP_TRY(hipGetLastError());
+    return MXR_SUCCESS;
+  }
+  b.part = (float *)stream_workspace("psl_taps_adjoint", sizeof(float) * (size_t)a.T * b.P, s);
+  if (!b.part) return workspace_failure(fn);
+  {
+    ProfScope ps("psl_taps_adj_kernel", s);
+    psl_taps_adj_kernel<<<dim3((unsigned)a.T), dim3(LH_NT), 0, s>>>(a, b);
+  }
+  MXR_HIP_TRY(hipGetLastError());
+  {
+    ProfScope ps("psl_taps_sum_kernel", s);
+    psl_taps_sum_kernel<<<dim3((unsigned)((b.P + LH_NT - 1) / LH_NT)), dim3(LH_NT), 0, s>>>(b, a.T);
+  }
+  MXR_HIP_TRY(hipGetLastError());
+  return MXR_SUCCESS;
+}
+
 }  // namespace mxr
 
 extern "C" {
@@ -347,6 +488,14 @@ mxr_status mxr_psl_adjoint(const mxr_tensor *gpow, const mxr_tensor *gpsl, const
                            const mxr_tensor *targets, int ml, int q, hipStream_t stream) {
   return mxr::guarded("mxr_psl_adjoint",
                       [&] { return mxr::psl_adjoint_impl(gpow, gpsl, pw, targets, ml, q, stream); });
+}
+
+mxr_status mxr_psl_taps_adjoint(const mxr_tensor *gtaps, const mxr_tensor *gpsl, const mxr_tensor *pw,
+                                const mxr_tensor *x, const mxr_tensor *taps, const mxr_tensor *targets,
+                                int ml, int q, hipStream_t stream) {
+  return mxr::guarded("mxr_psl_taps_adjoint", [&] {
+    return mxr::psl_taps_adjoint_impl(gtaps, gpsl, pw, x, taps, targets, ml, q, stream);
+  });
 }
 
 }  // extern "C"

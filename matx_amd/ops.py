@@ -360,6 +360,16 @@ def psl_adjoint(gpsl, pw, targets, ml: int = 2, q: int = 8, stream=None):
     return gp
 
 
+def psl_taps_adjoint(gpsl, pw, x, taps, targets, ml: int = 2, q: int = 8, stream=None):
+    """d(sum_t gpsl_t psl_t)/dtaps for pw = doppler_taps(x, taps)'s power,
+    from the target columns alone: [P] f32."""
+    gt = torch.empty_like(taps)
+    L.check(L.lib().mxr_psl_taps_adjoint(L.ref(L.desc(gt)), L.ref(L.desc(gpsl)), L.ref(L.desc(pw)),
+                                         L.ref(L.desc(x)), L.ref(L.desc(taps)), L.ref(L.desc(targets)),
+                                         int(ml), int(q), L.stream_handle(stream)))
+    return gt
+
+
 def conv1d(x, f, mode="full", method="direct", out=None, stream=None):
     """conv1d along the last axis (include/matx/operators/conv.h:244-266):
     method "direct" (mxr_conv1d) or "fft" (mxr_conv1d_fft, MatX's

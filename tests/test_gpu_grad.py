@@ -193,6 +193,44 @@ def test_phase_isl_errors(mx):
         mx.ops.phase_isl(torch.zeros((2, 2000), device="cuda"), 1024)
 
 
+# shared columns (two targets on one column), a target outside the map
+# (NaN, no gradient), fewer pulses than Doppler bins
+@pytest.mark.parametrize("C,P,R,nd,T", [(2, 32, 50, 32, 5), (3, 128, 200, 128, 24), (1, 20, 30, 64, 3)])
+def test_psl_taps_adjoint_matches_dense(mx, C, P, R, nd, T):
+    from matx_amd import autograd as ag
+    rng = np.random.default_rng(C * 100 + P + T)
+    x = crandn(rng, C, P, R)
+    h = (np.hamming(P) + 0.1 * rng.standard_normal(P)).astype(np.float32)
+    tg = np.stack([rng.integers(0, C, T), rng.integers(0, R, T), rng.integers(0, nd, T)], 1).astype(np.int32)
+    tg[-1, :2] = tg[0, :2]          # a column shared by two targets
+    if T > 3:
+        tg[1, 0] = C                # outside the map
+    g = rng.standard_normal(T).astype(np.float32)
+    xd, hd, td, gd = dev(x), dev(h, torch.float32), torch.as_tensor(tg).cuda(), dev(g, torch.float32)
+    pw, _ = mx.ops.doppler_taps(xd, hd, nd)
+    gt = host(mx.ops.psl_taps_adjoint(gd, pw, xd, hd, td, 2, 8))
+    gp = mx.ops.psl_adjoint(gd, pw, td, 2, 8)
+    _, gt_dense = mx.ops.doppler_taps_adjoint(gp, xd, hd, nd)
+    assert np.all(np.isfinite(gt))
+    assert rel(gt, host(gt_dense)) < GTOL
+    # the autograd function: sparse backward (x without gradient) and the
+    # dense fallback (x with gradient) against the unfused composition
+    for x_grad in (False, True):
+        x1 = xd.clone().requires_grad_(x_grad)
+        x2 = xd.clone().requires_grad_(x_grad)
+        h1 = hd.clone().requires_grad_()
+        h2 = hd.clone().requires_grad_()
+        valid = torch.as_tensor(tg[:, 0] < C).cuda()
+        y1 = ag.DopplerPsl.apply(x1, h1, td, nd, 2, 8)
+        y2 = ag.Psl.apply(ag.doppler_power(x2, h2, nd), td, 2, 8)
+        assert rel(host(y1[valid]), host(y2[valid])) < RTOL
+        (y1[valid] * gd[valid]).sum().backward()
+        (y2[valid] * gd[valid]).sum().backward()
+        assert rel(host(h1.grad), host(h2.grad)) < GTOL
+        if x_grad:
+            assert rel(host(x1.grad), host(x2.grad)) < GTOL
+
+
 def test_isl_head_and_adjoint(mx):
     rng = np.random.default_rng(31)
     B, n = 5, 4096
