@@ -16,6 +16,7 @@
 #include <chrono>
 #include <map>
 #include <mutex>
+#include <set>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -310,12 +311,26 @@ __device__ __forceinline__ float ham(long long i, int n) {
   if (n <= 1) return 1.0f;
   return 0.54f - 0.46f * cosf(6.28318530717958647692f * (float)i / (float)(n - 1));
 }
+// x^y for a base x that is a constant of the expression, with lx = log2(x)
+// in double formed once per kernel: exp2 of t = y lx, t split into float
+// high and low parts (exp2(th + tl) = exp2(th) (1 + tl ln 2) to second
+// order, |tl| <= 2^-17), so the result carries the rounding of one exp2f and
+// one multiply.  Bases that are not positive and finite, x = 1 and products
+// outside the normal float exponent range take powf.
+__device__ __forceinline__ float pow_cl(float x, float y, double lx) {
+  const double t = (double)y * lx;
+  if (!(x > 0.f) || x == 1.f || !__builtin_isfinite(x) || !(t > -126.0 && t < 126.0)) return powf(x, y);
+  const float th = (float)t;
+  const float tl = (float)(t - (double)th);
+  return exp2f(th) * fmaf(tl, 0.693147180559945309f, 1.0f);
+}
 )";
 
 namespace {
 struct EwjVal {
   std::string name;
   bool real;
+  int cst = -1;  // the constant it is, when it is one (a CONST operand)
 };
 
 std::string ewj_c(const EwjVal &v) { return v.real ? "mk(" + v.name + ", 0.f)" : v.name; }
@@ -369,6 +384,7 @@ std::string ewj_source(const mxr_ew_instr *prog, int nprog, const int *idt, int 
       else if (idt[i] == MXR_F32) k << "  const float " << n << " = ((const float *)" << p << ")[" << off << "];\n";
       else k << "  const float " << n << " = (float)((const int *)" << p << ")[" << off << "];\n";
     }
+  std::set<int> pow_logs;  // constants whose log2 the kernel has formed
   for (int e = 0; e < NE; ++e) {
     const std::string c = "c" + std::to_string(e);
     std::vector<EwjVal> st;
@@ -393,6 +409,7 @@ std::string ewj_source(const mxr_ew_instr *prog, int nprog, const int *idt, int 
           const std::string c = std::to_string(2 * ins.a);
           if (creal >> ins.a & 1u) def(true, "a.cst[" + c + "]");
           else def(false, "mk(a.cst[" + c + "], a.cst[" + c + " + 1])");
+          st.back().cst = ins.a;
           break;
         }
         case MXR_EW_HAMMING: def(true, "ham(" + pos(ins.a) + ", " + std::to_string(ins.b) + ")"); break;
@@ -426,7 +443,19 @@ std::string ewj_source(const mxr_ew_instr *prog, int nprog, const int *idt, int 
                 else def(false, "(" + y.name + ".y == 0.f) ? mk(" + X + ".x / " + y.name + ".x, " + X + ".y / " +
                                      y.name + ".x) : cdiv(" + X + ", " + y.name + ")");
                 break;
-              case MXR_EW_POW: def(true, "powf(" + xr + ", " + yr + ")"); break;
+              case MXR_EW_POW:
+                if (x.real && x.cst >= 0) {
+                  // log2 of the constant base once per kernel (before its
+                  // first use, in scope for every later column)
+                  const std::string lg = "plg" + std::to_string(x.cst);
+                  if (pow_logs.insert(x.cst).second)
+                    k << "  const double " << lg << " = a.cst[" << 2 * x.cst << "] > 0.f ? log2((double)a.cst["
+                      << 2 * x.cst << "]) : 0.0;\n";
+                  def(true, "pow_cl(" + xr + ", " + yr + ", " + lg + ")");
+                } else {
+                  def(true, "powf(" + xr + ", " + yr + ")");
+                }
+                break;
               case MXR_EW_GT: def(true, "(" + xr + " > " + yr + ") ? 1.f : 0.f"); break;
               case MXR_EW_LT: def(true, "(" + xr + " < " + yr + ") ? 1.f : 0.f"); break;
               case MXR_EW_GE: def(true, "(" + xr + " >= " + yr + ") ? 1.f : 0.f"); break;

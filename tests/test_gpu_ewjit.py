@@ -115,6 +115,38 @@ def mx():
     return matx_amd
 
 
+def test_pow_constant_base_accuracy(mx):
+    """pow with a constant base (the calcDets form pow(pfa, -1/norm)) takes
+    exp2(y log2(base)) with log2 in double: every element within 4 float
+    ulps of the FP64 value across exponents up to the float range, and the
+    special cases (results outside the normal range, NaN, base 1, zero and
+    negative bases) as powf gives them."""
+    from matx_amd import expr as E
+    rng = np.random.default_rng(7)
+    y = np.concatenate([-1.0 / rng.uniform(1.0, 65.0, 4000),          # the CFAR exponents
+                        rng.uniform(-10.0, 10.0, 4000),
+                        rng.uniform(-125.0 / 16.6, 125.0 / 16.6, 4000)]).astype(np.float32)
+    yd = torch.as_tensor(y).cuda()
+    for base in (1e-5, 2.5, 0.75, 3.0e4):
+        o = torch.empty_like(yd)
+        E.run(o, E.pow(float(base), yd))
+        got = o.cpu().numpy().astype(np.float64)
+        ref = np.power(np.float64(np.float32(base)), y.astype(np.float64))
+        ok = (ref > 1.2e-38) & (ref < 3.0e38)
+        ulp = np.spacing(ref[ok].astype(np.float32)).astype(np.float64)
+        assert np.max(np.abs(got[ok] - ref[ok]) / ulp) <= 4.0, base
+    sp = np.array([np.nan, 0.0, 1.0, -1.0, 200.0, -200.0, 1e30, -1e30], np.float32)
+    spd = torch.as_tensor(sp).cuda()
+    for base in (1e-5, 2.5, 1.0, 0.0, -2.0):
+        o = torch.empty_like(spd)
+        E.run(o, E.pow(float(base), spd))
+        ref = torch.pow(torch.tensor(float(base), dtype=torch.float32), torch.as_tensor(sp)).numpy()
+        got = o.cpu().numpy()
+        assert np.array_equal(np.isnan(got), np.isnan(ref)), base
+        fin = ~np.isnan(ref)
+        assert np.allclose(got[fin], ref[fin], rtol=1e-6, atol=0.0), (base, got, ref)
+
+
 def test_every_opcode_vs_fp64(mx):
     from matx_amd import _lib
     lib = _lib.lib()
