@@ -107,11 +107,15 @@ struct GenericCfg {
   static constexpr int NT = N >= 4096 ? N / 16 : 256;
 };
 
-template <int N, bool INV>
+// COL: element i of transform t at i * TB + t (batch_fast calls, where the
+// workgroup's transforms are neighbours in memory and the gather and
+// scatter run lanes across them: consecutive lanes touch consecutive LDS
+// words, where the transform-major layout puts them a padded row apart).
+template <int N, bool INV, bool COL>
 __global__ __launch_bounds__(GenericCfg<N>::NT) void fft_generic_kernel(FftArgs a) {
   constexpr int TB = GenericCfg<N>::TB;
   constexpr int NT = GenericCfg<N>::NT;
-  using LY = Layout<N, TB, false>;
+  using LY = Layout<N, TB, COL>;
   using SQ = Seq<N, false>;
   __shared__ float2 lds[LY::LDS_ELEMS];
   __shared__ int64_t base_in[TB], base_out[TB];
@@ -186,8 +190,16 @@ static mxr_status launch_generic(const FftArgs &a, bool inv, hipStream_t s) {
   if (grid == 0) return MXR_SUCCESS;
   MXR_REQUIRE(grid <= 0x7fffffff, MXR_INVALID_SIZE, "fft: batch too large");
   ProfScope ps("fft_generic_kernel", s);
-  if (inv) fft_generic_kernel<N, true><<<dim3((unsigned)grid), dim3(C::NT), 0, s>>>(a);
-  else fft_generic_kernel<N, false><<<dim3((unsigned)grid), dim3(C::NT), 0, s>>>(a);
+  if constexpr (C::TB > 1) {
+    if (a.batch_fast) {
+      if (inv) fft_generic_kernel<N, true, true><<<dim3((unsigned)grid), dim3(C::NT), 0, s>>>(a);
+      else fft_generic_kernel<N, false, true><<<dim3((unsigned)grid), dim3(C::NT), 0, s>>>(a);
+      MXR_HIP_TRY(hipGetLastError());
+      return MXR_SUCCESS;
+    }
+  }
+  if (inv) fft_generic_kernel<N, true, false><<<dim3((unsigned)grid), dim3(C::NT), 0, s>>>(a);
+  else fft_generic_kernel<N, false, false><<<dim3((unsigned)grid), dim3(C::NT), 0, s>>>(a);
   MXR_HIP_TRY(hipGetLastError());
   return MXR_SUCCESS;
 }
