@@ -236,6 +236,57 @@ class DopplerPsl(torch.autograd.Function):
         return None, gt, None, None, None, None
 
 
+_LOSS_WEIGHTS = {}
+
+
+def _loss_weights(B: int, T: int, lam: float, device) -> torch.Tensor:
+    """[B + T] f32: 1/B for the ISL entries, lam/T for the PSL entries (built
+    once per shape and weight, so a captured step allocates nothing)."""
+    key = (B, T, float(lam), str(device))
+    w = _LOSS_WEIGHTS.get(key)
+    if w is None:
+        w = torch.cat([torch.full((B,), 1.0 / B), torch.full((T,), float(lam) / T)]).to(
+            device=device, dtype=torch.float32)
+        _LOSS_WEIGHTS[key] = w
+    return w
+
+
+class DesignLoss(torch.autograd.Function):
+    """mean ISL + lam * mean PSL (design_loss) with the per-row heads written
+    into one [B + T] vector whose weighted sum is the loss: one dot product
+    forward and one scale backward around the two heads and their adjoints
+    (PhaseIsl's and DopplerPsl's kernels), instead of the means, their
+    weighting and their backward as separate elementwise launches."""
+
+    @staticmethod
+    def forward(ctx, phi, taps, cube, targets, n: int, nd: int, lam: float, ml: int, q: int):
+        phi, taps, cube = phi.contiguous(), taps.contiguous(), cube.contiguous()
+        B, T = phi.shape[0], targets.shape[0]
+        v = torch.empty((B + T,), dtype=torch.float32, device=phi.device)
+        ops.phase_isl(phi, n, out=v[:B])
+        pw, _ = ops.doppler_taps(cube, taps, nd)
+        ops.psl(pw, targets, ml, q, out=v[B:])
+        w = _loss_weights(B, T, lam, phi.device)
+        ctx.save_for_backward(phi, taps, cube, pw, targets, w)
+        ctx.n, ctx.nd, ctx.ml, ctx.q, ctx.B = n, nd, ml, q, B
+        return torch.dot(v, w)
+
+    @staticmethod
+    def backward(ctx, g):
+        phi, taps, cube, pw, targets, w = ctx.saved_tensors
+        B = ctx.B
+        gv = w * g
+        gphi = ops.phase_isl_adjoint(gv[:B], phi, ctx.n) if ctx.needs_input_grad[0] else None
+        gt = gx = None
+        if ctx.needs_input_grad[2]:
+            gp = ops.psl_adjoint(gv[B:], pw, targets, ctx.ml, ctx.q)
+            gx, gt = ops.doppler_taps_adjoint(gp, cube, taps, ctx.nd, want_gx=True,
+                                              want_gtaps=ctx.needs_input_grad[1])
+        elif ctx.needs_input_grad[1]:
+            gt = ops.psl_taps_adjoint(gv[B:], pw, cube, taps, targets, ctx.ml, ctx.q)
+        return gphi, gt, gx, None, None, None, None, None, None
+
+
 def fft_conv(x, w, n: int, conj: bool = True):
     return FFTConv.apply(x, w, n, conj)
 
@@ -260,13 +311,12 @@ def design_loss(phi, taps, cube, targets, n: int, nd: int, lam: float = 1.0, ml:
     """phi: [B, Lw] phases; taps: [P] Doppler window; cube: [C, P, R] c64;
     targets: [T, 3] int (channel, range bin, Doppler bin; int32 avoids a
     conversion per call).  Returns mean ISL + lam * mean PSL (the
-    definitions of oracle/grad.py); both heads are single HIP kernels, and
-    the waveform term (phases -> samples -> autocorrelation -> ISL) is one
-    kernel forward and one backward."""
-    isl = PhaseIsl.apply(phi, n)
+    definitions of oracle/grad.py); both heads are single HIP kernels, the
+    waveform term (phases -> samples -> autocorrelation -> ISL) is one
+    kernel forward and one backward, and the two means are one weighted sum
+    (DesignLoss)."""
     tg = targets if targets.dtype == torch.int32 else targets.to(torch.int32)
-    psl = DopplerPsl.apply(cube, taps, tg, nd, ml, q)
-    return isl.mean() + lam * psl.mean()
+    return DesignLoss.apply(phi, taps, cube, tg, n, nd, lam, ml, q)
 
 
 def make_design_problem(seed: int, B: int = 64, Lw: int = 1024, C: int = 16, P: int = 128,

@@ -320,10 +320,11 @@ def isl(r, stream=None):
     return out
 
 
-def phase_isl(phi, n: int, stream=None):
+def phase_isl(phi, n: int, out=None, stream=None):
     """ISL of the n-point autocorrelation of exp(j phi) per row (phi: [B, L]
     f32), one kernel: equal to isl(fft_conv(s, s, n)) with s = polar(1, phi)."""
-    out = torch.empty((phi.shape[0],), dtype=torch.float32, device=phi.device)
+    if out is None:
+        out = torch.empty((phi.shape[0],), dtype=torch.float32, device=phi.device)
     L.check(L.lib().mxr_phase_isl(L.ref(L.desc(out)), L.ref(L.desc(phi)), int(n),
                                   L.stream_handle(stream)))
     return out
@@ -343,10 +344,11 @@ def isl_adjoint(gisl, r, stream=None):
     return gr
 
 
-def psl(pw, targets, ml: int = 2, q: int = 8, stream=None):
+def psl(pw, targets, ml: int = 2, q: int = 8, out=None, stream=None):
     """q-norm peak-sidelobe ratio of the Doppler columns pw[c, :, r] at
     targets ([T, 3] int32: channel, range bin, Doppler bin): [T] f32."""
-    out = torch.empty((targets.shape[0],), dtype=torch.float32, device=pw.device)
+    if out is None:
+        out = torch.empty((targets.shape[0],), dtype=torch.float32, device=pw.device)
     L.check(L.lib().mxr_psl(L.ref(L.desc(out)), L.ref(L.desc(pw)), L.ref(L.desc(targets)),
                             int(ml), int(q), L.stream_handle(stream)))
     return out
