@@ -47,7 +47,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=1440)
     ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
