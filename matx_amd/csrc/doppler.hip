@@ -122,12 +122,15 @@ __global__ __launch_bounds__(DopCfg<ND>::NT) void doppler_kernel(DopArgs a, int 
 // size (jobs[j].item0 are prefix sums of C*tiles); item_job[it] is the job
 // of item it (a table built with the plan).  PERSIST: a resident grid loops
 // over the items; otherwise one workgroup per item.
-// Batch tiles stay 32 range bins wide up to 256 Doppler bins: the per-item
-// decode (job lookup, record copy) is paid per tile, and at 256 the wider
-// tile measured faster in the batch although the single call prefers 16.
+// Batch tiles hold 32 KiB of pulses where that fits: 128 range bins at 16
+// and 32 Doppler bins, 64 at 64, 32 at 128 and 256 (the per-item decode, a
+// job lookup and a record copy, is paid per tile, and wider rows read whole
+// 512- and 1024-byte runs per pulse).  At 256 the 32-wide tile measured
+// faster in the batch although the single call prefers 16; 64 at 128 (a
+// 64 KiB tile) measured slower.
 template <int ND>
 struct DopBatchCfg {
-  static constexpr int TR = ND <= 256 ? 32 : DopCfg<ND>::TR;
+  static constexpr int TR = ND <= 32 ? 128 : (ND <= 64 ? 64 : (ND <= 256 ? 32 : DopCfg<ND>::TR));
   // 512 threads once the first radix-16 pass has 512 butterflies (256 x 32)
   static constexpr int NT = ND >= 256 ? 512 : 256;
 };
