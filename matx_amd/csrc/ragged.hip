@@ -217,6 +217,8 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
                 "%s: job %d needs C, P, S >= 1 and 1 <= L <= S", fn, i);
     MXR_REQUIRE(next_pow2(j.S) <= 16384 && next_pow2(j.P) <= 1024 && j.C <= 65535,
                 MXR_NOT_SUPPORTED, "%s: job %d exceeds S <= 16384, P <= 1024", fn, i);
+    MXR_REQUIRE(j.P >= 2, MXR_NOT_SUPPORTED,
+                "%s: job %d has one pulse; the Doppler transform needs P >= 2", fn, i);
     MXR_REQUIRE(j.x_ps >= j.S && j.x_cs >= j.x_ps * j.P && j.pw_ks >= j.S - j.L + 1 &&
                     j.pw_cs >= j.pw_ks * next_pow2(j.P),
                 MXR_INVALID_PARAMETER, "%s: job %d strides overlap rows", fn, i);

@@ -187,6 +187,9 @@ def test_ragged_plan_checks(L):
     assert r == 2
     r, _ = status(L, lib.mxr_ragged_plan_create, ctypes.byref(h), jobs, 1, 8, None)
     assert r == 2                                 # unknown flag
+    j.x_ps, j.P, j.pw_cs = 1000, 1, 1000          # one pulse: no Doppler transform
+    r, msg = status(L, lib.mxr_ragged_plan_create, ctypes.byref(h), jobs, 1, 0, None)
+    assert r == 1 and "P >= 2" in msg
 
 
 def test_header_matches_python_constants(L):
