@@ -390,7 +390,8 @@ mxr_status mxr_psl_taps_adjoint(const mxr_tensor *gtaps, const mxr_tensor *gpsl,
  *   pw = |FFT_nd(hamming_P .* y)|^2 along pulses, nd = next_pow2(P)
  *        ([C, nd, R] f32, unit range stride), as mxr_doppler(pw, NULL, y,
  *        MXR_TPC_NONE, P, nd).
- * Strides are in elements.  S <= 16384, 2 <= P <= 1024. */
+ * Strides are in elements.  513 <= S <= 16384 (block sizes 1024 to 16384),
+ * 2 <= P <= 1024. */
 typedef struct mxr_ragged_job {
   const void *x;
   int64_t C, P, S;

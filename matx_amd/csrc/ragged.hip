@@ -219,6 +219,9 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
                 MXR_NOT_SUPPORTED, "%s: job %d exceeds S <= 16384, P <= 1024", fn, i);
     MXR_REQUIRE(j.P >= 2, MXR_NOT_SUPPORTED,
                 "%s: job %d has one pulse; the Doppler transform needs P >= 2", fn, i);
+    MXR_REQUIRE(next_pow2(j.S) >= 1024, MXR_NOT_SUPPORTED,
+                "%s: job %d has S = %lld; the batch's block sizes start at 1024, so S >= 513", fn, i,
+                (long long)j.S);
     MXR_REQUIRE(j.x_ps >= j.S && j.x_cs >= j.x_ps * j.P && j.pw_ks >= j.S - j.L + 1 &&
                     j.pw_cs >= j.pw_ks * next_pow2(j.P),
                 MXR_INVALID_PARAMETER, "%s: job %d strides overlap rows", fn, i);
@@ -238,6 +241,12 @@ static mxr_status ragged_create(mxr_ragged_plan *out, const mxr_ragged_job *jobs
     if (flags & MXR_RAGGED_AUTOTUNE) st = autotune_block(next_pow2(j.L), n, s, &Mreq);
     if (!st && (pc_plan_blocks(j.S, j.L, n, R, Mreq, &M, &V, &nblk) != MXR_SUCCESS))
       st = pc_plan_blocks(j.S, j.L, n, R, 0, &M, &V, &nblk);  // tuned size does not fit this job
+    // the batch kernels take blocks of 1024 .. 16384 (n >= 1024 here)
+    if (!st && M < 1024) st = pc_plan_blocks(j.S, j.L, n, R, 1024, &M, &V, &nblk);
+    if (!st && (M < 1024 || M > 16384)) {
+      set_error(MXR_NOT_SUPPORTED, "%s: job %d: block size %d outside [1024, 16384]", fn, i, M);
+      st = MXR_NOT_SUPPORTED;
+    }
     p->M[i] = M;
   }
   // scratch: compressed cubes (rows padded to 64 lags) and spectra

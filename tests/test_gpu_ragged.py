@@ -80,6 +80,36 @@ def test_ragged_against_oracle(mx, orc):
         assert rel(out[i].cpu().numpy(), orc.abs2(D)) < RTOL
 
 
+@pytest.mark.parametrize("autotune", [False, True])
+def test_ragged_edge_shapes(mx, orc, autotune):
+    """Uneven corners of the job contract against the FP64 oracle: pulse
+    counts that are not powers of two (P = 2, 3, 5, 17, 100, so nd = 2, 4,
+    8, 32, 128 with zero rows), a single kept lag (S = L), a one-sample
+    waveform, the shortest rows the batch takes (S = 513 .. 1024, one
+    1024-point block) and a transform at 8192."""
+    rng = np.random.default_rng(12)
+    shapes = [(1, 2, 600, 600), (2, 3, 700, 1), (1, 5, 1023, 512), (1, 17, 2000, 7),
+              (3, 100, 4097, 2048)]
+    jobs = []
+    for C, P, S, L in shapes:
+        x = (rng.standard_normal((C, P, S)) + 1j * rng.standard_normal((C, P, S))).astype(np.complex64)
+        w = (rng.standard_normal(L) + 1j * rng.standard_normal(L)).astype(np.complex64)
+        jobs.append((torch.from_numpy(x).cuda(), torch.from_numpy(w).cuda()))
+    rb = mx.ragged.RaggedBatch(jobs, autotune=autotune)
+    out = rb.execute()
+    torch.cuda.synchronize()
+    for i, ((x, w), (C, P, S, L)) in enumerate(zip(jobs, shapes)):
+        nd = mx.ragged.next_pow2(P)
+        assert tuple(out[i].shape) == (C, nd, S - L + 1)
+        xh, wh = x.cpu().numpy(), w.cpu().numpy()
+        n = mx.ragged.next_pow2(S)
+        Wc = orc.waveform_spectrum(wh, n, True, True)
+        y = orc.pulse_compress(xh.reshape(C * P, S), Wc, S - L + 1).reshape(C, P, S - L + 1)
+        ref = orc.abs2(orc.doppler(y, P, nd))
+        assert rel(out[i].cpu().numpy(), ref) < RTOL, (i, shapes[i], rb.block_size(i))
+    rb.close()
+
+
 def test_ragged_graph_replays(mx):
     jobs = make_jobs(3, 6)
     s = torch.cuda.Stream()

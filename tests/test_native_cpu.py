@@ -190,6 +190,9 @@ def test_ragged_plan_checks(L):
     j.x_ps, j.P, j.pw_cs = 1000, 1, 1000          # one pulse: no Doppler transform
     r, msg = status(L, lib.mxr_ragged_plan_create, ctypes.byref(h), jobs, 1, 0, None)
     assert r == 1 and "P >= 2" in msg
+    j.P, j.pw_cs, j.S, j.x_ps, j.x_cs, j.pw_ks = 2, 2 * 100, 100, 100, 200, 100
+    r, msg = status(L, lib.mxr_ragged_plan_create, ctypes.byref(h), jobs, 1, 0, None)
+    assert r == 1 and "S >= 513" in msg           # rows shorter than every block size
 
 
 def test_header_matches_python_constants(L):
