@@ -4,8 +4,10 @@ Each job is (x [C, P, S] c64, w [L] c64): the plan computes
 pw = |FFT_nd(hamming_P .* pulse_compress(x, w))|^2 along pulses,
 nd = next_pow2(P), into pw [C, nd, S-L+1] f32 -- the result of
 ops.pulse_compress(x, w, next_pow2(S)) then ops.doppler(y, nd, tpc="none",
-win_rows=P) -- for every job in a few launches.  `graph()` captures one
-execution into a torch CUDA (HIP) graph to replay.
+win_rows=P) -- for every job in a few launches.  Jobs need 513 <= S <=
+16384 (the batch's overlap-save blocks hold 1024 to 16384 samples) and
+2 <= P <= 1024; the plan raises MatxError (NOT_SUPPORTED) otherwise.
+`graph()` captures one execution into a torch CUDA (HIP) graph to replay.
 """
 from __future__ import annotations
 
